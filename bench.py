@@ -1,0 +1,210 @@
+"""Benchmark: pressure solves/s to 1e-8 relative residual on one MI355X.
+
+Workload (one step = one pressure solve from x0 = 0): the 128^3 jittered hex box
+(2,097,152 cells, interior vertices moved by +-20% of the spacing), pressure Poisson system with
+coefficients |S_f| / |C_N - C_P|, zero-gradient walls, pressure pinned at cell 0, seeded
+zero-mean right-hand side; smoothed-aggregation AMG (threshold 0.08 halved per level) built once
+on the host; solved to |b - A x| <= 1e-8 |b| with
+  * value: mixed precision - FP32 Jacobi V-cycle inside FP64 flexible CG (the metric's "mixed"),
+  * beside it: the plain FP64 AMG-PCG (the reference algorithm) and the FP64 Jacobi-BiCGStab
+    momentum system of the same mesh.
+Inputs are resident in HBM before the timed region. The CPU baseline is the reference's own host
+AMG-PCG (solve_amg_cg_host, compiled from the reference sources into oracle/_ref) on the same
+hierarchy and right-hand side, timed on this machine's host cores for a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--skip-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "pressure solves/s @1e-8 rel-res, mixed vs FP64 AMG-PCG; HBM GB/s; p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(hier_view, b, tol, budget_s):
+    """Times the reference's solve_amg_cg_host (oracle/_ref) or, where that library was not
+    built, the oracle's restatement, on the host cores for about budget_s seconds."""
+    from oracle import bind as ob
+
+    levels = hier_view.levels
+    a_l = [ob.Csr(lv.a.n, lv.a.n, lv.a.row_ptr, lv.a.col, lv.a.val) for lv in levels]
+    p_l = [None if lv.p is None else ob.Csr(lv.p.n, lv.p.n_cols, lv.p.row_ptr, lv.p.col, lv.p.val)
+           for lv in levels]
+    r_l = [None if lv.r is None else ob.Csr(lv.r.n, lv.r.n_cols, lv.r.row_ptr, lv.r.col, lv.r.val)
+           for lv in levels]
+    ci = hier_view.coarse_inverse
+    cores = os.cpu_count() or 1
+    ob.set_threads(cores)
+    kind = "reference" if ob.ref_available() else "port"
+    times, iters = [], []
+    t_start = time.perf_counter()
+    while True:
+        x = np.zeros(b.shape[0])
+        t0 = time.perf_counter()
+        if kind == "reference":
+            st = ob.ref_solve_amg_cg_host(a_l, p_l, r_l, ci, b, x, tol, 500)
+        else:
+            arrs = ob._LevelArrays(a_l, p_l, r_l)
+            cinv = np.ascontiguousarray(ci, np.float64).reshape(-1)
+            res = ob.orc().orc_amg_cg(len(a_l), ob.ip(arrs.n), *arrs.a, *arrs.p, *arrs.r,
+                                      ob.dp(cinv), 2, 0.7, ob.dp(b), ob.dp(x), tol, 500)
+            st = ob.Stats(res.iterations, bool(res.converged), res.residual)
+        times.append(time.perf_counter() - t0)
+        iters.append(st.iterations)
+        if time.perf_counter() - t_start > budget_s:
+            break
+    med = float(np.median(times))
+    return {"value": 1.0 / med, "unit": "solves/s", "cores": cores, "kind": kind,
+            "sample": f"{len(times)} FP64 Jacobi-V-cycle AMG-PCG solves of the same 128^3 system "
+                      f"(same hierarchy, x0 = 0, 1e-8), median {med * 1e3:.1f} ms, "
+                      f"{int(np.median(iters))} iterations; reference threads loops with n > 20000"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(batched and served systems shard across GPUs by system)"}))
+        return 2
+
+    import cfd_amd
+    from cfd_amd import AmgHierarchy, AmgSettings, DeviceSolver, Method, Precision, SolveControl, cases
+
+    if not cfd_amd.device_available():
+        raise RuntimeError("no HIP device visible")
+    log(cfd_amd.device_description())
+    t0 = time.perf_counter()
+    mesh, p_sys, m_sys = cases.config2(args.size)
+    pat = cases.csr_pattern(p_sys.n, p_sys.owner, p_sys.neighbour)
+    a = cases.ldu_to_csr(pat, p_sys.diag, p_sys.upper, p_sys.lower)
+    log(f"generated {p_sys.n} cells / {p_sys.n_faces} faces in {time.perf_counter() - t0:.1f} s")
+    hier = AmgHierarchy(a, AmgSettings(strength_decay=0.5))
+    log(f"AMG setup {hier.setup_ms:.0f} ms, levels {hier.level_sizes()}, "
+        f"operator complexity {hier.operator_complexity():.3f}")
+    view = hier.view()
+    dev = DeviceSolver(pat)
+    dev.assemble(p_sys.diag, p_sys.upper, p_sys.lower)
+    dev.set_amg(view)
+    b = p_sys.source
+    dev.set_rhs(b)
+    bnorm = float(np.linalg.norm(b))
+    tol = 1e-8 * bnorm
+    ctl = SolveControl(tol, 500)
+
+    def run(prec, k):
+        lat, dev_ms, its = [], [], []
+        for _ in range(k):
+            t = time.perf_counter()
+            dev.set_x(None)
+            st = dev.solve_resident(Method.Cg, True, prec, ctl)
+            lat.append(time.perf_counter() - t)
+            dev_ms.append(st.solve_ms)
+            its.append(st.iterations)
+            if not st.converged:
+                raise RuntimeError(f"{prec.name} solve did not converge: {st}")
+        return np.array(lat), np.array(dev_ms), its, st
+
+    run(Precision.MIXED_FP32, args.warmup)
+    run(Precision.FP64, max(1, args.warmup // 2))
+    t_mixed = time.perf_counter()
+    lat_m, dms_m, its_m, st_m = run(Precision.MIXED_FP32, args.steps)
+    t_mixed = time.perf_counter() - t_mixed
+    x_mixed = dev.get_x()
+    t_fp64 = time.perf_counter()
+    lat_d, dms_d, its_d, st_d = run(Precision.FP64, args.steps)
+    t_fp64 = time.perf_counter() - t_fp64
+    x_fp64 = dev.get_x()
+    log(f"mixed: {args.steps / t_mixed:.2f} solves/s, it {its_m[-1]}, true rel res "
+        f"{st_m.true_residual / bnorm:.2e}; fp64: {args.steps / t_fp64:.2f} solves/s, it {its_d[-1]}, "
+        f"true rel res {st_d.true_residual / bnorm:.2e}")
+
+    # Momentum system of the same mesh: Jacobi-BiCGStab in FP64 (config 2b).
+    mdev = DeviceSolver(pat)
+    mdev.assemble(m_sys.diag, m_sys.upper, m_sys.lower)
+    mdev.set_rhs(m_sys.source)
+    mtol = 1e-8 * float(np.linalg.norm(m_sys.source))
+    mdev.set_x(None)
+    mdev.solve_resident(Method.BiCgStab, False, Precision.FP64, SolveControl(mtol, 2000))
+    t_mom = time.perf_counter()
+    k_mom = max(1, args.steps // 2)
+    for _ in range(k_mom):
+        mdev.set_x(None)
+        st_mom = mdev.solve_resident(Method.BiCgStab, False, Precision.FP64, SolveControl(mtol, 2000))
+    t_mom = time.perf_counter() - t_mom
+
+    err = float(np.linalg.norm(x_mixed - x_fp64) / np.linalg.norm(x_fp64))
+    value = args.steps / t_mixed
+    fp64_value = args.steps / t_fp64
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_mixed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"pressure Poisson, {args.size}^3 jittered hex box ({p_sys.n} cells), "
+                        "SA-AMG-PCG to 1e-8 rel. residual from x0 = 0",
+            "cells": p_sys.n,
+            "nnz": int(a.nnz),
+            "amg_levels": hier.level_sizes(),
+            "operator_complexity": round(hier.operator_complexity(), 4),
+            "strength_threshold": "0.08 * 0.5^level",
+            "vcycle": "Jacobi w=0.7, 2+2 sweeps, dense FP64 coarsest inverse",
+        },
+        "iterations_mixed": int(its_m[-1]),
+        "iterations_fp64": int(its_d[-1]),
+        "fp64_solves_per_s": round(fp64_value, 3),
+        "speedup_vs_fp64": round(value / fp64_value, 3),
+        "p50_ms": round(1e3 * float(np.percentile(lat_m, 50)), 3),
+        "p99_ms": round(1e3 * float(np.percentile(lat_m, 99)), 3),
+        "fp64_p99_ms": round(1e3 * float(np.percentile(lat_d, 99)), 3),
+        "device_ms_mixed": round(float(np.median(dms_m)), 3),
+        "device_ms_fp64": round(float(np.median(dms_d)), 3),
+        "true_rel_residual_mixed": st_m.true_residual / bnorm,
+        "true_rel_residual_fp64": st_d.true_residual / bnorm,
+        "mixed_vs_fp64_rel_error": err,
+        "momentum_bicgstab_fp64": {"solves_per_s": round(k_mom / t_mom, 3),
+                                   "iterations": st_mom.iterations,
+                                   "true_rel_residual": st_mom.true_residual / np.linalg.norm(m_sys.source)},
+        "setup_ms": round(hier.setup_ms, 1),
+        "footprint_bytes": dev.footprint(),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    if not args.skip_cpu:
+        out["cpu_baseline"] = cpu_baseline(view, b, tol, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,21 @@
+"""MI355X (gfx950) linear-solver backend for the Yipexnn/CFD finite-volume solver.
+
+The compute path is libcfdgpu.so (C++ host code + HIP kernels, C-ABI in include/cfdgpu.h);
+this package is its Python mirror of the reference's cfd::gpu interface. Importing it loads
+the library and fails loudly if it has not been built (python -m cfd_amd.build).
+"""
+from . import _lib
+
+_lib.lib()  # fail at import time when the native library is missing
+
+from .amg import AmgHierarchy, AmgSettings  # noqa: E402
+from .krylov import (AmgLevelView, AmgView, CsrView, DeviceSolver, LduPattern, Method,  # noqa: E402
+                     Precision, SolveControl, SolveStats, assemble_convection_diffusion,
+                     backend_name, device_available, device_description, solve, solve_amg_cg)
+
+__all__ = [
+    "AmgHierarchy", "AmgSettings", "AmgLevelView", "AmgView", "CsrView", "DeviceSolver",
+    "LduPattern", "Method", "Precision", "SolveControl", "SolveStats",
+    "assemble_convection_diffusion", "backend_name", "device_available", "device_description",
+    "solve", "solve_amg_cg",
+]

@@ -1,0 +1,47 @@
+"""Builds libcfdgpu.so for gfx950 in-tree (cfd_amd/lib/) and, for tests and the CPU baseline,
+the oracle under oracle/ (plus oracle/_ref from the reference checkout when it is present).
+
+    python -m cfd_amd.build            # library + oracle
+    python -m cfd_amd.build --lib-only
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _make(directory: Path, *targets: str) -> None:
+    jobs = str(min(16, os.cpu_count() or 4))
+    cmd = ["make", "-C", str(directory), f"-j{jobs}", *targets]
+    subprocess.run(cmd, check=True)
+
+
+def build_library() -> Path:
+    _make(ROOT / "cfd_amd" / "csrc")
+    out = ROOT / "cfd_amd" / "lib" / "libcfdgpu.so"
+    if not out.exists():
+        raise RuntimeError("libcfdgpu.so was not produced")
+    return out
+
+
+def build_oracle() -> None:
+    _make(ROOT / "oracle")
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib-only", action="store_true")
+    args = ap.parse_args(argv)
+    build_library()
+    if not args.lib_only:
+        build_oracle()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

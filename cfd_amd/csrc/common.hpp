@@ -1,0 +1,85 @@
+// Shared host-side helpers of libcfdgpu: error capture at the C-ABI and small CSR containers.
+#pragma once
+
+#include <cfdgpu.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace cfdg {
+
+// Exceptions used inside the library; the C-ABI maps them to cfdg_status codes.
+struct InvalidArgument : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NoDevice : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct OutOfMemory : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct CapacityError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void set_last_error(const std::string& msg);
+const std::string& last_error();
+
+// Runs f and converts any exception into a status plus the per-thread message.
+template <class F>
+cfdg_status guarded(F&& f) noexcept {
+    try {
+        f();
+        return CFDG_OK;
+    } catch (const InvalidArgument& e) {
+        set_last_error(e.what());
+        return CFDG_ERR_INVALID_ARGUMENT;
+    } catch (const NoDevice& e) {
+        set_last_error(e.what());
+        return CFDG_ERR_NO_DEVICE;
+    } catch (const HipError& e) {
+        set_last_error(e.what());
+        return CFDG_ERR_HIP;
+    } catch (const OutOfMemory& e) {
+        set_last_error(e.what());
+        return CFDG_ERR_OUT_OF_MEMORY;
+    } catch (const CapacityError& e) {
+        set_last_error(e.what());
+        return CFDG_ERR_CAPACITY;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return CFDG_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return CFDG_ERR_INTERNAL;
+    } catch (...) {
+        set_last_error("unknown error");
+        return CFDG_ERR_INTERNAL;
+    }
+}
+
+inline void require(bool ok, const char* what) {
+    if (!ok) {
+        throw InvalidArgument(what);
+    }
+}
+
+// Owned CSR matrix (row-major, sorted columns).
+struct Csr {
+    int n_rows = 0;
+    int n_cols = 0;
+    std::vector<int> rp{0};
+    std::vector<int> ci;
+    std::vector<double> va;
+    [[nodiscard]] int nnz() const { return static_cast<int>(ci.size()); }
+};
+
+// Validates a borrowed CSR view and copies it.
+Csr copy_csr(const cfdg_csr& v, const char* what);
+
+} // namespace cfdg

@@ -1,0 +1,73 @@
+#include "ell.hpp"
+
+#include <algorithm>
+#include <limits>
+
+namespace cfdg {
+
+HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci) {
+    HostEll e;
+    e.n_rows = n_rows;
+    e.n_cols = n_cols;
+    e.n_slices = (n_rows + k_wave - 1) / k_wave;
+    e.sp.assign(static_cast<std::size_t>(e.n_slices) + 1, 0);
+    long long total = 0;
+    for (int s = 0; s < e.n_slices; ++s) {
+        int w = 0;
+        const int r_end = std::min(n_rows, (s + 1) * k_wave);
+        for (int r = s * k_wave; r < r_end; ++r) {
+            w = std::max(w, rp[r + 1] - rp[r]);
+        }
+        total += static_cast<long long>(w) * k_wave;
+        if (total > std::numeric_limits<int>::max()) {
+            throw InvalidArgument("matrix too large for 32-bit sliced-ELL offsets");
+        }
+        e.sp[static_cast<std::size_t>(s) + 1] = static_cast<int>(total);
+    }
+    e.col.resize(static_cast<std::size_t>(total));
+    e.src.resize(static_cast<std::size_t>(total));
+#pragma omp parallel for schedule(static) if (e.n_slices > 256)
+    for (int s = 0; s < e.n_slices; ++s) {
+        const int base = e.sp[static_cast<std::size_t>(s)];
+        const int w = (e.sp[static_cast<std::size_t>(s) + 1] - base) / k_wave;
+        for (int lane = 0; lane < k_wave; ++lane) {
+            const int r = s * k_wave + lane;
+            const int len = r < n_rows ? rp[r + 1] - rp[r] : 0;
+            int pad = 0;
+            if (r < n_rows) {
+                pad = (n_rows == n_cols) ? r : (len > 0 ? ci[rp[r] + len - 1] : 0);
+            }
+            for (int k = 0; k < w; ++k) {
+                const std::size_t slot = static_cast<std::size_t>(base) + static_cast<std::size_t>(k) * k_wave + lane;
+                if (k < len) {
+                    e.col[slot] = ci[rp[r] + k];
+                    e.src[slot] = rp[r] + k;
+                } else {
+                    e.col[slot] = pad;
+                    e.src[slot] = -1;
+                }
+            }
+        }
+    }
+    return e;
+}
+
+void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hipStream_t s) {
+    n_rows = e.n_rows;
+    n_cols = e.n_cols;
+    n_slices = e.n_slices;
+    sp.upload(e.sp.data(), e.sp.size(), s);
+    col.upload(e.col.data(), e.col.size(), s);
+    if (va != nullptr) {
+        if (fp64) {
+            const std::vector<double> v = ell_values<double>(e, va);
+            v64.upload(v.data(), v.size(), s);
+        }
+        if (fp32) {
+            const std::vector<float> v = ell_values<float>(e, va);
+            v32.upload(v.data(), v.size(), s);
+        }
+    }
+}
+
+} // namespace cfdg

@@ -1,0 +1,53 @@
+// Host-side conversion of CSR matrices to the sliced-ELL layout of the device kernels.
+#pragma once
+
+#include "hip_util.hpp"
+#include "layout.hpp"
+
+#include <vector>
+
+namespace cfdg {
+
+struct HostEll {
+    int n_rows = 0;
+    int n_cols = 0;
+    int n_slices = 0;
+    std::vector<int> sp;  // n_slices + 1
+    std::vector<int> col; // sp[n_slices]
+    std::vector<int> src; // CSR position per slot, -1 for padding
+};
+
+// Slices of 64 rows; slot (64 k + lane) of slice s holds entry k of row 64 s + lane in CSR
+// order, or padding (value 0) pointing at a column that the row touches anyway: the row itself
+// for square matrices, else the row's last column (0 for rows past the end).
+HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci);
+
+template <class T>
+std::vector<T> ell_values(const HostEll& e, const double* va) {
+    std::vector<T> v(e.col.size());
+    for (std::size_t k = 0; k < v.size(); ++k) {
+        const int s = e.src[k];
+        v[k] = s >= 0 ? static_cast<T>(va[s]) : T(0);
+    }
+    return v;
+}
+
+// Device copy of a sliced-ELL matrix with FP64 and / or FP32 values.
+struct DevEll {
+    int n_rows = 0;
+    int n_cols = 0;
+    int n_slices = 0;
+    DevBuf<int> sp;
+    DevBuf<int> col;
+    DevBuf<double> v64;
+    DevBuf<float> v32;
+
+    void upload(const HostEll& e, const double* va, bool fp64, bool fp32, hipStream_t s);
+    [[nodiscard]] EllView view64() const { return {n_rows, n_slices, sp.get(), col.get(), v64.get()}; }
+    [[nodiscard]] EllView view32() const { return {n_rows, n_slices, sp.get(), col.get(), v32.get()}; }
+    [[nodiscard]] std::size_t bytes() const {
+        return sp.bytes() + col.bytes() + v64.bytes() + v32.bytes();
+    }
+};
+
+} // namespace cfdg

@@ -1,0 +1,167 @@
+// HIP runtime helpers: error checks that throw, owning device buffers, pinned host buffers.
+#pragma once
+
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+#include <utility>
+
+namespace cfdg {
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) {
+        return;
+    }
+    const std::string msg = std::string("HIP ") + what + ": " + hipGetErrorString(e);
+    if (e == hipErrorOutOfMemory) {
+        throw OutOfMemory(msg);
+    }
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) {
+        throw NoDevice(msg);
+    }
+    throw HipError(msg);
+}
+
+// Throws NoDevice unless a HIP device is visible.
+void require_device();
+
+// Owning device allocation (hipMalloc / hipFree).
+template <class T>
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(std::size_t n) { resize(n); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) {
+        o.p_ = nullptr;
+        o.n_ = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p_ = o.p_;
+            n_ = o.n_;
+            o.p_ = nullptr;
+            o.n_ = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+
+    void resize(std::size_t n) {
+        if (n == n_) {
+            return;
+        }
+        release();
+        if (n > 0) {
+            hip_check(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), "hipMalloc");
+        }
+        n_ = n;
+    }
+    void upload(const T* h, std::size_t n, hipStream_t s) {
+        resize(n);
+        if (n > 0) {
+            hip_check(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s), "H2D copy");
+            hip_check(hipStreamSynchronize(s), "H2D copy sync");
+        }
+    }
+    void zero(hipStream_t s) {
+        if (n_ > 0) {
+            hip_check(hipMemsetAsync(p_, 0, n_ * sizeof(T), s), "memset");
+        }
+    }
+    [[nodiscard]] T* get() const { return p_; }
+    [[nodiscard]] std::size_t size() const { return n_; }
+    [[nodiscard]] std::size_t bytes() const { return n_ * sizeof(T); }
+
+private:
+    void release() noexcept {
+        if (p_ != nullptr) {
+            (void)hipFree(p_);
+            p_ = nullptr;
+        }
+        n_ = 0;
+    }
+    T* p_ = nullptr;
+    std::size_t n_ = 0;
+};
+
+// Pinned host allocation (hipHostMalloc), used for small asynchronous read-backs.
+template <class T>
+class PinnedBuf {
+public:
+    PinnedBuf() = default;
+    explicit PinnedBuf(std::size_t n) {
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T), hipHostMallocDefault),
+                  "hipHostMalloc");
+        n_ = n;
+    }
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    PinnedBuf(PinnedBuf&& o) noexcept : p_(o.p_), n_(o.n_) {
+        o.p_ = nullptr;
+        o.n_ = 0;
+    }
+    PinnedBuf& operator=(PinnedBuf&& o) noexcept {
+        if (this != &o) {
+            if (p_ != nullptr) {
+                (void)hipHostFree(p_);
+            }
+            p_ = o.p_;
+            n_ = o.n_;
+            o.p_ = nullptr;
+            o.n_ = 0;
+        }
+        return *this;
+    }
+    ~PinnedBuf() {
+        if (p_ != nullptr) {
+            (void)hipHostFree(p_);
+        }
+    }
+    [[nodiscard]] T* get() const { return p_; }
+    T& operator[](std::size_t i) const { return p_[i]; }
+
+private:
+    T* p_ = nullptr;
+    std::size_t n_ = 0;
+};
+
+// Owning stream at the default priority.
+class Stream {
+public:
+    Stream() { hip_check(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "stream create"); }
+    Stream(const Stream&) = delete;
+    Stream& operator=(const Stream&) = delete;
+    ~Stream() {
+        if (s_ != nullptr) {
+            (void)hipStreamDestroy(s_);
+        }
+    }
+    [[nodiscard]] hipStream_t get() const { return s_; }
+
+private:
+    hipStream_t s_ = nullptr;
+};
+
+class Event {
+public:
+    Event() { hip_check(hipEventCreate(&e_), "event create"); }
+    Event(const Event&) = delete;
+    Event& operator=(const Event&) = delete;
+    ~Event() {
+        if (e_ != nullptr) {
+            (void)hipEventDestroy(e_);
+        }
+    }
+    [[nodiscard]] hipEvent_t get() const { return e_; }
+
+private:
+    hipEvent_t e_ = nullptr;
+};
+
+} // namespace cfdg

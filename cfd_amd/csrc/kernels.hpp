@@ -1,0 +1,961 @@
+// Device kernels of the AMG-PCG / Krylov backend for gfx950 (CDNA4, wave64).
+//
+// Layout. Every matrix is sliced ELL with a slice height of 64 rows, one wavefront per slice:
+// entry k of row (64 s + lane) sits at sp[s] + 64 k + lane, so each load instruction of a wave
+// reads 64 consecutive column indices / values. Rows keep the CSR (ascending column) order of
+// the reference and padding entries (value 0) come last, so an FP64 row sum equals the
+// reference's serial sum (src/gpu/src/krylov_kernels.inc:12-25) bit for bit. A workgroup is
+// four wavefronts = four slices = 256 rows.
+//
+// Work mapping. A launch covers the rows of one level for a set of systems: block b processes
+// the 256-row block blocks[b].y of system blocks[b].x. A single solve is a set of one system.
+//
+// Reductions. Dot products never use atomics: each block writes one partial sum (wave
+// butterfly, then the four wave sums in order), and a finalize kernel with one workgroup per
+// system adds the partials in a fixed order. Results are reproducible run to run and do not
+// depend on which other systems share the launch.
+//
+// Control. Each system carries a SolveState; kernels of a system whose status is non-zero
+// return at once, so the host may enqueue iterations ahead of the convergence test.
+#pragma once
+
+#include "layout.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cfdg {
+
+// ------------------------------------------------------------------------------------------
+// Arithmetic helpers. The library is compiled with -ffp-contract=off; FP32 kernels use
+// explicit fused multiply-adds, FP64 kernels keep the reference's separate multiply and add.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double madd(double a, double b, double c) { return c + a * b; }
+__device__ __forceinline__ float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off, k_wave);
+    }
+    return v;
+}
+
+// Sums one value per thread over the block; thread 0 receives the result.
+template <int N>
+__device__ __forceinline__ void block_sums(double (&v)[N], double (&out)[N]) {
+    __shared__ double red[N][k_slices_per_block];
+    const int wave = threadIdx.x / k_wave;
+    const int lane = threadIdx.x % k_wave;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const double w = wave_sum(v[q]);
+        if (lane == 0) {
+            red[q][wave] = w;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            double s = red[q][0];
+#pragma unroll
+            for (int w = 1; w < k_slices_per_block; ++w) {
+                s += red[q][w];
+            }
+            out[q] = s;
+        }
+    }
+}
+
+// Row sum of an ELL row: sum_k val_k * x[col_k] in storage order.
+template <class T>
+__device__ __forceinline__ T ell_row(const EllView& m, int slice, int lane, const T* __restrict__ x) {
+    const int base = m.sp[slice];
+    const int w = (m.sp[slice + 1] - base) / k_wave;
+    const int* __restrict__ col = m.col + base + lane;
+    const T* __restrict__ val = static_cast<const T*>(m.val) + base + lane;
+    T s = T(0);
+    int k = 0;
+    for (; k + 4 <= w; k += 4) {
+        const int c0 = col[(k + 0) * k_wave];
+        const int c1 = col[(k + 1) * k_wave];
+        const int c2 = col[(k + 2) * k_wave];
+        const int c3 = col[(k + 3) * k_wave];
+        const T a0 = val[(k + 0) * k_wave];
+        const T a1 = val[(k + 1) * k_wave];
+        const T a2 = val[(k + 2) * k_wave];
+        const T a3 = val[(k + 3) * k_wave];
+        const T x0 = x[c0];
+        const T x1 = x[c1];
+        const T x2 = x[c2];
+        const T x3 = x[c3];
+        s = madd(a0, x0, s);
+        s = madd(a1, x1, s);
+        s = madd(a2, x2, s);
+        s = madd(a3, x3, s);
+    }
+    for (; k < w; ++k) {
+        s = madd(val[k * k_wave], x[col[k * k_wave]], s);
+    }
+    return s;
+}
+
+// Row sum with the first Jacobi iterate x1_j = wd_j * b_j formed on the fly (the reference's
+// first sweep multiplies A by x = 0, so x1 = (w dinv) b exactly).
+template <class T>
+__device__ __forceinline__ T ell_row_x1(const EllView& m, int slice, int lane, const T* __restrict__ wd,
+                                        const T* __restrict__ b) {
+    const int base = m.sp[slice];
+    const int w = (m.sp[slice + 1] - base) / k_wave;
+    const int* __restrict__ col = m.col + base + lane;
+    const T* __restrict__ val = static_cast<const T*>(m.val) + base + lane;
+    T s = T(0);
+    int k = 0;
+    for (; k + 4 <= w; k += 4) {
+        const int c0 = col[(k + 0) * k_wave];
+        const int c1 = col[(k + 1) * k_wave];
+        const int c2 = col[(k + 2) * k_wave];
+        const int c3 = col[(k + 3) * k_wave];
+        const T a0 = val[(k + 0) * k_wave];
+        const T a1 = val[(k + 1) * k_wave];
+        const T a2 = val[(k + 2) * k_wave];
+        const T a3 = val[(k + 3) * k_wave];
+        const T x0 = wd[c0] * b[c0];
+        const T x1 = wd[c1] * b[c1];
+        const T x2 = wd[c2] * b[c2];
+        const T x3 = wd[c3] * b[c3];
+        s = madd(a0, x0, s);
+        s = madd(a1, x1, s);
+        s = madd(a2, x2, s);
+        s = madd(a3, x3, s);
+    }
+    for (; k < w; ++k) {
+        const int c = col[k * k_wave];
+        s = madd(val[k * k_wave], wd[c] * b[c], s);
+    }
+    return s;
+}
+
+// Outer vectors by id: 0 x, 1 r, 2 p, 3 q, 4 z, 5 r0, 6 v, 7 s, 8 t, 9 ph, 10 sh.
+__device__ __forceinline__ double* outer_vec(const SysDev& S, int id) {
+    switch (id) {
+    case 0: return S.x;
+    case 1: return S.r;
+    case 2: return S.p;
+    case 3: return S.q;
+    case 4: return S.z;
+    case 5: return S.r0;
+    case 6: return S.v;
+    case 7: return S.s;
+    case 8: return S.t;
+    case 9: return S.ph;
+    default: return S.sh;
+    }
+}
+
+#define CFDG_BLOCK_PROLOGUE(sysarr, blocks)                                                     \
+    const int2 bk_ = (blocks)[blockIdx.x];                                                        \
+    const SysDev& S = (sysarr)[bk_.x];                                                            \
+    const int wave_ = threadIdx.x / k_wave;                                                       \
+    const int lane = threadIdx.x % k_wave;                                                        \
+    const int slice = bk_.y * k_slices_per_block + wave_;                                         \
+    const int row = slice * k_wave + lane;                                                        \
+    (void)row;
+
+// ------------------------------------------------------------------------------------------
+// Outer Krylov kernels (level 0, FP64)
+// ------------------------------------------------------------------------------------------
+
+// r = b - A x, partials r.r and b.b; p = q = 0. For mixed precision also r32 = (float) r.
+template <bool Mixed>
+__global__ void __launch_bounds__(k_block) k_residual0(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    double rr = 0.0;
+    double bb = 0.0;
+    if (slice < S.a64.n_slices) {
+        const double ax = ell_row<double>(S.a64, slice, lane, S.x);
+        if (row < S.n) {
+            const double bi = S.b[row];
+            const double ri = bi - ax;
+            S.r[row] = ri;
+            S.p[row] = 0.0;
+            S.q[row] = 0.0;
+            if (Mixed) {
+                S.r32[row] = static_cast<float>(ri);
+            }
+            rr = ri * ri;
+            bb = bi * bi;
+        }
+    }
+    double v[2] = {rr, bb};
+    double o[2];
+    block_sums<2>(v, o);
+    if (threadIdx.x == 0) {
+        S.partials[SLOT_RR * S.nblk0 + bk_.y] = o[0];
+        S.partials[SLOT_BB * S.nblk0 + bk_.y] = o[1];
+    }
+}
+
+// Jacobi CG start: r = b - A x, z = dinv r, p = 0; partials r.r, r.z, b.b.
+__global__ void __launch_bounds__(k_block) k_cg_init(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    double rr = 0.0;
+    double rz = 0.0;
+    double bb = 0.0;
+    if (slice < S.a64.n_slices) {
+        const double ax = ell_row<double>(S.a64, slice, lane, S.x);
+        if (row < S.n) {
+            const double bi = S.b[row];
+            const double ri = bi - ax;
+            const double zi = ri * S.dinv[row];
+            S.r[row] = ri;
+            S.z[row] = zi;
+            S.p[row] = 0.0;
+            rr = ri * ri;
+            rz = ri * zi;
+            bb = bi * bi;
+        }
+    }
+    double v[3] = {rr, rz, bb};
+    double o[3];
+    block_sums<3>(v, o);
+    if (threadIdx.x == 0) {
+        S.partials[SLOT_RR * S.nblk0 + bk_.y] = o[0];
+        S.partials[SLOT_RZ * S.nblk0 + bk_.y] = o[1];
+        S.partials[SLOT_BB * S.nblk0 + bk_.y] = o[2];
+    }
+}
+
+// y = A x (level-0 FP64 operator) with the partial sum of w.y into `slot`; with Two, also the
+// partial of y.w2 into slot2. Used for q = A p (p.q), v = A ph (r0.v), t = A sh (t.t, t.s).
+template <bool Two>
+__global__ void __launch_bounds__(k_block) k_spmv_dot(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int which_x,
+                           int which_y, int which_w, int slot, int which_w2, int slot2) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const double* x = outer_vec(S, which_x);
+    double* y = outer_vec(S, which_y);
+    const double* w = outer_vec(S, which_w);
+    double d1 = 0.0;
+    double d2 = 0.0;
+    if (slice < S.a64.n_slices) {
+        const double yi = ell_row<double>(S.a64, slice, lane, x);
+        if (row < S.n) {
+            y[row] = yi;
+            d1 = w[row] * yi;
+            if (Two) {
+                d2 = yi * outer_vec(S, which_w2)[row];
+            }
+        }
+    }
+    if (Two) {
+        double v[2] = {d1, d2};
+        double o[2];
+        block_sums<2>(v, o);
+        if (threadIdx.x == 0) {
+            S.partials[slot * S.nblk0 + bk_.y] = o[0];
+            S.partials[slot2 * S.nblk0 + bk_.y] = o[1];
+        }
+    } else {
+        double v[1] = {d1};
+        double o[1];
+        block_sums<1>(v, o);
+        if (threadIdx.x == 0) {
+            S.partials[slot * S.nblk0 + bk_.y] = o[0];
+        }
+    }
+}
+
+// x += alpha p; r -= alpha q; partial r.r. Jacobi CG also forms z = dinv r and r.z; the
+// mixed solver writes r32 = (float) r for the FP32 V-cycle.
+template <int Mode> // 0 AMG FP64, 1 Jacobi CG, 2 mixed AMG
+__global__ void __launch_bounds__(k_block) k_update_xr(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const double alpha = S.st->alpha;
+    double rr = 0.0;
+    double rz = 0.0;
+    if (row < S.n) {
+        S.x[row] = S.x[row] + alpha * S.p[row];
+        const double ri = S.r[row] - alpha * S.q[row];
+        S.r[row] = ri;
+        rr = ri * ri;
+        if (Mode == 1) {
+            const double zi = ri * S.dinv[row];
+            S.z[row] = zi;
+            rz = ri * zi;
+        }
+        if (Mode == 2) {
+            S.r32[row] = static_cast<float>(ri);
+        }
+    }
+    if (Mode == 1) {
+        double v[2] = {rr, rz};
+        double o[2];
+        block_sums<2>(v, o);
+        if (threadIdx.x == 0) {
+            S.partials[SLOT_RR * S.nblk0 + bk_.y] = o[0];
+            S.partials[SLOT_RZ * S.nblk0 + bk_.y] = o[1];
+        }
+    } else {
+        double v[1] = {rr};
+        double o[1];
+        block_sums<1>(v, o);
+        if (threadIdx.x == 0) {
+            S.partials[SLOT_RR * S.nblk0 + bk_.y] = o[0];
+        }
+    }
+}
+
+// p = z + beta p. which = 0: z is the Jacobi-preconditioned residual S.z (FP64); which = 1:
+// z is the level-0 V-cycle output v[z_buf] (FP64, or FP32 in the mixed solver).
+template <class TZ>
+__global__ void __launch_bounds__(k_block) k_p_update(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int which,
+                           int z_buf) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const TZ* z = which == 0 ? reinterpret_cast<const TZ*>(S.z)
+                             : static_cast<const TZ*>(S.lv[0].v[z_buf]);
+    const double beta = S.st->beta;
+    if (row < S.n) {
+        S.p[row] = static_cast<double>(z[row]) + beta * S.p[row];
+    }
+}
+
+// True residual |b - A x|^2 partials (always runs, after the solve).
+__global__ void __launch_bounds__(k_block) k_true_residual(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    double t = 0.0;
+    if (slice < S.a64.n_slices) {
+        const double ax = ell_row<double>(S.a64, slice, lane, S.x);
+        if (row < S.n) {
+            const double ri = S.b[row] - ax;
+            t = ri * ri;
+        }
+    }
+    double v[1] = {t};
+    double o[1];
+    block_sums<1>(v, o);
+    if (threadIdx.x == 0) {
+        S.partials[SLOT_TRUE * S.nblk0 + bk_.y] = o[0];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// BiCGStab kernels (krylov_kernels.inc:96-169)
+// ------------------------------------------------------------------------------------------
+
+// r = b - A x; r0 = r; p = v = 0; partials r.r, b.b and r0.r.
+__global__ void __launch_bounds__(k_block) k_bicg_init(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    double rr = 0.0;
+    double bb = 0.0;
+    if (slice < S.a64.n_slices) {
+        const double ax = ell_row<double>(S.a64, slice, lane, S.x);
+        if (row < S.n) {
+            const double bi = S.b[row];
+            const double ri = bi - ax;
+            S.r[row] = ri;
+            S.r0[row] = ri;
+            S.p[row] = 0.0;
+            S.v[row] = 0.0;
+            rr = ri * ri;
+            bb = bi * bi;
+        }
+    }
+    double v[2] = {rr, bb};
+    double o[2];
+    block_sums<2>(v, o);
+    if (threadIdx.x == 0) {
+        S.partials[SLOT_RR * S.nblk0 + bk_.y] = o[0];
+        S.partials[SLOT_BB * S.nblk0 + bk_.y] = o[1];
+        S.partials[SLOT_R0R * S.nblk0 + bk_.y] = o[0];
+    }
+}
+
+// p = r + beta (p - omega v); ph = dinv p.
+__global__ void __launch_bounds__(k_block) k_bicg_p(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const double beta = S.st->beta;
+    const double omega = S.st->omega;
+    if (row < S.n) {
+        const double pi = S.r[row] + beta * (S.p[row] - omega * S.v[row]);
+        S.p[row] = pi;
+        S.ph[row] = pi * S.dinv[row];
+    }
+}
+
+// s = r - alpha v; sh = dinv s; partial s.s.
+__global__ void __launch_bounds__(k_block) k_bicg_s(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const double alpha = S.st->alpha;
+    double ss = 0.0;
+    if (row < S.n) {
+        const double si = S.r[row] - alpha * S.v[row];
+        S.s[row] = si;
+        S.sh[row] = si * S.dinv[row];
+        ss = si * si;
+    }
+    double v[1] = {ss};
+    double o[1];
+    block_sums<1>(v, o);
+    if (threadIdx.x == 0) {
+        S.partials[SLOT_SS * S.nblk0 + bk_.y] = o[0];
+    }
+}
+
+// Early exit of BiCGStab: x += alpha ph (status ST_CONVERGED_PENDING only).
+__global__ void __launch_bounds__(k_block) k_bicg_early_x(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_CONVERGED_PENDING) {
+        return;
+    }
+    if (row < S.n) {
+        S.x[row] = S.x[row] + S.st->alpha * S.ph[row];
+    }
+}
+
+// x += alpha ph + omega sh; r = s - omega t; partials r.r and r0.r.
+__global__ void __launch_bounds__(k_block) k_bicg_xr(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const double alpha = S.st->alpha;
+    const double omega = S.st->omega;
+    double rr = 0.0;
+    double r0r = 0.0;
+    if (row < S.n) {
+        S.x[row] = S.x[row] + (alpha * S.ph[row] + omega * S.sh[row]);
+        const double ri = S.s[row] - omega * S.t[row];
+        S.r[row] = ri;
+        rr = ri * ri;
+        r0r = S.r0[row] * ri;
+    }
+    double v[2] = {rr, r0r};
+    double o[2];
+    block_sums<2>(v, o);
+    if (threadIdx.x == 0) {
+        S.partials[SLOT_RR * S.nblk0 + bk_.y] = o[0];
+        S.partials[SLOT_R0R * S.nblk0 + bk_.y] = o[1];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// V-cycle kernels (krylov_kernels.inc:182-238), element type T (double or float)
+// ------------------------------------------------------------------------------------------
+
+// First two Jacobi sweeps from x = 0: v[out] = x1 + wd (b - A x1), x1 = wd b.
+template <class T>
+__global__ void __launch_bounds__(k_block) k_presmooth2(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const LevelDev& L = S.lv[level];
+    if (L.coarsest || slice >= L.a.n_slices) {
+        return;
+    }
+    const T* b = static_cast<const T*>(L.b);
+    const T* wd = static_cast<const T*>(L.wd);
+    const T ax = ell_row_x1<T>(L.a, slice, lane, wd, b);
+    if (row < L.n) {
+        const T x1 = wd[row] * b[row];
+        static_cast<T*>(L.v[out])[row] = x1 + wd[row] * (b[row] - ax);
+    }
+}
+
+// One Jacobi sweep from x = 0: v[out] = wd b.
+template <class T>
+__global__ void __launch_bounds__(k_block) k_jacobi_zero(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const LevelDev& L = S.lv[level];
+    if (L.coarsest || row >= L.n) {
+        return;
+    }
+    const T* b = static_cast<const T*>(L.b);
+    const T* wd = static_cast<const T*>(L.wd);
+    static_cast<T*>(L.v[out])[row] = wd[row] * b[row];
+}
+
+// v[out] = 0 (smoothing-free V-cycles).
+template <class T>
+__global__ void __launch_bounds__(k_block) k_zero_level(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const LevelDev& L = S.lv[level];
+    if (L.coarsest || row >= L.n) {
+        return;
+    }
+    static_cast<T*>(L.v[out])[row] = T(0);
+}
+
+// Jacobi sweep v[out] = v[in] + wd (b - A v[in]). Dots: 0 none; 1 partial r.z with r = b
+// (FP64 level 0, where b aliases r); 2 partials r.z and z.q with r, q the FP64 outer vectors
+// (mixed level 0).
+template <class T, int Dots>
+__global__ void __launch_bounds__(k_block) k_jacobi(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in, int out) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const LevelDev& L = S.lv[level];
+    if (L.coarsest) {
+        return;
+    }
+    double d1 = 0.0;
+    double d2 = 0.0;
+    if (slice < L.a.n_slices) {
+        const T* x = static_cast<const T*>(L.v[in]);
+        const T* b = static_cast<const T*>(L.b);
+        const T* wd = static_cast<const T*>(L.wd);
+        const T ax = ell_row<T>(L.a, slice, lane, x);
+        if (row < L.n) {
+            const T z = x[row] + wd[row] * (b[row] - ax);
+            static_cast<T*>(L.v[out])[row] = z;
+            if (Dots == 1) {
+                d1 = static_cast<double>(b[row]) * static_cast<double>(z);
+            }
+            if (Dots == 2) {
+                const double zd = static_cast<double>(z);
+                d1 = S.r[row] * zd;
+                d2 = zd * S.q[row];
+            }
+        }
+    }
+    if (Dots == 1) {
+        double v[1] = {d1};
+        double o[1];
+        block_sums<1>(v, o);
+        if (threadIdx.x == 0) {
+            S.partials[SLOT_RZ * S.nblk0 + bk_.y] = o[0];
+        }
+    }
+    if (Dots == 2) {
+        double v[2] = {d1, d2};
+        double o[2];
+        block_sums<2>(v, o);
+        if (threadIdx.x == 0) {
+            S.partials[SLOT_RZ * S.nblk0 + bk_.y] = o[0];
+            S.partials[SLOT_ZQ * S.nblk0 + bk_.y] = o[1];
+        }
+    }
+}
+
+// v[out] = b - A v[in].
+template <class T>
+__global__ void __launch_bounds__(k_block) k_residual(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in, int out) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const LevelDev& L = S.lv[level];
+    if (L.coarsest || slice >= L.a.n_slices) {
+        return;
+    }
+    const T* x = static_cast<const T*>(L.v[in]);
+    const T ax = ell_row<T>(L.a, slice, lane, x);
+    if (row < L.n) {
+        static_cast<T*>(L.v[out])[row] = static_cast<const T*>(L.b)[row] - ax;
+    }
+}
+
+// b_{l+1} = R_l v_l[in]; blocks index the rows of level l + 1.
+template <class T>
+__global__ void __launch_bounds__(k_block) k_restrict(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const LevelDev& L = S.lv[level];
+    if (L.coarsest || slice >= L.r.n_slices) {
+        return;
+    }
+    const T y = ell_row<T>(L.r, slice, lane, static_cast<const T*>(L.v[in]));
+    if (row < L.n_next) {
+        static_cast<T*>(S.lv[level + 1].b)[row] = y;
+    }
+}
+
+// v_l[io] += P_l v_{l+1}[coarse_buf].
+template <class T>
+__global__ void __launch_bounds__(k_block) k_prolong(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int io,
+                          int coarse_buf) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const LevelDev& L = S.lv[level];
+    if (L.coarsest || slice >= L.p.n_slices) {
+        return;
+    }
+    const T t = ell_row<T>(L.p, slice, lane, static_cast<const T*>(S.lv[level + 1].v[coarse_buf]));
+    if (row < L.n) {
+        T* x = static_cast<T*>(L.v[io]);
+        x[row] = x[row] + t;
+    }
+}
+
+// Coarsest level: v[out] = C^-1 b with the dense FP64 inverse, one wavefront per row; blocks
+// index groups of four rows.
+template <class T>
+__global__ void __launch_bounds__(k_block) k_coarse(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
+    const int2 bk = blocks[blockIdx.x];
+    const SysDev& S = sys[bk.x];
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const LevelDev& L = S.lv[level];
+    const int wave = threadIdx.x / k_wave;
+    const int lane = threadIdx.x % k_wave;
+    const int i = bk.y * k_slices_per_block + wave;
+    if (!L.coarsest || i >= L.n) {
+        return;
+    }
+    const int n = L.n;
+    const double* m = L.cinv + static_cast<long long>(i) * n;
+    const T* b = static_cast<const T*>(L.b);
+    double s = 0.0;
+    for (int j = lane; j < n; j += k_wave) {
+        s += m[j] * static_cast<double>(b[j]);
+    }
+    s = wave_sum(s);
+    if (lane == 0) {
+        static_cast<T*>(L.v[out])[i] = static_cast<T>(s);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Finalize kernels: one workgroup per system, partials added in a fixed order.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double reduce_slot(const SysDev& S, int slot) {
+    __shared__ double red[k_slices_per_block];
+    const double* p = S.partials + static_cast<long long>(slot) * S.nblk0;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < S.nblk0; i += k_block) {
+        s += p[i];
+    }
+    s = wave_sum(s);
+    if (threadIdx.x % k_wave == 0) {
+        red[threadIdx.x / k_wave] = s;
+    }
+    __syncthreads();
+    double t = red[0];
+#pragma unroll
+    for (int w = 1; w < k_slices_per_block; ++w) {
+        t += red[w];
+    }
+    __syncthreads();
+    return t;
+}
+
+enum FinOp : int {
+    FIN_AMG_INIT = 0,   // res0, |b|; converged before iterating?
+    FIN_ALPHA = 1,      // pq; breakdown if 0; alpha = rz / pq
+    FIN_RES = 2,        // res; converged?
+    FIN_BETA = 3,       // rz_new; beta = rz_new / rz (0 first); it++ / limit
+    FIN_BETA_FLEX = 4,  // rz_new, zq; beta = -zq / pq (0 first); it++ / limit
+    FIN_CG_INIT = 5,    // Jacobi CG start: res0, |b|, rz; beta = 0; it = 1
+    FIN_CG_RES_BETA = 6,// Jacobi CG: res check then beta
+    FIN_TRUE = 7,       // true residual
+    FIN_BICG_INIT = 8,
+    FIN_BICG_RHO = 9,
+    FIN_BICG_ALPHA = 10,
+    FIN_BICG_S = 11,
+    FIN_BICG_OMEGA = 12,
+    FIN_BICG_RES = 13
+};
+
+template <int Op>
+__global__ void __launch_bounds__(k_block) k_finalize(const SysDev* __restrict__ sys, const int* __restrict__ ids) {
+    const SysDev& S = sys[ids[blockIdx.x]];
+    SolveState* st = S.st;
+    const int status = st->status;
+    if (Op == FIN_TRUE) {
+        const double t = reduce_slot(S, SLOT_TRUE);
+        if (threadIdx.x == 0) {
+            st->true_res = sqrt(t);
+        }
+        return;
+    }
+    if (status != ST_RUNNING && !(Op == FIN_BICG_OMEGA && status == ST_CONVERGED_PENDING)) {
+        return;
+    }
+    if (Op == FIN_AMG_INIT || Op == FIN_CG_INIT || Op == FIN_BICG_INIT) {
+        const double rr = reduce_slot(S, SLOT_RR);
+        const double bb = reduce_slot(S, SLOT_BB);
+        double rz = 0.0;
+        if (Op == FIN_CG_INIT) {
+            rz = reduce_slot(S, SLOT_RZ);
+        }
+        double r0r = 0.0;
+        if (Op == FIN_BICG_INIT) {
+            r0r = reduce_slot(S, SLOT_R0R);
+        }
+        if (threadIdx.x == 0) {
+            const double res = sqrt(rr);
+            st->res = res;
+            st->bnorm = sqrt(bb);
+            st->it = 0;
+            if (res <= st->tol) {
+                st->status = ST_CONVERGED;
+            } else if (Op == FIN_CG_INIT) {
+                st->rz = rz;
+                st->beta = 0.0;
+                st->it = 1;
+                if (st->it > st->max_it) {
+                    st->status = ST_MAXIT;
+                }
+            } else if (Op == FIN_BICG_INIT) {
+                st->rho = 1.0;
+                st->alpha = 1.0;
+                st->omega = 1.0;
+                st->rz = r0r; // rho_new of the first iteration
+                st->it = 1;
+            }
+        }
+        return;
+    }
+    if (Op == FIN_ALPHA) {
+        const double pq = reduce_slot(S, SLOT_PQ);
+        if (threadIdx.x == 0) {
+            st->pq = pq;
+            if (pq == 0.0) {
+                st->status = ST_BREAKDOWN;
+            } else {
+                st->alpha = st->rz / pq;
+            }
+        }
+        return;
+    }
+    if (Op == FIN_RES) {
+        const double rr = reduce_slot(S, SLOT_RR);
+        if (threadIdx.x == 0) {
+            const double res = sqrt(rr);
+            st->res = res;
+            if (res <= st->tol) {
+                st->status = ST_CONVERGED;
+            }
+        }
+        return;
+    }
+    if (Op == FIN_BETA || Op == FIN_BETA_FLEX) {
+        const double rz_new = reduce_slot(S, SLOT_RZ);
+        double zq = 0.0;
+        if (Op == FIN_BETA_FLEX) {
+            zq = reduce_slot(S, SLOT_ZQ);
+        }
+        if (threadIdx.x == 0) {
+            if (st->it == 0) {
+                st->beta = 0.0;
+            } else if (Op == FIN_BETA) {
+                st->beta = rz_new / st->rz;
+            } else {
+                st->beta = -zq / st->pq;
+            }
+            st->rz = rz_new;
+            st->it += 1;
+            if (st->it > st->max_it) {
+                st->status = ST_MAXIT;
+            }
+        }
+        return;
+    }
+    if (Op == FIN_CG_RES_BETA) {
+        const double rr = reduce_slot(S, SLOT_RR);
+        const double rz_new = reduce_slot(S, SLOT_RZ);
+        if (threadIdx.x == 0) {
+            const double res = sqrt(rr);
+            st->res = res;
+            if (res <= st->tol) {
+                st->status = ST_CONVERGED;
+            } else {
+                st->beta = rz_new / st->rz;
+                st->rz = rz_new;
+                st->it += 1;
+                if (st->it > st->max_it) {
+                    st->status = ST_MAXIT;
+                }
+            }
+        }
+        return;
+    }
+    if (Op == FIN_BICG_RHO) {
+        if (threadIdx.x == 0) {
+            const double rho_new = st->rz;
+            if (st->it > st->max_it) {
+                st->status = ST_MAXIT;
+            } else if (rho_new == 0.0 || st->omega == 0.0) {
+                st->status = ST_BREAKDOWN;
+            } else {
+                st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
+                st->rho = rho_new;
+            }
+        }
+        return;
+    }
+    if (Op == FIN_BICG_ALPHA) {
+        const double r0v = reduce_slot(S, SLOT_R0V);
+        if (threadIdx.x == 0) {
+            if (r0v == 0.0) {
+                st->status = ST_BREAKDOWN;
+            } else {
+                st->alpha = st->rho / r0v;
+            }
+        }
+        return;
+    }
+    if (Op == FIN_BICG_S) {
+        const double ss = reduce_slot(S, SLOT_SS);
+        if (threadIdx.x == 0) {
+            const double s_norm = sqrt(ss);
+            if (s_norm <= st->tol) {
+                st->res = s_norm;
+                st->status = ST_CONVERGED_PENDING;
+            }
+        }
+        return;
+    }
+    if (Op == FIN_BICG_OMEGA) {
+        const double tt = reduce_slot(S, SLOT_TT);
+        const double ts = reduce_slot(S, SLOT_TS);
+        if (threadIdx.x == 0) {
+            if (status == ST_CONVERGED_PENDING) {
+                st->status = ST_CONVERGED;
+            } else {
+                st->omega = tt > 0.0 ? ts / tt : 0.0;
+            }
+        }
+        return;
+    }
+    if (Op == FIN_BICG_RES) {
+        const double rr = reduce_slot(S, SLOT_RR);
+        const double r0r = reduce_slot(S, SLOT_R0R);
+        if (threadIdx.x == 0) {
+            const double res = sqrt(rr);
+            st->res = res;
+            if (res <= st->tol) {
+                st->status = ST_CONVERGED;
+            } else {
+                st->rz = r0r;
+                st->it += 1;
+            }
+        }
+        return;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Assembly kernels
+// ------------------------------------------------------------------------------------------
+
+// Level-0 values from LDU coefficients: val[k] = ldu[src[k]] (0 for padding), with
+// ldu = [diag | upper | lower]; dinv = 1 / diag (1.0 for a zero diagonal,
+// krylov_cuda.cu:557-562); wd = w dinv in FP64 and FP32; FP32 copy of the values.
+__global__ void __launch_bounds__(k_block) k_ldu_gather(long long n_slots, const int* __restrict__ src, const double* __restrict__ ldu,
+                             double* __restrict__ val64, float* __restrict__ val32) {
+    const long long k = static_cast<long long>(blockIdx.x) * k_block + threadIdx.x;
+    if (k >= n_slots) {
+        return;
+    }
+    const int s = src[k];
+    const double v = s >= 0 ? ldu[s] : 0.0;
+    val64[k] = v;
+    if (val32 != nullptr) {
+        val32[k] = static_cast<float>(v);
+    }
+}
+
+__global__ void __launch_bounds__(k_block) k_diag_inverse(int n, const double* __restrict__ diag, double w, double* __restrict__ dinv,
+                               double* __restrict__ wd64, float* __restrict__ wd32) {
+    const int i = blockIdx.x * k_block + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const double d = diag[i];
+    const double di = d != 0.0 ? 1.0 / d : 1.0;
+    dinv[i] = di;
+    const double wdi = w * di;
+    wd64[i] = wdi;
+    if (wd32 != nullptr) {
+        wd32[i] = static_cast<float>(wdi);
+    }
+}
+
+// Face coefficients of the convection-diffusion operator (krylov_openmp.cpp:289-295).
+__global__ void __launch_bounds__(k_block) k_assemble_faces(int nf, const double* __restrict__ flux, const double* __restrict__ gamma,
+                                 const double* __restrict__ delta, double convection,
+                                 double* __restrict__ upper, double* __restrict__ lower) {
+    const int f = blockIdx.x * k_block + threadIdx.x;
+    if (f >= nf) {
+        return;
+    }
+    const double fl = flux[f];
+    const double g = gamma[f] * delta[f];
+    const double pos = fl > 0.0 ? convection * fl : 0.0;
+    const double neg = fl < 0.0 ? convection * fl : 0.0;
+    upper[f] = neg - g;
+    lower[f] = -pos - g;
+}
+
+// Per-cell accumulation of diag / divergence over the cell's faces in ascending face order,
+// the order of the reference's serial face loop (krylov_openmp.cpp:296-299).
+__global__ void __launch_bounds__(k_block) k_assemble_cells(int nc, const int* __restrict__ cf_ptr, const int* __restrict__ cf_face,
+                                 const int* __restrict__ owner, const double* __restrict__ flux,
+                                 const double* __restrict__ gamma, const double* __restrict__ delta,
+                                 double convection, double* __restrict__ diag,
+                                 double* __restrict__ divergence) {
+    const int c = blockIdx.x * k_block + threadIdx.x;
+    if (c >= nc) {
+        return;
+    }
+    double d = diag[c];
+    double dv = divergence[c];
+    for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+        const int f = cf_face[k];
+        const double fl = flux[f];
+        const double g = gamma[f] * delta[f];
+        const double pos = fl > 0.0 ? convection * fl : 0.0;
+        const double neg = fl < 0.0 ? convection * fl : 0.0;
+        if (owner[f] == c) {
+            d += pos + g;
+            dv += fl;
+        } else {
+            d += -neg + g;
+            dv -= fl;
+        }
+    }
+    diag[c] = d;
+    divergence[c] = dv;
+}
+
+} // namespace cfdg

@@ -1,0 +1,102 @@
+// Device data layout shared by the host driver and the kernels: sliced-ELL views, per-level and
+// per-system descriptors, and the per-system solve state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cfdg {
+
+constexpr int k_wave = 64;
+constexpr int k_block = 256;
+constexpr int k_slices_per_block = k_block / k_wave;
+constexpr int k_max_levels = 25;
+
+enum Slot : int {
+    SLOT_PQ = 0,
+    SLOT_RR = 1,
+    SLOT_RZ = 2,
+    SLOT_ZQ = 3,
+    SLOT_BB = 4,
+    SLOT_TRUE = 5,
+    SLOT_R0R = 6,
+    SLOT_R0V = 7,
+    SLOT_SS = 8,
+    SLOT_TT = 9,
+    SLOT_TS = 10,
+    k_slots = 11
+};
+
+enum Status : int {
+    ST_RUNNING = 0,
+    ST_CONVERGED = 1,
+    ST_BREAKDOWN = 2,
+    ST_MAXIT = 3,
+    ST_CONVERGED_PENDING = 4 // BiCGStab early exit: x += alpha ph still to apply
+};
+
+struct SolveState {
+    double tol;
+    double rz;
+    double pq;
+    double alpha;
+    double beta;
+    double res;
+    double bnorm;
+    double true_res;
+    double rho;
+    double omega;
+    double zq;
+    int it;
+    int max_it;
+    int status;
+    int pad;
+};
+
+struct EllView {
+    int n_rows;
+    int n_slices;
+    const int* sp;   // n_slices + 1 element offsets
+    const int* col;  // sliced column indices
+    const void* val; // double or float values
+};
+
+struct LevelDev {
+    EllView a;        // n x n
+    EllView p;        // n x n_next
+    EllView r;        // n_next x n
+    int n;
+    int n_next;
+    int coarsest;     // 1 on the coarsest level
+    int pad;
+    void* v[3];       // work vectors (iterate / residual / ping-pong), element type T
+    void* b;          // right-hand side of the level, element type T
+    const void* wd;   // jacobi_weight * dinv, element type T
+    const double* cinv; // dense row-major inverse (coarsest level only)
+};
+
+struct SysDev {
+    SolveState* st;
+    double* partials; // k_slots * nblk0
+    int nblk0;        // 256-row blocks of level 0
+    int n;
+    int n_levels;
+    int pad;
+    EllView a64;            // level-0 operator in FP64
+    const double* b;        // right-hand side
+    double* x;              // iterate
+    double* r;
+    double* p;
+    double* q;
+    double* z;              // Jacobi-preconditioned residual / BiCGStab scratch
+    float* r32;             // FP32 copy of r (mixed V-cycle input)
+    const double* dinv;     // 1 / a_ii (1.0 where a_ii == 0)
+    double* r0;             // BiCGStab
+    double* v;
+    double* s;
+    double* t;
+    double* ph;
+    double* sh;
+    LevelDev lv[k_max_levels];
+};
+
+} // namespace cfdg

@@ -1,0 +1,101 @@
+// Device-resident linear systems and the batched solve driver.
+#pragma once
+
+#include "ell.hpp"
+#include "hip_util.hpp"
+#include "kernels.hpp"
+
+#include <memory>
+#include <vector>
+
+namespace cfdg {
+
+// One AMG level on the device. Level 0 uses the system's assembled operator instead of `a`.
+struct DevLevel {
+    int n = 0;
+    int n_next = 0;
+    DevEll a;
+    DevEll p;
+    DevEll r;
+    DevBuf<double> wd64;
+    DevBuf<float> wd32;
+    DevBuf<double> v64[3];
+    DevBuf<double> b64;
+    DevBuf<float> v32[3];
+    DevBuf<float> b32;
+    DevBuf<double> cinv; // coarsest level only
+    [[nodiscard]] std::size_t bytes() const;
+};
+
+// A linear system resident in HBM: level-0 operator (FP64 + FP32 values), right-hand side,
+// iterate, Krylov work vectors, and optionally an AMG hierarchy.
+class System {
+public:
+    explicit System(const cfdg_ldu_pattern& pattern); // values assembled later from LDU
+    explicit System(const cfdg_csr& a);               // values given
+    System(const System&) = delete;
+    System& operator=(const System&) = delete;
+
+    void set_amg(const cfdg_amg_view& v);
+    void assemble(const double* diag, const double* upper, const double* lower);
+    void set_rhs(const double* b);
+    void set_x(const double* x);
+    void get_x(double* x);
+    [[nodiscard]] std::size_t footprint() const;
+
+    [[nodiscard]] int n() const { return n_; }
+    [[nodiscard]] int n_faces() const { return nf_; }
+    [[nodiscard]] int n_levels() const { return has_amg_ ? static_cast<int>(levels_.size()) : 0; }
+    [[nodiscard]] bool has_amg() const { return has_amg_; }
+    [[nodiscard]] int sweeps() const { return sweeps_; }
+    [[nodiscard]] double weight() const { return weight_; }
+    [[nodiscard]] hipStream_t stream() const { return stream_.get(); }
+    [[nodiscard]] SolveState* state() const { return state_.get(); }
+    [[nodiscard]] int level_rows(int l) const { return levels_[static_cast<std::size_t>(l)].n; }
+
+    // Descriptor of this system for kernels of the given precision (FP64 or FP32 V-cycle).
+    [[nodiscard]] SysDev descriptor(bool fp32_vcycle, bool use_amg);
+
+    // Generation counter, bumped when the hierarchy or vector set changes (descriptor caches).
+    [[nodiscard]] unsigned generation() const { return generation_; }
+
+private:
+    void init_vectors();
+    void ensure_level_vectors(bool fp32);
+
+    int n_ = 0;
+    int nf_ = 0;
+    Stream stream_;
+    DevEll a0_;
+    DevBuf<int> a0_src_;      // ELL slot -> index into [diag | upper | lower], -1 padding
+    DevBuf<double> ldu_;      // staging of LDU coefficients
+    DevBuf<double> dinv0_;
+    DevBuf<double> wd0_64_;
+    DevBuf<float> wd0_32_;
+    DevBuf<double> b_, x_, r_, p_, q_, z_, r0_, v_, s_, t_, ph_, sh_;
+    DevBuf<float> r32_;
+    DevBuf<double> partials_;
+    DevBuf<SolveState> state_;
+    std::vector<DevLevel> levels_;
+    bool has_amg_ = false;
+    bool assembled_ = false;
+    int sweeps_ = 2;
+    double weight_ = 0.7;
+    unsigned generation_ = 0;
+};
+
+// Solve options shared by every system of a batch.
+struct BatchOptions {
+    cfdg_method method = CFDG_METHOD_CG;
+    bool use_amg = true;
+    cfdg_precision precision = CFDG_PREC_FP64;
+};
+
+// Solves a set of resident systems together: one launch per stage covers all of them.
+// Every system keeps its own reductions, scalars and convergence test, so each result is
+// independent of the other members of the set.
+void solve_batch(const std::vector<System*>& systems, const BatchOptions& opt,
+                 const std::vector<cfdg_solve_control>& control,
+                 std::vector<cfdg_solve_stats>& stats, hipStream_t stream);
+
+} // namespace cfdg

@@ -1,0 +1,266 @@
+/*
+ * cfdgpu.h — C-ABI of the MI355X (gfx950) linear-solver backend for the unstructured
+ * finite-volume solver Yipexnn/CFD.
+ *
+ * The library replaces the device side of the reference's accelerator library
+ * (namespace cfd::gpu, src/gpu/include/cfd/gpu/krylov.hpp). Every entry point below names the
+ * reference declaration it stands in for. All functions return a cfdg_status; on failure the
+ * message is kept per thread and can be read with cfdg_last_error(). Non-convergence is not an
+ * error: it is reported in cfdg_solve_stats.converged, as the reference reports it in
+ * SolveStats (krylov.hpp:26-31).
+ *
+ * Arrays are plain host pointers unless the name says "device"; the caller owns them and they
+ * are only borrowed for the duration of the call (krylov.hpp:79-103, device_solver.cpp:58-73).
+ * Indices are int32 (core/types.hpp:15), values are IEEE binary64 (core/types.hpp:11).
+ * A handle may be used by one host thread at a time; different handles are independent.
+ * There is no host fallback: without a HIP device every solve returns CFDG_ERR_NO_DEVICE.
+ */
+#ifndef CFDGPU_H
+#define CFDGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFDG_VERSION_MAJOR 0
+#define CFDG_VERSION_MINOR 1
+
+typedef enum cfdg_status {
+    CFDG_OK = 0,
+    CFDG_ERR_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument */
+    CFDG_ERR_NO_DEVICE = 2,        /* no HIP device visible */
+    CFDG_ERR_HIP = 3,              /* reference: std::runtime_error("CUDA ...") */
+    CFDG_ERR_OUT_OF_MEMORY = 4,
+    CFDG_ERR_INTERNAL = 5,
+    CFDG_ERR_CAPACITY = 6          /* request larger than the admitted footprint */
+} cfdg_status;
+
+/* Method (krylov.hpp:19). */
+typedef enum cfdg_method { CFDG_METHOD_CG = 0, CFDG_METHOD_BICGSTAB = 1 } cfdg_method;
+
+/* Arithmetic of the AMG-PCG solve. FP64 is the reference algorithm (damped-Jacobi V-cycle,
+ * standard PCG; krylov_kernels.inc:182-301). The mixed modes run the V-cycle in FP32
+ * (level-0 off-diagonals optionally stored as per-row-scaled BF16) inside an FP64 flexible-CG
+ * outer loop; they have no counterpart in the reference. */
+typedef enum cfdg_precision {
+    CFDG_PREC_FP64 = 0,
+    CFDG_PREC_MIXED_FP32 = 1,
+    CFDG_PREC_MIXED_BF16 = 2
+} cfdg_precision;
+
+/* CsrView (krylov.hpp:10-16): square (or, for transfer operators, rectangular) CSR matrix. */
+typedef struct cfdg_csr {
+    int n_rows;
+    int n_cols; /* 0 means n_cols = n_rows */
+    int nnz;
+    const int* row_ptr; /* n_rows + 1 */
+    const int* col;     /* nnz */
+    const double* val;  /* nnz */
+} cfdg_csr;
+
+/* SolveControl (krylov.hpp:21-24). */
+typedef struct cfdg_solve_control {
+    double abs_tolerance; /* stop when the recurrence |r|_2 <= abs_tolerance */
+    int max_iterations;
+} cfdg_solve_control;
+
+/* SolveStats (krylov.hpp:26-31), extended. */
+typedef struct cfdg_solve_stats {
+    int iterations;
+    int converged;
+    int on_device;
+    int exit_reason;      /* 0 running, 1 converged, 2 breakdown, 3 iteration limit */
+    double residual;      /* recurrence |r|_2 at exit (reference: SolveStats.residual) */
+    double true_residual; /* |b - A x|_2 recomputed in FP64 at exit */
+    double rhs_norm;      /* |b|_2 */
+    double solve_ms;      /* device time of the solve, HIP events on the solve stream */
+} cfdg_solve_stats;
+
+/* AmgLevelView / AmgView (krylov.hpp:48-58): per level the operator a and, except on the
+ * coarsest level, the prolongation p (n_l x n_{l+1}) and restriction r (n_{l+1} x n_l);
+ * coarse_inverse is the dense row-major inverse of the coarsest operator. */
+typedef struct cfdg_amg_level_view {
+    cfdg_csr a;
+    cfdg_csr p;
+    cfdg_csr r;
+} cfdg_amg_level_view;
+
+typedef struct cfdg_amg_view {
+    int n_levels;
+    const cfdg_amg_level_view* levels;
+    const double* coarse_inverse;
+    int smooth_sweeps;    /* reference default 2 */
+    double jacobi_weight; /* reference default 0.7 */
+} cfdg_amg_view;
+
+/* LduPattern (krylov.hpp:68-77). */
+typedef struct cfdg_ldu_pattern {
+    int n;
+    int n_faces;
+    int nnz;
+    const int* row_ptr;
+    const int* col;
+    const int* diag_pos;
+    const int* upper_pos;
+    const int* lower_pos;
+} cfdg_ldu_pattern;
+
+/* ------------------------------------------------------------------------------------------
+ * Library / device information
+ * ---------------------------------------------------------------------------------------- */
+
+/* Replaces backend_name() (krylov.hpp:119): writes "hip". */
+cfdg_status cfdg_backend_name(char* buf, size_t len);
+/* Replaces device_available() (krylov.hpp:121). Never fails; *available = 0 without a GPU. */
+cfdg_status cfdg_device_available(int* available);
+/* Replaces device_description() (krylov.hpp:123), e.g. "hip (device: AMD Instinct MI355X)". */
+cfdg_status cfdg_device_description(char* buf, size_t len);
+/* Message of the last failed call on this thread (reference: exception what()). */
+cfdg_status cfdg_last_error(char* buf, size_t len);
+/* Library version as major * 1000 + minor. */
+int cfdg_version(void);
+
+/* ------------------------------------------------------------------------------------------
+ * One-shot solves (device)
+ * ---------------------------------------------------------------------------------------- */
+
+/* Replaces solve() (krylov.hpp:37-38; dispatch.cpp:5-13): Jacobi-preconditioned CG or
+ * BiCGStab, x is the initial guess on entry and the solution on exit. */
+cfdg_status cfdg_solve_csr(const cfdg_csr* a, const double* b, double* x, cfdg_method method,
+                           const cfdg_solve_control* control, cfdg_solve_stats* stats);
+
+/* Replaces solve_amg_cg() (krylov.hpp:61-62; dispatch.cpp:15-23): CG preconditioned by one
+ * V-cycle of the given hierarchy per iteration, A = levels[0].a. */
+cfdg_status cfdg_solve_amg_cg(const cfdg_amg_view* amg, const double* b, double* x,
+                              const cfdg_solve_control* control, cfdg_solve_stats* stats);
+
+/* Same, with the V-cycle precision selectable (no counterpart in the reference). */
+cfdg_status cfdg_solve_amg_cg_prec(const cfdg_amg_view* amg, const double* b, double* x,
+                                   cfdg_precision precision, const cfdg_solve_control* control,
+                                   cfdg_solve_stats* stats);
+
+/* Replaces assemble_convection_diffusion() (krylov.hpp:105-116; host loop
+ * krylov_openmp.cpp:289-300). upper/lower are written, diag/divergence are accumulated.
+ * The device kernel gathers per cell in ascending face order, so every cell sees its faces in
+ * the order of the reference's serial loop and the result equals it exactly. */
+cfdg_status cfdg_assemble_convection_diffusion(int n_cells, int n_faces, const int* owner,
+                                               const int* neighbour, const double* flux,
+                                               const double* gamma, const double* delta_coeff,
+                                               double convection, double* upper, double* lower,
+                                               double* diag, double* divergence);
+
+/* ------------------------------------------------------------------------------------------
+ * Persistent system: replaces class DeviceSolver (krylov.hpp:79-103, device_solver.cpp)
+ * ---------------------------------------------------------------------------------------- */
+
+typedef struct cfdg_system cfdg_system;
+
+/* DeviceSolver(const LduPattern&) (device_solver.cpp:58-73). */
+cfdg_status cfdg_system_create(const cfdg_ldu_pattern* pattern, cfdg_system** out);
+/* ~DeviceSolver() (device_solver.cpp:75-81). NULL is accepted. */
+cfdg_status cfdg_system_destroy(cfdg_system* sys);
+/* DeviceSolver::on_device() (device_solver.cpp:83-85). */
+cfdg_status cfdg_system_on_device(const cfdg_system* sys, int* on_device);
+/* DeviceSolver::set_amg() (device_solver.cpp:87-115): levels[0].a is ignored (the assembled
+ * matrix is used), everything else is copied to the device now. An empty hierarchy is
+ * CFDG_ERR_INVALID_ARGUMENT, as the reference throws std::invalid_argument. */
+cfdg_status cfdg_system_set_amg(cfdg_system* sys, const cfdg_amg_view* amg);
+/* DeviceSolver::solve() (device_solver.cpp:117-150): uploads diag (n), upper / lower
+ * (n_faces), b and x, assembles A on the device and solves; x receives the solution. */
+cfdg_status cfdg_system_solve(cfdg_system* sys, const double* diag, const double* upper,
+                              const double* lower, const double* b, double* x,
+                              cfdg_method method, int use_amg, const cfdg_solve_control* control,
+                              cfdg_solve_stats* stats);
+
+/* Resident use (no counterpart in the reference): the matrix, right-hand side and solution
+ * stay in HBM between calls, so repeated solves move no data over PCIe. */
+cfdg_status cfdg_system_assemble(cfdg_system* sys, const double* diag, const double* upper,
+                                 const double* lower);
+cfdg_status cfdg_system_set_rhs(cfdg_system* sys, const double* b);
+cfdg_status cfdg_system_set_x(cfdg_system* sys, const double* x); /* NULL: x = 0 */
+cfdg_status cfdg_system_get_x(cfdg_system* sys, double* x);
+cfdg_status cfdg_system_solve_resident(cfdg_system* sys, cfdg_method method, int use_amg,
+                                       cfdg_precision precision,
+                                       const cfdg_solve_control* control,
+                                       cfdg_solve_stats* stats);
+/* Device bytes held by the system (matrices, hierarchy, work vectors). */
+cfdg_status cfdg_system_footprint(const cfdg_system* sys, size_t* bytes);
+
+/* ------------------------------------------------------------------------------------------
+ * AMG hierarchy setup on the host (replaces fv::AmgPreconditioner::setup, amg.cpp:122-190,
+ * and coarse_inverse, amg.cpp:227-240, which the reference runs through Eigen in its caller)
+ * ---------------------------------------------------------------------------------------- */
+
+typedef struct cfdg_amg cfdg_amg;
+
+/* AmgSettings (fv/include/cfd/fv/amg.hpp:17-23) plus the V-cycle parameters of AmgView. */
+typedef struct cfdg_amg_settings {
+    double strength_threshold; /* 0.08 */
+    int max_levels;            /* 25 */
+    int coarse_size;           /* 400 */
+    int smooth_sweeps;         /* 2 */
+    double jacobi_weight;      /* 0.7 */
+    double strength_decay;     /* threshold factor per level: level l uses
+                                  strength_threshold * strength_decay^l. 1.0 is the reference
+                                  (fixed threshold); 0.5 is the halving per level of Vanek,
+                                  Mandel & Brezina (1996), which the reference cites. */
+} cfdg_amg_settings;
+
+cfdg_status cfdg_amg_default_settings(cfdg_amg_settings* s);
+cfdg_status cfdg_amg_build(const cfdg_csr* a, const cfdg_amg_settings* settings,
+                           cfdg_amg** out);
+cfdg_status cfdg_amg_destroy(cfdg_amg* amg);
+cfdg_status cfdg_amg_n_levels(const cfdg_amg* amg, int* n_levels);
+/* which: 0 = a, 1 = p, 2 = r. Sizes of a level's matrix (nnz = 0 on the coarsest p / r). */
+cfdg_status cfdg_amg_level_size(const cfdg_amg* amg, int level, int which, int* n_rows,
+                                int* n_cols, int* nnz);
+/* Copies a level's CSR arrays out (any of the three pointers may be NULL). */
+cfdg_status cfdg_amg_level_copy(const cfdg_amg* amg, int level, int which, int* row_ptr,
+                                int* col, double* val);
+/* Aggregate index per row of a level (the aggregation that produced level + 1). */
+cfdg_status cfdg_amg_aggregates(const cfdg_amg* amg, int level, int* agg);
+/* Dense row-major inverse of the coarsest operator (n_c * n_c doubles). */
+cfdg_status cfdg_amg_coarse_inverse(const cfdg_amg* amg, double* out);
+/* Borrowed view of the hierarchy (valid while amg lives); levels must hold n_levels slots. */
+cfdg_status cfdg_amg_view_of(const cfdg_amg* amg, cfdg_amg_level_view* levels,
+                             cfdg_amg_view* view);
+/* Setup wall time of the last build in milliseconds. */
+cfdg_status cfdg_amg_setup_ms(const cfdg_amg* amg, double* ms);
+
+/* ------------------------------------------------------------------------------------------
+ * Case generators (restate the reference's box mesher and FV geometry so that the benchmark
+ * systems can be synthesised; make_box_mesh box_mesher.cpp:73-145, Mesh::compute_geometry
+ * mesh.cpp:100-186, face weights fv_geometry.cpp:52-58)
+ * ---------------------------------------------------------------------------------------- */
+
+typedef struct cfdg_box_spec {
+    int nx, ny, nz;
+    double lo[3];
+    double hi[3];
+    double jitter;  /* interior vertices moved by U[-jitter, jitter] * spacing per axis */
+    uint64_t seed;  /* mt19937_64 seed of the jitter */
+} cfdg_box_spec;
+
+cfdg_status cfdg_box_sizes(const cfdg_box_spec* spec, int* n_cells, int* n_internal_faces);
+/* Internal faces in the reference order (owner ascending, then neighbour); per face the
+ * coefficient |S_f| / |C_N - C_P| and the interpolation weight; per cell volume and centre
+ * (3 doubles). Any output pointer except owner / neighbour may be NULL. */
+cfdg_status cfdg_box_build(const cfdg_box_spec* spec, int* owner, int* neighbour,
+                           double* area_over_distance, double* weight, double* cell_volume,
+                           double* cell_centre);
+/* n draws of mt19937_64(seed) mapped to lo + (hi - lo) * (u >> 11) * 2^-53. */
+cfdg_status cfdg_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out);
+/* CSR pattern of the cell graph with value positions (build_csr_pattern,
+ * ldu_matrix.cpp:572-612). row_ptr n+1, col n+2*n_faces. */
+cfdg_status cfdg_csr_pattern(int n, int n_faces, const int* owner, const int* neighbour,
+                             int* row_ptr, int* col, int* diag_pos, int* upper_pos,
+                             int* lower_pos);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFDGPU_H */

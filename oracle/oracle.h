@@ -1,0 +1,96 @@
+/*
+ * oracle.h — CPU restatement of the reference's pressure / momentum linear-solver path.
+ *
+ * TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load this library, and only as the checker or the timed CPU baseline; the product path
+ * (cfd_amd/, libcfdgpu) never links or calls it.
+ *
+ * Every function cites the reference file:line it follows (paths relative to the Yipexnn/CFD
+ * repository). Arithmetic is written in the reference's order and the library is built with
+ * -ffp-contract=off, so a serial run reproduces the reference's host results bit for bit where
+ * the reference itself is serial (n <= 20000, src/gpu/src/krylov_openmp.cpp:21-25).
+ */
+#ifndef CFD_ORACLE_H
+#define CFD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Thread count used by the OpenMP loops (the reference threads loops with n > 20000). */
+void orc_set_threads(int n);
+
+/* ---- LDU matrix (src/fv/src/ldu_matrix.cpp) ---- */
+void orc_ldu_multiply(int n, int nf, const int* own, const int* nei, const double* diag,
+                      const double* upper, const double* lower, const double* x, double* y);
+void orc_ldu_residual(int n, int nf, const int* own, const int* nei, const double* diag,
+                      const double* upper, const double* lower, const double* source,
+                      const double* x, double* r);
+double orc_ldu_normalised_residual(int n, int nf, const int* own, const int* nei,
+                                   const double* diag, const double* upper, const double* lower,
+                                   const double* source, const double* x);
+void orc_ldu_set_reference(double* diag, double* source, int cell, double value);
+int orc_csr_pattern(int n, int nf, const int* own, const int* nei, int* rp, int* ci,
+                    int* diag_pos, int* upper_pos, int* lower_pos);
+void orc_ldu_to_csr(int n, int nf, const int* diag_pos, const int* upper_pos,
+                    const int* lower_pos, const double* diag, const double* upper,
+                    const double* lower, double* val);
+void orc_assemble(int n_cells, int n_faces, const int* own, const int* nei, const double* flux,
+                  const double* gamma, const double* delta, double convection, double* upper,
+                  double* lower, double* diag, double* divergence);
+
+/* ---- Krylov kernels (src/gpu/src/krylov_kernels.inc) ---- */
+typedef struct orc_stats {
+    int iterations;
+    int converged;
+    double residual;
+} orc_stats;
+
+void orc_spmv(int n, const int* rp, const int* ci, const double* va, const double* x, double* y);
+double orc_dot(int n, const double* x, const double* y);
+/* method 0 = Jacobi CG, 1 = Jacobi BiCGStab (solve_openmp, krylov_openmp.cpp:93-138). */
+orc_stats orc_solve(int n, const int* rp, const int* ci, const double* va, const double* b,
+                    double* x, int method, double abs_tol, int max_it);
+
+/* ---- AMG hierarchy (src/fv/src/amg.cpp) ---- */
+typedef struct orc_amg orc_amg;
+int orc_aggregate(int n, const int* rp, const int* ci, const double* va, const double* diag,
+                  double theta, int* agg);
+orc_amg* orc_amg_setup(int n, const int* rp, const int* ci, const double* va, double theta,
+                       int max_levels, int coarse_size);
+/* Same with the threshold of level l set to theta * decay^l (decay 1: the reference). */
+orc_amg* orc_amg_setup_decay(int n, const int* rp, const int* ci, const double* va, double theta,
+                             double decay, int max_levels, int coarse_size);
+void orc_amg_free(orc_amg* h);
+int orc_amg_levels(const orc_amg* h);
+/* which: 0 a, 1 p, 2 r. */
+void orc_amg_size(const orc_amg* h, int level, int which, int* rows, int* cols, int* nnz);
+void orc_amg_copy(const orc_amg* h, int level, int which, int* rp, int* ci, double* va);
+void orc_amg_aggregates(const orc_amg* h, int level, int* agg);
+/* Dense row-major inverse of the coarsest operator; returns 1 if the LDL^T succeeded. */
+int orc_amg_coarse_inverse(const orc_amg* h, double* out);
+
+/* Jacobi-V-cycle AMG-PCG on a hierarchy given as CSR arrays per level (the reference's
+ * solve_amg_cg_host: krylov_openmp.cpp:187-255 -> krylov_kernels.inc:182-301).
+ * Level l has a (n_l rows), and for l < L-1 p (n_l x n_{l+1}) and r (n_{l+1} x n_l). */
+orc_stats orc_amg_cg(int n_levels, const int* n, const int* const* a_rp, const int* const* a_ci,
+                     const double* const* a_va, const int* const* p_rp, const int* const* p_ci,
+                     const double* const* p_va, const int* const* r_rp, const int* const* r_ci,
+                     const double* const* r_va, const double* coarse_inverse, int sweeps,
+                     double weight, const double* b, double* x, double abs_tol, int max_it);
+
+/* The same, taking the hierarchy built by orc_amg_setup with level 0 replaced by (rp, ci, va)
+ * when rp != NULL (DeviceSolver's stale-hierarchy semantics, device_solver.cpp:138-147). */
+orc_stats orc_amg_cg_h(const orc_amg* h, const int* rp, const int* ci, const double* va,
+                       int sweeps, double weight, const double* b, double* x, double abs_tol,
+                       int max_it);
+
+/* CPU AmgCg: PCG with the symmetric Gauss-Seidel V-cycle and a direct coarsest solve
+ * (fv::pcg amg.cpp:253-285, cycle amg.cpp:192-216). Breakdown when p.Ap <= 0. */
+orc_stats orc_sgs_pcg(const orc_amg* h, const double* b, double* x, double tol, int max_it);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
