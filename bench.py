@@ -128,14 +128,32 @@ def main(argv=None) -> int:
 
     run(Precision.MIXED_FP32, args.warmup)
     run(Precision.FP64, max(1, args.warmup // 2))
+    # Timed region: HIP events on the solve stream bracket every launch of the dominant kernel
+    # (the last level-0 Jacobi sweep of the V-cycle, fused with the two FCG dot products).
+    dev.probe(1)
     t_mixed = time.perf_counter()
     lat_m, dms_m, its_m, st_m = run(Precision.MIXED_FP32, args.steps)
     t_mixed = time.perf_counter() - t_mixed
+    probe_m = dev.probe_read(Precision.MIXED_FP32)
     x_mixed = dev.get_x()
+    dev.probe(1)
     t_fp64 = time.perf_counter()
     lat_d, dms_d, its_d, st_d = run(Precision.FP64, args.steps)
     t_fp64 = time.perf_counter() - t_fp64
+    probe_d = dev.probe_read(Precision.FP64)
+    dev.probe(2)
+    run(Precision.FP64, 2)
+    probe_spmv = dev.probe_read(Precision.FP64)
+    dev.probe(0)
     x_fp64 = dev.get_x()
+
+    def roof(p, name):
+        ms, launches, nbytes = p
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+                "bytes_per_launch": int(nbytes), "avg_us": round(ms * 1e3, 3),
+                "launches": int(launches)}
     log(f"mixed: {args.steps / t_mixed:.2f} solves/s, it {its_m[-1]}, true rel res "
         f"{st_m.true_residual / bnorm:.2e}; fp64: {args.steps / t_fp64:.2f} solves/s, it {its_d[-1]}, "
         f"true rel res {st_d.true_residual / bnorm:.2e}")
@@ -197,7 +215,10 @@ def main(argv=None) -> int:
                                    "true_rel_residual": st_mom.true_residual / np.linalg.norm(m_sys.source)},
         "setup_ms": round(hier.setup_ms, 1),
         "footprint_bytes": dev.footprint(),
-        "roofline": None,
+        "roofline": roof(probe_m, "k_jacobi<float,2>: last level-0 FP32 Jacobi sweep + r.z, z.q"),
+        "roofline_fp64_sweep": roof(probe_d, "k_jacobi<double,1>: last level-0 FP64 Jacobi sweep + r.z"),
+        "roofline_fp64_spmv": roof(probe_spmv, "k_spmv_dot<false>: FP64 q = A p + p.q"),
+        "working_set_mb": round(dev.footprint() / 1e6, 1),
         "cpu_baseline": None,
     }
     if not args.skip_cpu:

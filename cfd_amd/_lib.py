@@ -151,6 +151,8 @@ _SIGNATURES = {
     "cfdg_system_solve_resident": (ST, [VP, C.c_int, C.c_int, C.c_int, C.POINTER(SolveControl),
                                         C.POINTER(SolveStats)]),
     "cfdg_system_footprint": (ST, [VP, C.POINTER(C.c_size_t)]),
+    "cfdg_system_probe": (ST, [VP, C.c_int]),
+    "cfdg_system_probe_read": (ST, [VP, C.c_int, P_DBL, C.POINTER(C.c_longlong), P_DBL]),
     "cfdg_amg_default_settings": (ST, [C.POINTER(AmgSettings)]),
     "cfdg_amg_build": (ST, [C.POINTER(Csr), C.POINTER(AmgSettings), C.POINTER(VP)]),
     "cfdg_amg_destroy": (ST, [VP]),

@@ -164,6 +164,7 @@ System::System(const cfdg_ldu_pattern& pt) {
         require(pt.col[k] >= 0 && pt.col[k] < pt.n, "LduPattern: column out of range");
     }
     n_ = pt.n;
+    nnz0_ = pt.nnz;
     nf_ = pt.n_faces;
     std::vector<int> csr_src(static_cast<std::size_t>(pt.nnz), -1);
     auto place = [&](int pos, int src) {
@@ -196,6 +197,7 @@ System::System(const cfdg_csr& a) {
     require(a.n_rows >= 1, "CsrView: empty matrix");
     check_csr(a, a.n_rows, a.n_rows, "CsrView: invalid square CSR matrix");
     n_ = a.n_rows;
+    nnz0_ = a.nnz;
     nf_ = 0;
     const HostEll e = csr_to_ell(n_, n_, a.row_ptr, a.col);
     hipStream_t s = stream_.get();
@@ -515,6 +517,33 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
         }                                                                                         \
     } while (0)
 
+// Kernel timing of one solve: HIP events on the solve stream bracket every launch of the
+// probed kernel. The first `useful` launches are the ones that did work (later launches of
+// the iteration pipeline return at once and are not counted).
+thread_local Probe* g_probe = nullptr;
+
+void probe_begin(int kind, hipStream_t s) {
+    if (g_probe == nullptr || g_probe->kind != kind) {
+        return;
+    }
+    if (g_probe->used == g_probe->events.size()) {
+        hipEvent_t a = nullptr;
+        hipEvent_t b = nullptr;
+        hip_check(hipEventCreate(&a), "event create");
+        hip_check(hipEventCreate(&b), "event create");
+        g_probe->events.emplace_back(a, b);
+    }
+    hip_check(hipEventRecord(g_probe->events[g_probe->used].first, s), "event record");
+}
+
+void probe_end(int kind, hipStream_t s) {
+    if (g_probe == nullptr || g_probe->kind != kind) {
+        return;
+    }
+    hip_check(hipEventRecord(g_probe->events[g_probe->used].second, s), "event record");
+    ++g_probe->used;
+}
+
 int final_index(int sweeps) {
     const int cur = (sweeps >= 2 && (sweeps - 2) % 2 == 1) ? 2 : 0;
     return sweeps % 2 == 0 ? cur : 1;
@@ -559,7 +588,9 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
             const int o = c == 1 ? c0 : 1;
             const bool last = k == sweeps - 1;
             if (l == 0 && last) {
+                probe_begin(PROBE_L0_SWEEP, s);
                 CFDG_LAUNCH((k_jacobi<T, Dots>), g, s, d, blk, l, c, o);
+                probe_end(PROBE_L0_SWEEP, s);
             } else {
                 CFDG_LAUNCH((k_jacobi<T, 0>), g, s, d, blk, l, c, o);
             }
@@ -634,7 +665,9 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
         fin<FIN_BICG_RES>(d, tb, s);
         return;
     }
+    probe_begin(PROBE_OUTER_SPMV, s);
     CFDG_LAUNCH(k_spmv_dot<false>, g0, s, d, b0, V_P, V_Q, V_P, SLOT_PQ, 0, 0);
+    probe_end(PROBE_OUTER_SPMV, s);
     fin<FIN_ALPHA>(d, tb, s);
     if (!pl.amg) {
         CFDG_LAUNCH(k_update_xr<1>, g0, s, d, b0);
@@ -698,6 +731,12 @@ void solve_batch(const std::vector<System*>& systems, const BatchOptions& opt,
     d_init.upload(init.data(), init.size(), s);
     const int n_sys = static_cast<int>(systems.size());
     const SysDev* d = d_desc.get();
+    Probe* probe = (systems.size() == 1 && systems[0]->probe.kind != PROBE_OFF) ? &systems[0]->probe
+                                                                             : nullptr;
+    struct ProbeScope { // the thread's probe for the duration of this solve
+        explicit ProbeScope(Probe* p) { g_probe = p; if (p != nullptr) { p->used = 0; } }
+        ~ProbeScope() { g_probe = nullptr; }
+    } probe_scope(probe);
     const Plan pl{opt.method, amg, fp32, sweeps};
 
     PinnedBuf<int> running(2);
@@ -753,6 +792,23 @@ void solve_batch(const std::vector<System*>& systems, const BatchOptions& opt,
         o.true_residual = st.true_res;
         o.rhs_norm = st.bnorm;
         o.solve_ms = ms;
+    }
+    if (probe != nullptr) {
+        // Launches that did work: the initial V-cycle plus one per iteration that preconditioned
+        // (level-0 sweep), or one per iteration (outer SpMV).
+        const int its = stats[0].iterations;
+        std::size_t useful = static_cast<std::size_t>(its);
+        if (probe->kind == PROBE_L0_SWEEP && stats[0].exit_reason != ST_CONVERGED) {
+            useful = static_cast<std::size_t>(its) + 1;
+        }
+        useful = std::min(useful, probe->used);
+        for (std::size_t i = 0; i < useful; ++i) {
+            float e = 0.0f;
+            hip_check(hipEventElapsedTime(&e, probe->events[i].first, probe->events[i].second),
+                      "event elapsed");
+            probe->total_ms += e;
+        }
+        probe->launches += static_cast<long long>(useful);
     }
 }
 
@@ -875,6 +931,44 @@ extern "C" cfdg_status cfdg_system_solve(cfdg_system* sys, const double* diag, c
         const bool amg = use_amg != 0 && method == CFDG_METHOD_CG && s.has_amg();
         *stats = solve_one(s, method, amg, CFDG_PREC_FP64, *control);
         s.get_x(x);
+    });
+}
+
+extern "C" cfdg_status cfdg_system_probe(cfdg_system* sys, int kind) {
+    return guarded([&] {
+        require(sys != nullptr, "null system");
+        require(kind >= PROBE_OFF && kind <= PROBE_OUTER_SPMV, "unknown probe kind");
+        Probe& p = sys->sys->probe;
+        p.kind = kind;
+        p.total_ms = 0.0;
+        p.launches = 0;
+    });
+}
+
+extern "C" cfdg_status cfdg_system_probe_read(const cfdg_system* sys, int precision,
+                                              double* avg_ms, long long* launches,
+                                              double* bytes_per_launch) {
+    return guarded([&] {
+        require(sys != nullptr && avg_ms != nullptr && launches != nullptr &&
+                    bytes_per_launch != nullptr,
+                "null argument");
+        const System& s = *sys->sys;
+        const Probe& p = s.probe;
+        *launches = p.launches;
+        *avg_ms = p.launches > 0 ? p.total_ms / static_cast<double>(p.launches) : 0.0;
+        const double n = static_cast<double>(s.n());
+        const double nnz = static_cast<double>(s.nnz0());
+        const bool fp32 = precision != CFDG_PREC_FP64;
+        if (p.kind == PROBE_L0_SWEEP) {
+            // matrix (value + column index) + x_in, b, w*dinv, x_out at the V-cycle precision;
+            // the mixed sweep also reads r and q in FP64 for its two dot products.
+            *bytes_per_launch = fp32 ? nnz * (4.0 + 4.0) + 4.0 * 4.0 * n + 2.0 * 8.0 * n
+                                     : nnz * (8.0 + 4.0) + 4.0 * 8.0 * n;
+        } else if (p.kind == PROBE_OUTER_SPMV) {
+            *bytes_per_launch = nnz * (8.0 + 4.0) + 2.0 * 8.0 * n; // p in, q out
+        } else {
+            *bytes_per_launch = 0.0;
+        }
     });
 }
 

@@ -27,10 +27,33 @@ struct DevLevel {
     [[nodiscard]] std::size_t bytes() const;
 };
 
+enum ProbeKind : int {
+    PROBE_OFF = 0,
+    PROBE_L0_SWEEP = 1,   // last level-0 Jacobi sweep of the V-cycle (fused with the dots)
+    PROBE_OUTER_SPMV = 2  // outer FP64 q = A p fused with p.q
+};
+
+// HIP-event timing of one kernel class over the solves of a system.
+struct Probe {
+    int kind = PROBE_OFF;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::size_t used = 0;     // launches recorded in the current solve
+    double total_ms = 0.0;    // accumulated over solves (useful launches only)
+    long long launches = 0;
+    ~Probe() {
+        for (auto& e : events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    }
+};
+
 // A linear system resident in HBM: level-0 operator (FP64 + FP32 values), right-hand side,
 // iterate, Krylov work vectors, and optionally an AMG hierarchy.
 class System {
 public:
+    Probe probe;
+    [[nodiscard]] long long nnz0() const { return nnz0_; }
     explicit System(const cfdg_ldu_pattern& pattern); // values assembled later from LDU
     explicit System(const cfdg_csr& a);               // values given
     System(const System&) = delete;
@@ -65,6 +88,7 @@ private:
 
     int n_ = 0;
     int nf_ = 0;
+    long long nnz0_ = 0; // stored entries of the level-0 operator (padding excluded)
     Stream stream_;
     DevEll a0_;
     DevBuf<int> a0_src_;      // ELL slot -> index into [diag | upper | lower], -1 padding

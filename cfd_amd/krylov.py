@@ -299,6 +299,18 @@ class DeviceSolver:
                int(precision), C.byref(cc), C.byref(st))
         return SolveStats.from_c(st)
 
+    def probe(self, kind: int) -> None:
+        """Times one kernel class with HIP events during the next solves (1: last level-0
+        Jacobi sweep, 2: outer FP64 SpMV, 0: off)."""
+        L.call("cfdg_system_probe", self._h, int(kind))
+
+    def probe_read(self, precision: Precision):
+        """(average launch ms, launches, algorithmic bytes per launch)."""
+        ms, n, by = C.c_double(), C.c_longlong(), C.c_double()
+        L.call("cfdg_system_probe_read", self._h, int(precision), C.byref(ms), C.byref(n),
+               C.byref(by))
+        return ms.value, n.value, by.value
+
     def footprint(self) -> int:
         v = C.c_size_t()
         L.call("cfdg_system_footprint", self._h, C.byref(v))

@@ -186,6 +186,16 @@ cfdg_status cfdg_system_solve_resident(cfdg_system* sys, cfdg_method method, int
                                        cfdg_precision precision,
                                        const cfdg_solve_control* control,
                                        cfdg_solve_stats* stats);
+/* Kernel timing for benchmarks (no counterpart in the reference). kind 1 brackets every launch
+ * of the last level-0 Jacobi sweep of the V-cycle, kind 2 every outer FP64 q = A p, with HIP
+ * events on the system's stream during the following single-system solves; 0 turns it off.
+ * Setting a kind resets the accumulators. */
+cfdg_status cfdg_system_probe(cfdg_system* sys, int kind);
+/* Average duration of the probed launches that did work, their number, and the algorithmic
+ * bytes of one launch (stored matrix entries x (value + 4-byte index) + vectors read and written
+ * once) for the given precision. */
+cfdg_status cfdg_system_probe_read(const cfdg_system* sys, int precision, double* avg_ms,
+                                   long long* launches, double* bytes_per_launch);
 /* Device bytes held by the system (matrices, hierarchy, work vectors). */
 cfdg_status cfdg_system_footprint(const cfdg_system* sys, size_t* bytes);
 
