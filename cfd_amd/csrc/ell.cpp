@@ -1,6 +1,7 @@
 #include "ell.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <limits>
 
 namespace cfdg {
@@ -52,10 +53,38 @@ HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci) {
     return e;
 }
 
+namespace {
+double env_or(const char* name, double dflt) {
+    const char* v = std::getenv(name);
+    return v != nullptr ? std::atof(v) : dflt;
+}
+} // namespace
+
+int pick_tpr(int n_rows, long long nnz) {
+    if (n_rows <= 0) {
+        return 1;
+    }
+    // Tuning knobs (environment): rows with at most CFDG_ELL_MAX_AVG entries on average stay
+    // thread-per-row ELL; longer rows get lanes so that each lane sums about
+    // CFDG_TPR_ENTRIES entries.
+    static const double ell_max = env_or("CFDG_ELL_MAX_AVG", 12.0);
+    static const double per_lane = env_or("CFDG_TPR_ENTRIES", 4.0);
+    const double avg = static_cast<double>(nnz) / static_cast<double>(n_rows);
+    if (avg <= ell_max) {
+        return 1;
+    }
+    int t = 2;
+    while (t < 64 && per_lane * t * 2 <= avg) {
+        t *= 2;
+    }
+    return t;
+}
+
 void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hipStream_t s) {
     n_rows = e.n_rows;
     n_cols = e.n_cols;
     n_slices = e.n_slices;
+    tpr = 1;
     sp.upload(e.sp.data(), e.sp.size(), s);
     col.upload(e.col.data(), e.col.size(), s);
     if (va != nullptr) {
@@ -68,6 +97,28 @@ void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hi
             v32.upload(v.data(), v.size(), s);
         }
     }
+}
+
+void DevEll::upload_csr(int rows, int cols, const int* rp, const int* ci, const double* va,
+                        hipStream_t s, int tpr_override) {
+    const int t = tpr_override > 0 ? tpr_override : pick_tpr(rows, rp[rows]);
+    if (t == 1) {
+        upload(csr_to_ell(rows, cols, rp, ci), va, true, true, s);
+        return;
+    }
+    n_rows = rows;
+    n_cols = cols;
+    n_slices = 0;
+    tpr = t;
+    const std::size_t nnz = static_cast<std::size_t>(rp[rows]);
+    sp.upload(rp, static_cast<std::size_t>(rows) + 1, s);
+    col.upload(ci, nnz, s);
+    v64.upload(va, nnz, s);
+    std::vector<float> f(nnz);
+    for (std::size_t k = 0; k < nnz; ++k) {
+        f[k] = static_cast<float>(va[k]);
+    }
+    v32.upload(f.data(), nnz, s);
 }
 
 } // namespace cfdg

@@ -1,4 +1,4 @@
-// Host-side conversion of CSR matrices to the sliced-ELL layout of the device kernels.
+// Host-side conversion of CSR matrices to the device layouts of the kernels.
 #pragma once
 
 #include "hip_util.hpp"
@@ -32,19 +32,28 @@ std::vector<T> ell_values(const HostEll& e, const double* va) {
     return v;
 }
 
-// Device copy of a sliced-ELL matrix with FP64 and / or FP32 values.
+// Lanes per row for a matrix: 1 (sliced ELL) for short rows, else a power of two near a
+// quarter of the mean row length, at most 64.
+int pick_tpr(int n_rows, long long nnz);
+
+// Device copy of a sparse matrix (sliced ELL or CSR, see EllView) with FP64 and / or FP32
+// values.
 struct DevEll {
     int n_rows = 0;
     int n_cols = 0;
     int n_slices = 0;
+    int tpr = 1;
     DevBuf<int> sp;
     DevBuf<int> col;
     DevBuf<double> v64;
     DevBuf<float> v32;
 
     void upload(const HostEll& e, const double* va, bool fp64, bool fp32, hipStream_t s);
-    [[nodiscard]] EllView view64() const { return {n_rows, n_slices, sp.get(), col.get(), v64.get()}; }
-    [[nodiscard]] EllView view32() const { return {n_rows, n_slices, sp.get(), col.get(), v32.get()}; }
+    // Chooses the layout from the row lengths (tpr_override > 0 forces it).
+    void upload_csr(int n_rows, int n_cols, const int* rp, const int* ci, const double* va,
+                    hipStream_t s, int tpr_override = 0);
+    [[nodiscard]] EllView view64() const { return {n_rows, tpr, n_slices, 0, sp.get(), col.get(), v64.get()}; }
+    [[nodiscard]] EllView view32() const { return {n_rows, tpr, n_slices, 0, sp.get(), col.get(), v32.get()}; }
     [[nodiscard]] std::size_t bytes() const {
         return sp.bytes() + col.bytes() + v64.bytes() + v32.bytes();
     }

@@ -139,6 +139,71 @@ __device__ __forceinline__ T ell_row_x1(const EllView& m, int slice, int lane, c
     return s;
 }
 
+// Sum over groups of tpr consecutive lanes (tpr a power of two <= 64); every lane of the
+// wavefront must take part.
+template <class T>
+__device__ __forceinline__ T group_sum(T v, int tpr) {
+    for (int off = tpr >> 1; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off, k_wave);
+    }
+    return v;
+}
+
+// Row product of a matrix in either layout for the row / sub-lane of the calling thread; the
+// result is complete in every lane of the row's group. Rows past the end give 0.
+template <class T>
+__device__ __forceinline__ T mat_row(const EllView& m, int row, int sub, const T* __restrict__ x) {
+    if (m.tpr == 1) {
+        const int slice = row / k_wave;
+        return slice < m.n_slices ? ell_row<T>(m, slice, row % k_wave, x) : T(0);
+    }
+    T s = T(0);
+    if (row < m.n_rows) {
+        const int* __restrict__ col = m.col;
+        const T* __restrict__ val = static_cast<const T*>(m.val);
+        const int end = m.sp[row + 1];
+        for (int k = m.sp[row] + sub; k < end; k += m.tpr) {
+            s = madd(val[k], x[col[k]], s);
+        }
+    }
+    return group_sum(s, m.tpr);
+}
+
+template <class T>
+__device__ __forceinline__ T mat_row_x1(const EllView& m, int row, int sub, const T* __restrict__ wd,
+                                        const T* __restrict__ b) {
+    if (m.tpr == 1) {
+        const int slice = row / k_wave;
+        return slice < m.n_slices ? ell_row_x1<T>(m, slice, row % k_wave, wd, b) : T(0);
+    }
+    T s = T(0);
+    if (row < m.n_rows) {
+        const int* __restrict__ col = m.col;
+        const T* __restrict__ val = static_cast<const T*>(m.val);
+        const int end = m.sp[row + 1];
+        for (int k = m.sp[row] + sub; k < end; k += m.tpr) {
+            const int c = col[k];
+            s = madd(val[k], wd[c] * b[c], s);
+        }
+    }
+    return group_sum(s, m.tpr);
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on XCD i mod 8) and
+// every XCD has its own L2. This bijection hands each XCD a contiguous range of logical blocks,
+// so rows that share neighbours (and their x entries) meet in the same L2.
+constexpr int k_xcds = 8;
+
+__device__ __forceinline__ int xcd_block() {
+    const int g = static_cast<int>(gridDim.x);
+    const int b = static_cast<int>(blockIdx.x);
+    const int q = g / k_xcds;
+    const int r = g % k_xcds;
+    const int xcd = b % k_xcds;
+    const int k = b / k_xcds;
+    return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
 // Outer vectors by id: 0 x, 1 r, 2 p, 3 q, 4 z, 5 r0, 6 v, 7 s, 8 t, 9 ph, 10 sh.
 __device__ __forceinline__ double* outer_vec(const SysDev& S, int id) {
     switch (id) {
@@ -157,7 +222,7 @@ __device__ __forceinline__ double* outer_vec(const SysDev& S, int id) {
 }
 
 #define CFDG_BLOCK_PROLOGUE(sysarr, blocks)                                                     \
-    const int2 bk_ = (blocks)[blockIdx.x];                                                        \
+    const int2 bk_ = (blocks)[xcd_block()];                                                       \
     const SysDev& S = (sysarr)[bk_.x];                                                            \
     const int wave_ = threadIdx.x / k_wave;                                                       \
     const int lane = threadIdx.x % k_wave;                                                        \
@@ -469,27 +534,39 @@ __global__ void __launch_bounds__(k_block) k_bicg_xr(const SysDev* __restrict__ 
 // V-cycle kernels (krylov_kernels.inc:182-238), element type T (double or float)
 // ------------------------------------------------------------------------------------------
 
-// First two Jacobi sweeps from x = 0: v[out] = x1 + wd (b - A x1), x1 = wd b.
+// Row mapping of a V-cycle kernel over matrix M: 256 / tpr rows per workgroup, tpr lanes per
+// row. Early returns here are uniform over the workgroup.
+#define CFDG_LEVEL_PROLOGUE(sysarr, blocks, level, MAT)                                            \
+    const int2 bk_ = (blocks)[xcd_block()];                                                       \
+    const SysDev& S = (sysarr)[bk_.x];                                                            \
+    if (S.st->status != ST_RUNNING) {                                                             \
+        return;                                                                                   \
+    }                                                                                             \
+    const LevelDev& L = S.lv[level];                                                              \
+    if (L.coarsest) {                                                                             \
+        return;                                                                                   \
+    }                                                                                             \
+    const EllView& M = L.MAT;                                                                     \
+    const int tpr = M.tpr;                                                                        \
+    const int row = bk_.y * (k_block / tpr) + static_cast<int>(threadIdx.x) / tpr;                \
+    const int sub = static_cast<int>(threadIdx.x) & (tpr - 1);                                    \
+    (void)sub;
+
+// First two Jacobi sweeps from x = 0: v[out] = x1 + wd (b - A x1), x1 = wd b (sliced-ELL
+// operators; the gathers of wd and b stay cache-resident for short rows).
 template <class T>
 __global__ void __launch_bounds__(k_block) k_presmooth2(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
-    CFDG_BLOCK_PROLOGUE(sys, blocks)
-    if (S.st->status != ST_RUNNING) {
-        return;
-    }
-    const LevelDev& L = S.lv[level];
-    if (L.coarsest || slice >= L.a.n_slices) {
-        return;
-    }
+    CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
     const T* b = static_cast<const T*>(L.b);
     const T* wd = static_cast<const T*>(L.wd);
-    const T ax = ell_row_x1<T>(L.a, slice, lane, wd, b);
-    if (row < L.n) {
+    const T ax = mat_row_x1<T>(M, row, sub, wd, b);
+    if (sub == 0 && row < L.n) {
         const T x1 = wd[row] * b[row];
         static_cast<T*>(L.v[out])[row] = x1 + wd[row] * (b[row] - ax);
     }
 }
 
-// One Jacobi sweep from x = 0: v[out] = wd b.
+// One Jacobi sweep from x = 0: v[out] = wd b (row-wise, 256 rows per workgroup).
 template <class T>
 __global__ void __launch_bounds__(k_block) k_jacobi_zero(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
@@ -505,7 +582,7 @@ __global__ void __launch_bounds__(k_block) k_jacobi_zero(const SysDev* __restric
     static_cast<T*>(L.v[out])[row] = wd[row] * b[row];
 }
 
-// v[out] = 0 (smoothing-free V-cycles).
+// v[out] = 0 (smoothing-free V-cycles; row-wise).
 template <class T>
 __global__ void __launch_bounds__(k_block) k_zero_level(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
@@ -519,37 +596,28 @@ __global__ void __launch_bounds__(k_block) k_zero_level(const SysDev* __restrict
     static_cast<T*>(L.v[out])[row] = T(0);
 }
 
-// Jacobi sweep v[out] = v[in] + wd (b - A v[in]). Dots: 0 none; 1 partial r.z with r = b
-// (FP64 level 0, where b aliases r); 2 partials r.z and z.q with r, q the FP64 outer vectors
-// (mixed level 0).
+// Jacobi sweep v[out] = v[in] + wd (b - A v[in]). Dots (level 0 only, sliced ELL): 0 none;
+// 1 partial r.z with r = b (FP64 level 0, where b aliases r); 2 partials r.z and z.q with r, q
+// the FP64 outer vectors (mixed level 0).
 template <class T, int Dots>
 __global__ void __launch_bounds__(k_block) k_jacobi(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in, int out) {
-    CFDG_BLOCK_PROLOGUE(sys, blocks)
-    if (S.st->status != ST_RUNNING) {
-        return;
-    }
-    const LevelDev& L = S.lv[level];
-    if (L.coarsest) {
-        return;
-    }
+    CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
     double d1 = 0.0;
     double d2 = 0.0;
-    if (slice < L.a.n_slices) {
-        const T* x = static_cast<const T*>(L.v[in]);
-        const T* b = static_cast<const T*>(L.b);
-        const T* wd = static_cast<const T*>(L.wd);
-        const T ax = ell_row<T>(L.a, slice, lane, x);
-        if (row < L.n) {
-            const T z = x[row] + wd[row] * (b[row] - ax);
-            static_cast<T*>(L.v[out])[row] = z;
-            if (Dots == 1) {
-                d1 = static_cast<double>(b[row]) * static_cast<double>(z);
-            }
-            if (Dots == 2) {
-                const double zd = static_cast<double>(z);
-                d1 = S.r[row] * zd;
-                d2 = zd * S.q[row];
-            }
+    const T* x = static_cast<const T*>(L.v[in]);
+    const T* b = static_cast<const T*>(L.b);
+    const T* wd = static_cast<const T*>(L.wd);
+    const T ax = mat_row<T>(M, row, sub, x);
+    if (sub == 0 && row < L.n) {
+        const T z = x[row] + wd[row] * (b[row] - ax);
+        static_cast<T*>(L.v[out])[row] = z;
+        if (Dots == 1) {
+            d1 = static_cast<double>(b[row]) * static_cast<double>(z);
+        }
+        if (Dots == 2) {
+            const double zd = static_cast<double>(z);
+            d1 = S.r[row] * zd;
+            d2 = zd * S.q[row];
         }
     }
     if (Dots == 1) {
@@ -574,34 +642,20 @@ __global__ void __launch_bounds__(k_block) k_jacobi(const SysDev* __restrict__ s
 // v[out] = b - A v[in].
 template <class T>
 __global__ void __launch_bounds__(k_block) k_residual(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in, int out) {
-    CFDG_BLOCK_PROLOGUE(sys, blocks)
-    if (S.st->status != ST_RUNNING) {
-        return;
-    }
-    const LevelDev& L = S.lv[level];
-    if (L.coarsest || slice >= L.a.n_slices) {
-        return;
-    }
+    CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
     const T* x = static_cast<const T*>(L.v[in]);
-    const T ax = ell_row<T>(L.a, slice, lane, x);
-    if (row < L.n) {
+    const T ax = mat_row<T>(M, row, sub, x);
+    if (sub == 0 && row < L.n) {
         static_cast<T*>(L.v[out])[row] = static_cast<const T*>(L.b)[row] - ax;
     }
 }
 
-// b_{l+1} = R_l v_l[in]; blocks index the rows of level l + 1.
+// b_{l+1} = R_l v_l[in]; rows of level l + 1.
 template <class T>
 __global__ void __launch_bounds__(k_block) k_restrict(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in) {
-    CFDG_BLOCK_PROLOGUE(sys, blocks)
-    if (S.st->status != ST_RUNNING) {
-        return;
-    }
-    const LevelDev& L = S.lv[level];
-    if (L.coarsest || slice >= L.r.n_slices) {
-        return;
-    }
-    const T y = ell_row<T>(L.r, slice, lane, static_cast<const T*>(L.v[in]));
-    if (row < L.n_next) {
+    CFDG_LEVEL_PROLOGUE(sys, blocks, level, r)
+    const T y = mat_row<T>(M, row, sub, static_cast<const T*>(L.v[in]));
+    if (sub == 0 && row < L.n_next) {
         static_cast<T*>(S.lv[level + 1].b)[row] = y;
     }
 }
@@ -610,16 +664,9 @@ __global__ void __launch_bounds__(k_block) k_restrict(const SysDev* __restrict__
 template <class T>
 __global__ void __launch_bounds__(k_block) k_prolong(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int io,
                           int coarse_buf) {
-    CFDG_BLOCK_PROLOGUE(sys, blocks)
-    if (S.st->status != ST_RUNNING) {
-        return;
-    }
-    const LevelDev& L = S.lv[level];
-    if (L.coarsest || slice >= L.p.n_slices) {
-        return;
-    }
-    const T t = ell_row<T>(L.p, slice, lane, static_cast<const T*>(S.lv[level + 1].v[coarse_buf]));
-    if (row < L.n) {
+    CFDG_LEVEL_PROLOGUE(sys, blocks, level, p)
+    const T t = mat_row<T>(M, row, sub, static_cast<const T*>(S.lv[level + 1].v[coarse_buf]));
+    if (sub == 0 && row < L.n) {
         T* x = static_cast<T*>(L.v[io]);
         x[row] = x[row] + t;
     }
@@ -629,7 +676,7 @@ __global__ void __launch_bounds__(k_block) k_prolong(const SysDev* __restrict__ 
 // index groups of four rows.
 template <class T>
 __global__ void __launch_bounds__(k_block) k_coarse(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
-    const int2 bk = blocks[blockIdx.x];
+    const int2 bk = blocks[xcd_block()];
     const SysDev& S = sys[bk.x];
     if (S.st->status != ST_RUNNING) {
         return;
@@ -657,11 +704,13 @@ __global__ void __launch_bounds__(k_block) k_coarse(const SysDev* __restrict__ s
 // ------------------------------------------------------------------------------------------
 // Finalize kernels: one workgroup per system, partials added in a fixed order.
 // ------------------------------------------------------------------------------------------
+constexpr int k_fin_block = 1024;
+
 __device__ __forceinline__ double reduce_slot(const SysDev& S, int slot) {
-    __shared__ double red[k_slices_per_block];
+    __shared__ double red[k_fin_block / k_wave];
     const double* p = S.partials + static_cast<long long>(slot) * S.nblk0;
     double s = 0.0;
-    for (int i = threadIdx.x; i < S.nblk0; i += k_block) {
+    for (int i = threadIdx.x; i < S.nblk0; i += k_fin_block) {
         s += p[i];
     }
     s = wave_sum(s);
@@ -671,7 +720,7 @@ __device__ __forceinline__ double reduce_slot(const SysDev& S, int slot) {
     __syncthreads();
     double t = red[0];
 #pragma unroll
-    for (int w = 1; w < k_slices_per_block; ++w) {
+    for (int w = 1; w < k_fin_block / k_wave; ++w) {
         t += red[w];
     }
     __syncthreads();
@@ -696,7 +745,7 @@ enum FinOp : int {
 };
 
 template <int Op>
-__global__ void __launch_bounds__(k_block) k_finalize(const SysDev* __restrict__ sys, const int* __restrict__ ids) {
+__global__ void __launch_bounds__(k_fin_block) k_finalize(const SysDev* __restrict__ sys, const int* __restrict__ ids) {
     const SysDev& S = sys[ids[blockIdx.x]];
     SolveState* st = S.st;
     const int status = st->status;

@@ -52,11 +52,17 @@ struct SolveState {
     int pad;
 };
 
+// A sparse matrix as the kernels see it. tpr == 1: sliced ELL, one thread per row, slices of 64
+// rows (ptr = slice offsets, entry k of row 64 s + lane at ptr[s] + 64 k + lane). tpr > 1: CSR
+// (ptr = row offsets) with tpr consecutive lanes per row, used for the long rows of coarse
+// operators and restrictions. A workgroup covers 256 / tpr rows.
 struct EllView {
     int n_rows;
-    int n_slices;
-    const int* sp;   // n_slices + 1 element offsets
-    const int* col;  // sliced column indices
+    int tpr;
+    int n_slices;    // ELL only
+    int pad;
+    const int* sp;   // ELL slice offsets or CSR row offsets
+    const int* col;
     const void* val; // double or float values
 };
 

@@ -306,7 +306,7 @@ void System::set_amg(const cfdg_amg_view& v) {
         const bool coarsest = l == L - 1;
         if (l >= 1 && !coarsest) {
             const cfdg_csr& a = v.levels[l].a;
-            d.a.upload(csr_to_ell(d.n, d.n, a.row_ptr, a.col), a.val, true, true, s);
+            d.a.upload_csr(d.n, d.n, a.row_ptr, a.col, a.val, s);
             const std::vector<double> dinv = inverse_diagonal(a);
             std::vector<double> wd(dinv.size());
             std::vector<float> wd32(dinv.size());
@@ -320,8 +320,8 @@ void System::set_amg(const cfdg_amg_view& v) {
         if (!coarsest) {
             const cfdg_csr& p = v.levels[l].p;
             const cfdg_csr& r = v.levels[l].r;
-            d.p.upload(csr_to_ell(d.n, d.n_next, p.row_ptr, p.col), p.val, true, true, s);
-            d.r.upload(csr_to_ell(d.n_next, d.n, r.row_ptr, r.col), r.val, true, true, s);
+            d.p.upload_csr(d.n, d.n_next, p.row_ptr, p.col, p.val, s);
+            d.r.upload_csr(d.n_next, d.n, r.row_ptr, r.col, r.val, s);
         } else {
             d.cinv.upload(v.coarse_inverse, static_cast<std::size_t>(d.n) * static_cast<std::size_t>(d.n), s);
         }
@@ -455,6 +455,11 @@ struct Tables {
     std::vector<int> cn;
     DevBuf<int2> t0_single;        // level-0 blocks of one-level hierarchies
     int t0_single_n = 0;
+    // per level and matrix (0 operator, 1 prolongation, 2 restriction): blocks of 256 / tpr
+    // rows of that matrix for the systems whose level l is not the coarsest
+    std::vector<DevBuf<int2>> tm[3];
+    std::vector<int> tmn[3];
+    std::vector<char> a_ell; // every system's level-l operator is sliced ELL (fused presmooth)
 };
 
 void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hipStream_t s) {
@@ -508,6 +513,37 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
     }
     tb.t0_single_n = static_cast<int>(single.size());
     tb.t0_single.upload(single.data(), single.size(), s);
+    for (int which = 0; which < 3; ++which) {
+        tb.tm[which].clear();
+        tb.tm[which].resize(static_cast<std::size_t>(tb.l_max));
+        tb.tmn[which].assign(static_cast<std::size_t>(tb.l_max), 0);
+    }
+    tb.a_ell.assign(static_cast<std::size_t>(tb.l_max), 1);
+    if (!use_amg) {
+        return;
+    }
+    for (int l = 0; l + 1 < tb.l_max; ++l) {
+        for (int which = 0; which < 3; ++which) {
+            std::vector<int2> t;
+            for (int q = 0; q < tb.n_sys; ++q) {
+                System* x = sys[static_cast<std::size_t>(q)];
+                if (l + 1 >= x->n_levels()) {
+                    continue; // level l is this system's coarsest (or absent)
+                }
+                const int tpr = x->tpr(l, which);
+                if (which == 0 && tpr != 1) {
+                    tb.a_ell[static_cast<std::size_t>(l)] = 0;
+                }
+                const int rows = which == 2 ? x->level_rows(l + 1) : x->level_rows(l);
+                const int nb = ceil_div(static_cast<long long>(rows) * tpr, k_block);
+                for (int b = 0; b < nb; ++b) {
+                    t.push_back(make_int2(q, b));
+                }
+            }
+            tb.tmn[which][static_cast<std::size_t>(l)] = static_cast<int>(t.size());
+            tb.tm[which][static_cast<std::size_t>(l)].upload(t.data(), t.size(), s);
+        }
+    }
 }
 
 #define CFDG_LAUNCH(kernel, grid, stream, ...)                                                    \
@@ -556,43 +592,50 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
     CFDG_LAUNCH(k_coarse<T>, tb.cn[0], s, d, tb.c[0].get(), 0, fin);
     std::vector<int> cur(static_cast<std::size_t>(L), 0);
     for (int l = 0; l + 1 < L; ++l) {
-        const int g = tb.tn[static_cast<std::size_t>(l)];
-        const int2* blk = tb.t[static_cast<std::size_t>(l)].get();
+        const std::size_t li = static_cast<std::size_t>(l);
+        const int g = tb.tn[li];                        // row-wise blocks
+        const int2* blk = tb.t[li].get();
+        const int ga = tb.tmn[0][li];                   // operator blocks
+        const int2* ba = tb.tm[0][li].get();
         int c = 0;
         if (sweeps == 0) {
             CFDG_LAUNCH(k_zero_level<T>, g, s, d, blk, l, 0);
         } else if (sweeps == 1) {
             CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 0);
         } else {
-            CFDG_LAUNCH(k_presmooth2<T>, g, s, d, blk, l, 0);
+            if (tb.a_ell[li] != 0) {
+                CFDG_LAUNCH(k_presmooth2<T>, ga, s, d, ba, l, 0);
+            } else { // long rows: form x1 = wd b once instead of gathering it per entry
+                CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 2);
+                CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, 2, 0);
+            }
             for (int k = 2; k < sweeps; ++k) {
                 const int o = c == 0 ? 2 : 0;
-                CFDG_LAUNCH((k_jacobi<T, 0>), g, s, d, blk, l, c, o);
+                CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, c, o);
                 c = o;
             }
         }
-        cur[static_cast<std::size_t>(l)] = c;
-        CFDG_LAUNCH(k_residual<T>, g, s, d, blk, l, c, 1);
-        CFDG_LAUNCH(k_restrict<T>, tb.tn[static_cast<std::size_t>(l) + 1], s, d,
-                    tb.t[static_cast<std::size_t>(l) + 1].get(), l, 1);
-        CFDG_LAUNCH(k_coarse<T>, tb.cn[static_cast<std::size_t>(l) + 1], s, d,
-                    tb.c[static_cast<std::size_t>(l) + 1].get(), l + 1, fin);
+        cur[li] = c;
+        CFDG_LAUNCH(k_residual<T>, ga, s, d, ba, l, c, 1);
+        CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1);
+        CFDG_LAUNCH(k_coarse<T>, tb.cn[li + 1], s, d, tb.c[li + 1].get(), l + 1, fin);
     }
     for (int l = L - 2; l >= 0; --l) {
-        const int g = tb.tn[static_cast<std::size_t>(l)];
-        const int2* blk = tb.t[static_cast<std::size_t>(l)].get();
-        const int c0 = cur[static_cast<std::size_t>(l)];
-        CFDG_LAUNCH(k_prolong<T>, g, s, d, blk, l, c0, fin);
+        const std::size_t li = static_cast<std::size_t>(l);
+        const int ga = tb.tmn[0][li];
+        const int2* ba = tb.tm[0][li].get();
+        const int c0 = cur[li];
+        CFDG_LAUNCH(k_prolong<T>, tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
         int c = c0;
         for (int k = 0; k < sweeps; ++k) {
             const int o = c == 1 ? c0 : 1;
             const bool last = k == sweeps - 1;
             if (l == 0 && last) {
                 probe_begin(PROBE_L0_SWEEP, s);
-                CFDG_LAUNCH((k_jacobi<T, Dots>), g, s, d, blk, l, c, o);
+                CFDG_LAUNCH((k_jacobi<T, Dots>), ga, s, d, ba, l, c, o);
                 probe_end(PROBE_L0_SWEEP, s);
             } else {
-                CFDG_LAUNCH((k_jacobi<T, 0>), g, s, d, blk, l, c, o);
+                CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, c, o);
             }
             c = o;
         }
@@ -606,7 +649,7 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
 
 template <int Op>
 void fin(const SysDev* d, const Tables& tb, hipStream_t s) {
-    k_finalize<Op><<<tb.n_sys, k_block, 0, s>>>(d, tb.ids.get());
+    k_finalize<Op><<<tb.n_sys, k_fin_block, 0, s>>>(d, tb.ids.get());
 }
 
 enum VecId { V_X = 0, V_R, V_P, V_Q, V_Z, V_R0, V_V, V_S, V_T, V_PH, V_SH };
@@ -693,123 +736,248 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
 
 } // namespace
 
+// Everything a set of systems needs to be solved together, built once and reused: launch
+// tables, device descriptors, and (unless kernels are being probed) HIP graphs of the solver's
+// start-up sequence and of one iteration. Iteration k of the graph pipeline writes the number
+// of still-running systems into pinned host memory slot k mod 2 from the device; the host waits
+// for the previous slot while the next iteration is already queued.
+class Runner {
+public:
+    Runner(const std::vector<System*>& systems, const BatchOptions& opt, hipStream_t s)
+        : sys_(systems), s_(s), running_(2), init_host_(systems.size()), out_host_(systems.size()) {
+        require(!sys_.empty(), "solve_batch: no systems");
+        const bool amg = opt.use_amg && opt.method == CFDG_METHOD_CG;
+        const bool fp32 = amg && opt.precision != CFDG_PREC_FP64;
+        int sweeps = 2;
+        for (std::size_t i = 0; i < sys_.size(); ++i) {
+            if (amg) {
+                require(sys_[i]->has_amg(), "solve: use_amg requested but no hierarchy was set");
+                if (i == 0) {
+                    sweeps = sys_[i]->sweeps();
+                }
+                require(sys_[i]->sweeps() == sweeps, "solve_batch: systems differ in smooth_sweeps");
+            }
+            gens_.push_back(sys_[i]->generation());
+        }
+        pl_ = Plan{opt.method, amg, fp32, sweeps};
+        build_tables(tb_, sys_, amg, s_);
+        std::vector<SysDev> desc(sys_.size());
+        for (std::size_t i = 0; i < sys_.size(); ++i) {
+            desc[i] = sys_[i]->descriptor(fp32, amg);
+        }
+        desc_.upload(desc.data(), desc.size(), s_);
+        d_init_.resize(sys_.size());
+        d_out_.resize(sys_.size());
+        void* dev = nullptr;
+        hip_check(hipHostGetDevicePointer(&dev, running_.get(), 0), "hipHostGetDevicePointer");
+        running_dev_ = static_cast<int*>(dev);
+        use_graphs_ = std::getenv("CFDG_NO_GRAPHS") == nullptr;
+    }
+    Runner(const Runner&) = delete;
+    Runner& operator=(const Runner&) = delete;
+    ~Runner() {
+        for (hipGraphExec_t g : {init_exec_, iter_exec_[0], iter_exec_[1]}) {
+            if (g != nullptr) {
+                (void)hipGraphExecDestroy(g);
+            }
+        }
+    }
+
+    [[nodiscard]] bool matches(const std::vector<System*>& systems, const BatchOptions& opt) const {
+        if (systems != sys_) {
+            return false;
+        }
+        for (std::size_t i = 0; i < sys_.size(); ++i) {
+            if (sys_[i]->generation() != gens_[i]) {
+                return false;
+            }
+        }
+        const bool amg = opt.use_amg && opt.method == CFDG_METHOD_CG;
+        return opt.method == pl_.method && amg == pl_.amg &&
+               (amg && opt.precision != CFDG_PREC_FP64) == pl_.fp32;
+    }
+
+    void solve(const std::vector<cfdg_solve_control>& control, std::vector<cfdg_solve_stats>& stats,
+               Probe* probe) {
+        require(control.size() == sys_.size(), "solve_batch: one control per system");
+        const int n_sys = static_cast<int>(sys_.size());
+        int max_it = 0;
+        for (std::size_t i = 0; i < sys_.size(); ++i) {
+            require(control[i].max_iterations >= 0, "solve: max_iterations must be >= 0");
+            max_it = std::max(max_it, control[i].max_iterations);
+            SolveState& st = init_host_[i];
+            std::memset(&st, 0, sizeof(st));
+            st.tol = control[i].abs_tolerance;
+            st.max_it = control[i].max_iterations;
+            st.status = ST_RUNNING;
+        }
+        const SysDev* d = desc_.get();
+        struct ProbeScope { // the thread's probe for the duration of this solve
+            explicit ProbeScope(Probe* p) {
+                g_probe = p;
+                if (p != nullptr) {
+                    p->used = 0;
+                }
+            }
+            ~ProbeScope() { g_probe = nullptr; }
+        } probe_scope(probe);
+        // Graphs from the second solve of a runner on: one-shot solves skip the capture cost.
+        const bool graphs = use_graphs_ && probe == nullptr && solves_ > 0;
+        ++solves_;
+        if (graphs && init_exec_ == nullptr) {
+            capture_graphs();
+        }
+        running_[0] = running_[1] = 1;
+        hip_check(hipEventRecord(t0_.get(), s_), "event record");
+        hip_check(hipMemcpyAsync(d_init_.get(), init_host_.get(), sys_.size() * sizeof(SolveState),
+                                 hipMemcpyHostToDevice, s_),
+                  "H2D states");
+        k_init_states<<<ceil_div(n_sys, 64), 64, 0, s_>>>(d, n_sys, d_init_.get());
+        if (graphs) {
+            hip_check(hipGraphLaunch(init_exec_, s_), "graph launch");
+        } else {
+            enqueue_init(pl_, d, tb_, s_);
+        }
+        hip_check(hipGetLastError(), "init launch");
+        // Iterations are enqueued one ahead of the convergence read-back; surplus iterations are
+        // no-ops on the device because every kernel tests the system's status first.
+        for (int k = 1; k <= max_it + 2; ++k) {
+            const int slot = k & 1;
+            if (graphs) {
+                hip_check(hipGraphLaunch(iter_exec_[slot], s_), "graph launch");
+            } else {
+                enqueue_iteration(pl_, d, tb_, s_);
+                k_count_running<<<1, k_block, 0, s_>>>(d, n_sys, running_dev_ + slot);
+            }
+            hip_check(hipEventRecord(ev_[slot].get(), s_), "event record");
+            hip_check(hipGetLastError(), "iteration launch");
+            if (k >= 2) {
+                const int prev = (k - 1) & 1;
+                hip_check(hipEventSynchronize(ev_[prev].get()), "event sync");
+                if (running_[static_cast<std::size_t>(prev)] == 0) {
+                    break;
+                }
+            }
+        }
+        CFDG_LAUNCH(k_true_residual, tb_.tn[0], s_, d, tb_.t[0].get());
+        fin<FIN_TRUE>(d, tb_, s_);
+        k_gather_states<<<ceil_div(n_sys, 64), 64, 0, s_>>>(d, n_sys, d_out_.get());
+        hip_check(hipEventRecord(t1_.get(), s_), "event record");
+        hip_check(hipMemcpyAsync(out_host_.get(), d_out_.get(), sys_.size() * sizeof(SolveState),
+                                 hipMemcpyDeviceToHost, s_),
+                  "D2H states");
+        hip_check(hipStreamSynchronize(s_), "solve sync");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, t0_.get(), t1_.get()), "event elapsed");
+        stats.assign(sys_.size(), cfdg_solve_stats{});
+        for (std::size_t i = 0; i < sys_.size(); ++i) {
+            const SolveState& st = out_host_[i];
+            cfdg_solve_stats& o = stats[i];
+            o.iterations = std::min(st.it, st.max_it);
+            o.converged = st.status == ST_CONVERGED ? 1 : 0;
+            o.on_device = 1;
+            o.exit_reason = st.status;
+            o.residual = st.res;
+            o.true_residual = st.true_res;
+            o.rhs_norm = st.bnorm;
+            o.solve_ms = ms;
+        }
+        if (probe != nullptr) {
+            // Launches that did work: the initial V-cycle plus one per iteration that
+            // preconditioned (level-0 sweep), or one per iteration (outer SpMV).
+            const int its = stats[0].iterations;
+            std::size_t useful = static_cast<std::size_t>(its);
+            if (probe->kind == PROBE_L0_SWEEP && stats[0].exit_reason != ST_CONVERGED) {
+                useful = static_cast<std::size_t>(its) + 1;
+            }
+            useful = std::min(useful, probe->used);
+            for (std::size_t i = 0; i < useful; ++i) {
+                float e = 0.0f;
+                hip_check(hipEventElapsedTime(&e, probe->events[i].first, probe->events[i].second),
+                          "event elapsed");
+                probe->total_ms += e;
+            }
+            probe->launches += static_cast<long long>(useful);
+        }
+    }
+
+private:
+    void capture_graphs() {
+        const SysDev* d = desc_.get();
+        const int n_sys = static_cast<int>(sys_.size());
+        auto capture = [&](auto&& enqueue) {
+            hipGraph_t g = nullptr;
+            hip_check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "begin capture");
+            enqueue();
+            hip_check(hipStreamEndCapture(s_, &g), "end capture");
+            hipGraphExec_t e = nullptr;
+            hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "graph instantiate");
+            hip_check(hipGraphDestroy(g), "graph destroy");
+            return e;
+        };
+        init_exec_ = capture([&] { enqueue_init(pl_, d, tb_, s_); });
+        for (int slot = 0; slot < 2; ++slot) {
+            iter_exec_[slot] = capture([&] {
+                enqueue_iteration(pl_, d, tb_, s_);
+                k_count_running<<<1, k_block, 0, s_>>>(d, n_sys, running_dev_ + slot);
+            });
+        }
+    }
+
+    std::vector<System*> sys_;
+    std::vector<unsigned> gens_;
+    hipStream_t s_;
+    Plan pl_{};
+    Tables tb_;
+    DevBuf<SysDev> desc_;
+    DevBuf<SolveState> d_init_;
+    DevBuf<SolveState> d_out_;
+    PinnedBuf<int> running_;
+    int* running_dev_ = nullptr;
+    PinnedBuf<SolveState> init_host_;
+    PinnedBuf<SolveState> out_host_;
+    Event ev_[2];
+    Event t0_;
+    Event t1_;
+    bool use_graphs_ = true;
+    long long solves_ = 0;
+    hipGraphExec_t init_exec_ = nullptr;
+    hipGraphExec_t iter_exec_[2] = {nullptr, nullptr};
+};
+
+System::~System() = default;
+
+std::unique_ptr<Runner>& System::runner_slot(cfdg_method method, bool use_amg,
+                                             cfdg_precision precision) {
+    int k = 0;
+    if (method == CFDG_METHOD_BICGSTAB) {
+        k = 3;
+    } else if (!use_amg) {
+        k = 2;
+    } else if (precision != CFDG_PREC_FP64) {
+        k = 1;
+    }
+    return runners_[k];
+}
+
 void solve_batch(const std::vector<System*>& systems, const BatchOptions& opt,
                  const std::vector<cfdg_solve_control>& control,
                  std::vector<cfdg_solve_stats>& stats, hipStream_t s) {
     require(!systems.empty(), "solve_batch: no systems");
-    require(control.size() == systems.size(), "solve_batch: one control per system");
-    const bool amg = opt.use_amg && opt.method == CFDG_METHOD_CG;
-    const bool fp32 = amg && opt.precision != CFDG_PREC_FP64;
-    int sweeps = 2;
-    int max_it = 0;
-    for (std::size_t i = 0; i < systems.size(); ++i) {
-        require(control[i].max_iterations >= 0, "solve: max_iterations must be >= 0");
-        max_it = std::max(max_it, control[i].max_iterations);
-        if (amg) {
-            require(systems[i]->has_amg(), "solve: use_amg requested but no hierarchy was set");
-            if (i == 0) {
-                sweeps = systems[i]->sweeps();
-            }
-            require(systems[i]->sweeps() == sweeps, "solve_batch: systems differ in smooth_sweeps");
-        }
-    }
-    Tables tb;
-    build_tables(tb, systems, amg, s);
-    std::vector<SysDev> desc(systems.size());
-    std::vector<SolveState> init(systems.size());
-    for (std::size_t i = 0; i < systems.size(); ++i) {
-        desc[i] = systems[i]->descriptor(fp32, amg);
-        SolveState& st = init[i];
-        std::memset(&st, 0, sizeof(st));
-        st.tol = control[i].abs_tolerance;
-        st.max_it = control[i].max_iterations;
-        st.status = ST_RUNNING;
-    }
-    DevBuf<SysDev> d_desc;
-    d_desc.upload(desc.data(), desc.size(), s);
-    DevBuf<SolveState> d_init;
-    d_init.upload(init.data(), init.size(), s);
-    const int n_sys = static_cast<int>(systems.size());
-    const SysDev* d = d_desc.get();
     Probe* probe = (systems.size() == 1 && systems[0]->probe.kind != PROBE_OFF) ? &systems[0]->probe
                                                                              : nullptr;
-    struct ProbeScope { // the thread's probe for the duration of this solve
-        explicit ProbeScope(Probe* p) { g_probe = p; if (p != nullptr) { p->used = 0; } }
-        ~ProbeScope() { g_probe = nullptr; }
-    } probe_scope(probe);
-    const Plan pl{opt.method, amg, fp32, sweeps};
-
-    PinnedBuf<int> running(2);
-    running[0] = running[1] = 1;
-    DevBuf<int> d_running(2);
-    Event ev[2];
-    Event t0;
-    Event t1;
-    hip_check(hipEventRecord(t0.get(), s), "event record");
-    k_init_states<<<ceil_div(n_sys, 64), 64, 0, s>>>(d, n_sys, d_init.get());
-    enqueue_init(pl, d, tb, s);
-    hip_check(hipGetLastError(), "init launch");
-    // Iterations are enqueued one ahead of the convergence read-back; surplus iterations are
-    // no-ops on the device because every kernel tests the system's status first.
-    for (int k = 1; k <= max_it + 2; ++k) {
-        enqueue_iteration(pl, d, tb, s);
-        const int slot = k & 1;
-        k_count_running<<<1, k_block, 0, s>>>(d, n_sys, d_running.get() + slot);
-        hip_check(hipMemcpyAsync(running.get() + slot, d_running.get() + slot, sizeof(int),
-                                 hipMemcpyDeviceToHost, s),
-                  "D2H status");
-        hip_check(hipEventRecord(ev[slot].get(), s), "event record");
-        hip_check(hipGetLastError(), "iteration launch");
-        if (k >= 2) {
-            const int prev = (k - 1) & 1;
-            hip_check(hipEventSynchronize(ev[prev].get()), "event sync");
-            if (running[static_cast<std::size_t>(prev)] == 0) {
-                break;
-            }
+    if (systems.size() == 1) { // single system: keep the runner with the system
+        System& sys = *systems[0];
+        std::unique_ptr<Runner>& r = sys.runner_slot(opt.method, opt.use_amg, opt.precision);
+        if (!r || !r->matches(systems, opt)) {
+            r.reset();
+            r = std::make_unique<Runner>(systems, opt, s);
         }
+        r->solve(control, stats, probe);
+        return;
     }
-    CFDG_LAUNCH(k_true_residual, tb.tn[0], s, d, tb.t[0].get());
-    fin<FIN_TRUE>(d, tb, s);
-    DevBuf<SolveState> d_out(systems.size());
-    k_gather_states<<<ceil_div(n_sys, 64), 64, 0, s>>>(d, n_sys, d_out.get());
-    hip_check(hipEventRecord(t1.get(), s), "event record");
-    std::vector<SolveState> out(systems.size());
-    hip_check(hipMemcpyAsync(out.data(), d_out.get(), out.size() * sizeof(SolveState),
-                             hipMemcpyDeviceToHost, s),
-              "D2H states");
-    hip_check(hipStreamSynchronize(s), "solve sync");
-    float ms = 0.0f;
-    hip_check(hipEventElapsedTime(&ms, t0.get(), t1.get()), "event elapsed");
-    stats.assign(systems.size(), cfdg_solve_stats{});
-    for (std::size_t i = 0; i < systems.size(); ++i) {
-        const SolveState& st = out[i];
-        cfdg_solve_stats& o = stats[i];
-        o.iterations = std::min(st.it, st.max_it);
-        o.converged = st.status == ST_CONVERGED ? 1 : 0;
-        o.on_device = 1;
-        o.exit_reason = st.status;
-        o.residual = st.res;
-        o.true_residual = st.true_res;
-        o.rhs_norm = st.bnorm;
-        o.solve_ms = ms;
-    }
-    if (probe != nullptr) {
-        // Launches that did work: the initial V-cycle plus one per iteration that preconditioned
-        // (level-0 sweep), or one per iteration (outer SpMV).
-        const int its = stats[0].iterations;
-        std::size_t useful = static_cast<std::size_t>(its);
-        if (probe->kind == PROBE_L0_SWEEP && stats[0].exit_reason != ST_CONVERGED) {
-            useful = static_cast<std::size_t>(its) + 1;
-        }
-        useful = std::min(useful, probe->used);
-        for (std::size_t i = 0; i < useful; ++i) {
-            float e = 0.0f;
-            hip_check(hipEventElapsedTime(&e, probe->events[i].first, probe->events[i].second),
-                      "event elapsed");
-            probe->total_ms += e;
-        }
-        probe->launches += static_cast<long long>(useful);
-    }
+    Runner r(systems, opt, s);
+    r.solve(control, stats, nullptr);
 }
 
 } // namespace cfdg

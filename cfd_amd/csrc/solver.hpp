@@ -48,11 +48,16 @@ struct Probe {
     }
 };
 
+class Runner;
+
 // A linear system resident in HBM: level-0 operator (FP64 + FP32 values), right-hand side,
 // iterate, Krylov work vectors, and optionally an AMG hierarchy.
 class System {
 public:
+    ~System();
     Probe probe;
+    // Cached solve machinery (launch tables, descriptors, graphs) per method / precision.
+    std::unique_ptr<Runner>& runner_slot(cfdg_method method, bool use_amg, cfdg_precision precision);
     [[nodiscard]] long long nnz0() const { return nnz0_; }
     explicit System(const cfdg_ldu_pattern& pattern); // values assembled later from LDU
     explicit System(const cfdg_csr& a);               // values given
@@ -75,6 +80,14 @@ public:
     [[nodiscard]] hipStream_t stream() const { return stream_.get(); }
     [[nodiscard]] SolveState* state() const { return state_.get(); }
     [[nodiscard]] int level_rows(int l) const { return levels_[static_cast<std::size_t>(l)].n; }
+    // Lanes per row of level l's operator (which 0), prolongation (1) or restriction (2).
+    [[nodiscard]] int tpr(int l, int which) const {
+        const DevLevel& d = levels_[static_cast<std::size_t>(l)];
+        if (which == 0) {
+            return l == 0 ? 1 : d.a.tpr;
+        }
+        return which == 1 ? d.p.tpr : d.r.tpr;
+    }
 
     // Descriptor of this system for kernels of the given precision (FP64 or FP32 V-cycle).
     [[nodiscard]] SysDev descriptor(bool fp32_vcycle, bool use_amg);
@@ -106,6 +119,7 @@ private:
     int sweeps_ = 2;
     double weight_ = 0.7;
     unsigned generation_ = 0;
+    std::unique_ptr<Runner> runners_[4];
 };
 
 // Solve options shared by every system of a batch.
