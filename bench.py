@@ -75,6 +75,87 @@ def cpu_baseline(hier_view, b, tol, budget_s):
                       f"{int(np.median(iters))} iterations; reference threads loops with n > 20000"}
 
 
+def batched(steps: int, count: int = 256) -> dict:
+    """Config of 256 independent pressure systems (cell counts log-uniform in [4096, 500000],
+    half uniform and half jittered boxes, cell aspect 1:1:1 to 1:1:8, seeded zero-mean right-hand
+    sides) solved together to 1e-8: one launch per solver stage covers every system. Reported:
+    solves/s of the mixed (FP32 V-cycle, FP64 flexible CG) batch, of the FP64 batch, and of the
+    plain FP64 path solving the same systems one after another; the largest relative difference
+    between each mixed solution and its FP64 solution."""
+    import cfd_amd
+    from cfd_amd import AmgHierarchy, AmgSettings, DeviceSolver, Method, Precision, SolveControl, cases
+
+    t0 = time.perf_counter()
+    members, ctls = [], []
+    cells = 0
+    for s in cases.batch_shapes(count):
+        mesh = cases.box_mesh(s["nx"], s["ny"], s["nz"], hi=(1.0, 1.0, s["lz"]),
+                              jitter=s["jitter"], seed=s["seed"])
+        p = cases.pressure_system(mesh, rhs_seed=s["seed"] + 1)
+        pat = cases.csr_pattern(p.n, p.owner, p.neighbour)
+        a = cases.ldu_to_csr(pat, p.diag, p.upper, p.lower)
+        dev = DeviceSolver(pat)
+        dev.assemble(p.diag, p.upper, p.lower)
+        dev.set_amg(AmgHierarchy(a, AmgSettings(strength_decay=0.5)).view())
+        dev.set_rhs(p.source)
+        members.append(dev)
+        ctls.append(SolveControl(1e-8 * float(np.linalg.norm(p.source)), 500))
+        cells += p.n
+    log(f"batch: {count} systems, {cells} cells, set up in {time.perf_counter() - t0:.1f} s")
+
+    def timed_batch(prec):
+        batch = cfd_amd.Batch(members, Method.Cg, True, prec)
+        for rep in range(1 + steps):  # one untimed solve (graph capture), then the timed ones
+            if rep == 1:
+                t = time.perf_counter()
+            for d in members:
+                d.set_x(None)
+            st = batch.solve(ctls)
+        dt = time.perf_counter() - t
+        if not all(x.converged for x in st):
+            raise RuntimeError(f"batched {prec.name}: a system did not converge")
+        xs = [d.get_x() for d in members]
+        batch.close()
+        return count * steps / dt, dt / steps, st, xs
+
+    sps_m, t_m, st_m, x_m = timed_batch(Precision.MIXED_FP32)
+    sps_d, t_d, st_d, x_d = timed_batch(Precision.FP64)
+    sps_b, t_b, st_b, x_b = timed_batch(Precision.MIXED_BF16)
+    # Plain FP64 path: the same systems, one resident solve after another.
+    for d, c in zip(members, ctls):
+        d.set_x(None)
+        d.solve_resident(Method.Cg, True, Precision.FP64, c)
+    t = time.perf_counter()
+    for d, c in zip(members, ctls):
+        d.set_x(None)
+        d.solve_resident(Method.Cg, True, Precision.FP64, c)
+    t_seq = time.perf_counter() - t
+    err = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(x_m, x_d))
+    err_b = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(x_b, x_d))
+    footprint = sum(d.footprint() for d in members)
+    for d in members:
+        d.close()
+    its = lambda st: [int(min(x.iterations for x in st)), int(max(x.iterations for x in st))]
+    return {
+        "workload": f"{count} pressure systems, {cells} cells in all, log-uniform 4096..500000 "
+                    "cells, SA-AMG-PCG to 1e-8 rel. residual from x0 = 0",
+        "mixed_solves_per_s": round(sps_m, 2),
+        "fp64_batched_solves_per_s": round(sps_d, 2),
+        "fp64_one_by_one_solves_per_s": round(count / t_seq, 2),
+        "bf16_batched_solves_per_s": round(sps_b, 2),
+        "speedup_mixed_vs_fp64_batched": round(sps_m / sps_d, 3),
+        "speedup_mixed_vs_fp64_one_by_one": round(sps_m * t_seq / count, 3),
+        "ms_per_batch_mixed": round(1e3 * t_m, 2),
+        "ms_per_batch_fp64": round(1e3 * t_d, 2),
+        "iterations_mixed_min_max": its(st_m),
+        "iterations_fp64_min_max": its(st_d),
+        "max_true_rel_residual_mixed": max(x.true_residual / x.rhs_norm for x in st_m),
+        "max_rel_error_mixed_vs_fp64": err,
+        "max_rel_error_bf16_vs_fp64": err_b,
+        "footprint_bytes": footprint,
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +164,7 @@ def main(argv=None) -> int:
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--skip-batch", action="store_true")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -127,25 +209,37 @@ def main(argv=None) -> int:
         return np.array(lat), np.array(dev_ms), its, st
 
     run(Precision.MIXED_FP32, args.warmup)
-    run(Precision.FP64, max(1, args.warmup // 2))
-    # Timed region: HIP events on the solve stream bracket every launch of the dominant kernel
-    # (the last level-0 Jacobi sweep of the V-cycle, fused with the two FCG dot products).
-    dev.probe(1)
+    run(Precision.FP64, max(2, args.warmup))
+    # Timed region (the value): HIP-graph solves, no instrumentation.
     t_mixed = time.perf_counter()
     lat_m, dms_m, its_m, st_m = run(Precision.MIXED_FP32, args.steps)
     t_mixed = time.perf_counter() - t_mixed
-    probe_m = dev.probe_read(Precision.MIXED_FP32)
     x_mixed = dev.get_x()
-    dev.probe(1)
     t_fp64 = time.perf_counter()
     lat_d, dms_d, its_d, st_d = run(Precision.FP64, args.steps)
     t_fp64 = time.perf_counter() - t_fp64
+    x_fp64 = dev.get_x()
+    run(Precision.MIXED_BF16, max(2, args.warmup))
+    t_bf16 = time.perf_counter()
+    lat_b, dms_b, its_b, st_b = run(Precision.MIXED_BF16, args.steps)
+    t_bf16 = time.perf_counter() - t_bf16
+    x_bf16 = dev.get_x()
+    log(f"bf16: {args.steps / t_bf16:.2f} solves/s, it {its_b[-1]}, true rel res "
+        f"{st_b.true_residual / bnorm:.2e}")
+    # Instrumented pass over the same solves: HIP events on the solve stream bracket every
+    # launch of the dominant kernel (the last level-0 Jacobi sweep of the V-cycle, fused with the
+    # FCG dot products), and of the outer FP64 SpMV. Kernel-by-kernel launches, same kernels.
+    k_probe = max(2, args.steps // 2)
+    dev.probe(1)
+    run(Precision.MIXED_FP32, k_probe)
+    probe_m = dev.probe_read(Precision.MIXED_FP32)
+    dev.probe(1)
+    run(Precision.FP64, k_probe)
     probe_d = dev.probe_read(Precision.FP64)
     dev.probe(2)
-    run(Precision.FP64, 2)
+    run(Precision.FP64, k_probe)
     probe_spmv = dev.probe_read(Precision.FP64)
     dev.probe(0)
-    x_fp64 = dev.get_x()
 
     def roof(p, name):
         ms, launches, nbytes = p
@@ -210,6 +304,10 @@ def main(argv=None) -> int:
         "true_rel_residual_mixed": st_m.true_residual / bnorm,
         "true_rel_residual_fp64": st_d.true_residual / bnorm,
         "mixed_vs_fp64_rel_error": err,
+        "mixed_bf16": {"solves_per_s": round(args.steps / t_bf16, 3), "iterations": int(its_b[-1]),
+                       "p99_ms": round(1e3 * float(np.percentile(lat_b, 99)), 3),
+                       "true_rel_residual": st_b.true_residual / bnorm,
+                       "rel_error_vs_fp64": float(np.linalg.norm(x_bf16 - x_fp64) / np.linalg.norm(x_fp64))},
         "momentum_bicgstab_fp64": {"solves_per_s": round(k_mom / t_mom, 3),
                                    "iterations": st_mom.iterations,
                                    "true_rel_residual": st_mom.true_residual / np.linalg.norm(m_sys.source)},
@@ -221,6 +319,9 @@ def main(argv=None) -> int:
         "working_set_mb": round(dev.footprint() / 1e6, 1),
         "cpu_baseline": None,
     }
+    if not args.skip_batch:
+        del dev, mdev
+        out["batched"] = batched(max(2, args.steps // 4))
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(view, b, tol, args.cpu_seconds)
     print(json.dumps(out), flush=True)

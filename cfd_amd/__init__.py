@@ -9,12 +9,12 @@ from . import _lib
 _lib.lib()  # fail at import time when the native library is missing
 
 from .amg import AmgHierarchy, AmgSettings  # noqa: E402
-from .krylov import (AmgLevelView, AmgView, CsrView, DeviceSolver, LduPattern, Method,  # noqa: E402
+from .krylov import (AmgLevelView, AmgView, Batch, CsrView, DeviceSolver, LduPattern, Method,  # noqa: E402
                      Precision, SolveControl, SolveStats, assemble_convection_diffusion,
                      backend_name, device_available, device_description, solve, solve_amg_cg)
 
 __all__ = [
-    "AmgHierarchy", "AmgSettings", "AmgLevelView", "AmgView", "CsrView", "DeviceSolver",
+    "AmgHierarchy", "AmgSettings", "AmgLevelView", "AmgView", "Batch", "CsrView", "DeviceSolver",
     "LduPattern", "Method", "Precision", "SolveControl", "SolveStats",
     "assemble_convection_diffusion", "backend_name", "device_available", "device_description",
     "solve", "solve_amg_cg",

@@ -151,6 +151,10 @@ _SIGNATURES = {
     "cfdg_system_solve_resident": (ST, [VP, C.c_int, C.c_int, C.c_int, C.POINTER(SolveControl),
                                         C.POINTER(SolveStats)]),
     "cfdg_system_footprint": (ST, [VP, C.POINTER(C.c_size_t)]),
+    "cfdg_batch_create": (ST, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.POINTER(VP)]),
+    "cfdg_batch_destroy": (ST, [VP]),
+    "cfdg_batch_solve": (ST, [VP, C.POINTER(SolveControl), C.POINTER(SolveStats)]),
     "cfdg_system_probe": (ST, [VP, C.c_int]),
     "cfdg_system_probe_read": (ST, [VP, C.c_int, P_DBL, C.POINTER(C.c_longlong), P_DBL]),
     "cfdg_amg_default_settings": (ST, [C.POINTER(AmgSettings)]),

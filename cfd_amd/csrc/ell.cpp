@@ -34,7 +34,8 @@ HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci) {
         for (int lane = 0; lane < k_wave; ++lane) {
             const int r = s * k_wave + lane;
             const int len = r < n_rows ? rp[r + 1] - rp[r] : 0;
-            int pad = 0;
+            // rows past the end point at the last row of a square matrix (a short offset)
+            int pad = (n_rows == n_cols && n_rows > 0) ? n_rows - 1 : 0;
             if (r < n_rows) {
                 pad = (n_rows == n_cols) ? r : (len > 0 ? ci[rp[r] + len - 1] : 0);
             }
@@ -67,7 +68,7 @@ int pick_tpr(int n_rows, long long nnz) {
     // Tuning knobs (environment): rows with at most CFDG_ELL_MAX_AVG entries on average stay
     // thread-per-row ELL; longer rows get lanes so that each lane sums about
     // CFDG_TPR_ENTRIES entries.
-    static const double ell_max = env_or("CFDG_ELL_MAX_AVG", 12.0);
+    static const double ell_max = env_or("CFDG_ELL_MAX_AVG", 40.0);
     static const double per_lane = env_or("CFDG_TPR_ENTRIES", 4.0);
     const double avg = static_cast<double>(nnz) / static_cast<double>(n_rows);
     if (avg <= ell_max) {
@@ -86,7 +87,32 @@ void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hi
     n_slices = e.n_slices;
     tpr = 1;
     sp.upload(e.sp.data(), e.sp.size(), s);
-    col.upload(e.col.data(), e.col.size(), s);
+    // Square matrices whose columns all lie within 32767 of the row store 16-bit offsets.
+    bool short_ok = e.n_rows == e.n_cols;
+    std::vector<short> d;
+    if (short_ok) {
+        d.resize(e.col.size());
+        for (int sl = 0; sl < e.n_slices && short_ok; ++sl) {
+            const int base = e.sp[static_cast<std::size_t>(sl)];
+            const int end = e.sp[static_cast<std::size_t>(sl) + 1];
+            for (int k = base; k < end; ++k) {
+                const int row = sl * k_wave + (k - base) % k_wave;
+                const int off = e.col[static_cast<std::size_t>(k)] - row;
+                if (off < -32768 || off > 32767) {
+                    short_ok = false;
+                    break;
+                }
+                d[static_cast<std::size_t>(k)] = static_cast<short>(off);
+            }
+        }
+    }
+    if (short_ok && !std::getenv("CFDG_NO_SHORT_COLS")) {
+        dcol.upload(d.data(), d.size(), s);
+        col.resize(0);
+    } else {
+        dcol.resize(0);
+        col.upload(e.col.data(), e.col.size(), s);
+    }
     if (va != nullptr) {
         if (fp64) {
             const std::vector<double> v = ell_values<double>(e, va);
@@ -112,6 +138,7 @@ void DevEll::upload_csr(int rows, int cols, const int* rp, const int* ci, const 
     tpr = t;
     const std::size_t nnz = static_cast<std::size_t>(rp[rows]);
     sp.upload(rp, static_cast<std::size_t>(rows) + 1, s);
+    dcol.resize(0); // CSR rows always carry 32-bit columns
     col.upload(ci, nnz, s);
     v64.upload(va, nnz, s);
     std::vector<float> f(nnz);

@@ -45,17 +45,37 @@ struct DevEll {
     int tpr = 1;
     DevBuf<int> sp;
     DevBuf<int> col;
+    DevBuf<short> dcol;          // 16-bit column offsets (replaces col when present)
     DevBuf<double> v64;
     DevBuf<float> v32;
+    DevBuf<unsigned short> vbf;  // BF16 values scaled per row (diagonal slot 0)
+    DevBuf<float> rscale;        // per padded row
+    DevBuf<float> diag32;        // per padded row
 
     void upload(const HostEll& e, const double* va, bool fp64, bool fp32, hipStream_t s);
     // Chooses the layout from the row lengths (tpr_override > 0 forces it).
     void upload_csr(int n_rows, int n_cols, const int* rp, const int* ci, const double* va,
                     hipStream_t s, int tpr_override = 0);
-    [[nodiscard]] EllView view64() const { return {n_rows, tpr, n_slices, 0, sp.get(), col.get(), v64.get()}; }
-    [[nodiscard]] EllView view32() const { return {n_rows, tpr, n_slices, 0, sp.get(), col.get(), v32.get()}; }
+    [[nodiscard]] EllView view(const void* v, int fmt = 0) const {
+        EllView e{};
+        e.n_rows = n_rows;
+        e.tpr = tpr;
+        e.n_slices = n_slices;
+        e.fmt = fmt;
+        e.sp = sp.get();
+        e.col = col.size() > 0 ? col.get() : nullptr;
+        e.dcol = dcol.size() > 0 ? dcol.get() : nullptr;
+        e.val = v;
+        e.rscale = rscale.get();
+        e.diag = diag32.get();
+        return e;
+    }
+    [[nodiscard]] EllView view64() const { return view(v64.get()); }
+    [[nodiscard]] EllView view32() const { return view(v32.get()); }
+    [[nodiscard]] EllView viewbf() const { return view(vbf.get(), 1); }
     [[nodiscard]] std::size_t bytes() const {
-        return sp.bytes() + col.bytes() + v64.bytes() + v32.bytes();
+        return sp.bytes() + col.bytes() + dcol.bytes() + v64.bytes() + v32.bytes() + vbf.bytes() +
+               rscale.bytes() + diag32.bytes();
     }
 };
 

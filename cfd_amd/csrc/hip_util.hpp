@@ -124,6 +124,7 @@ public:
         }
     }
     [[nodiscard]] T* get() const { return p_; }
+    [[nodiscard]] std::size_t size() const { return n_; }
     T& operator[](std::size_t i) const { return p_[i]; }
 
 private:

@@ -70,37 +70,87 @@ __device__ __forceinline__ void block_sums(double (&v)[N], double (&out)[N]) {
     }
 }
 
-// Row sum of an ELL row: sum_k val_k * x[col_k] in storage order.
-template <class T>
-__device__ __forceinline__ T ell_row(const EllView& m, int slice, int lane, const T* __restrict__ x) {
-    const int base = m.sp[slice];
-    const int w = (m.sp[slice + 1] - base) / k_wave;
-    const int* __restrict__ col = m.col + base + lane;
-    const T* __restrict__ val = static_cast<const T*>(m.val) + base + lane;
+__device__ __forceinline__ float bf16_to_float(unsigned short h) {
+    return __uint_as_float(static_cast<unsigned>(h) << 16);
+}
+
+// Column index of slot k: int32 column or row + 16-bit offset.
+struct ColIdx {
+    const int* __restrict__ col;
+    const short* __restrict__ dcol;
+    int row;
+    __device__ __forceinline__ int operator()(int k) const {
+        return dcol != nullptr ? row + static_cast<int>(dcol[k]) : col[k];
+    }
+};
+
+// Storage-order row sum of sliced-ELL slots with values val (type V) converted to T.
+template <class T, class V, class F>
+__device__ __forceinline__ T ell_sum(const ColIdx& ci, const V* __restrict__ val, int w, F load) {
     T s = T(0);
     int k = 0;
     for (; k + 4 <= w; k += 4) {
-        const int c0 = col[(k + 0) * k_wave];
-        const int c1 = col[(k + 1) * k_wave];
-        const int c2 = col[(k + 2) * k_wave];
-        const int c3 = col[(k + 3) * k_wave];
-        const T a0 = val[(k + 0) * k_wave];
-        const T a1 = val[(k + 1) * k_wave];
-        const T a2 = val[(k + 2) * k_wave];
-        const T a3 = val[(k + 3) * k_wave];
-        const T x0 = x[c0];
-        const T x1 = x[c1];
-        const T x2 = x[c2];
-        const T x3 = x[c3];
+        const int c0 = ci((k + 0) * k_wave);
+        const int c1 = ci((k + 1) * k_wave);
+        const int c2 = ci((k + 2) * k_wave);
+        const int c3 = ci((k + 3) * k_wave);
+        T a0, a1, a2, a3;
+        if constexpr (sizeof(V) == 2) {
+            a0 = bf16_to_float(val[(k + 0) * k_wave]);
+            a1 = bf16_to_float(val[(k + 1) * k_wave]);
+            a2 = bf16_to_float(val[(k + 2) * k_wave]);
+            a3 = bf16_to_float(val[(k + 3) * k_wave]);
+        } else {
+            a0 = val[(k + 0) * k_wave];
+            a1 = val[(k + 1) * k_wave];
+            a2 = val[(k + 2) * k_wave];
+            a3 = val[(k + 3) * k_wave];
+        }
+        const T x0 = load(c0);
+        const T x1 = load(c1);
+        const T x2 = load(c2);
+        const T x3 = load(c3);
         s = madd(a0, x0, s);
         s = madd(a1, x1, s);
         s = madd(a2, x2, s);
         s = madd(a3, x3, s);
     }
     for (; k < w; ++k) {
-        s = madd(val[k * k_wave], x[col[k * k_wave]], s);
+        T a;
+        if constexpr (sizeof(V) == 2) {
+            a = bf16_to_float(val[k * k_wave]);
+        } else {
+            a = val[k * k_wave];
+        }
+        s = madd(a, load(ci(k * k_wave)), s);
     }
     return s;
+}
+
+// Row sum of an ELL row with the x entry of column c produced by load(c), in storage order.
+// BF16 rows (fmt 1, FP32 kernels only) give diag_i x_i + rscale_i sum_k bf16_k x_ck.
+template <class T, class F>
+__device__ __forceinline__ T ell_row_f(const EllView& m, int slice, int lane, F load) {
+    const int base = m.sp[slice];
+    const int w = (m.sp[slice + 1] - base) / k_wave;
+    const int row = slice * k_wave + lane;
+    const ColIdx ci{m.col == nullptr ? nullptr : m.col + base + lane,
+                    m.dcol == nullptr ? nullptr : m.dcol + base + lane, row};
+    if constexpr (sizeof(T) == 4) {
+        if (m.fmt == 1) {
+            const unsigned short* val = static_cast<const unsigned short*>(m.val) + base + lane;
+            const T s = ell_sum<T>(ci, val, w, load);
+            const T dx = row < m.n_rows ? m.diag[row] * load(row) : T(0);
+            return madd(m.rscale[row], s, dx);
+        }
+    }
+    return ell_sum<T>(ci, static_cast<const T*>(m.val) + base + lane, w, load);
+}
+
+// Row sum of an ELL row: sum_k val_k * x[col_k] in storage order (any column / value format).
+template <class T>
+__device__ __forceinline__ T ell_row(const EllView& m, int slice, int lane, const T* __restrict__ x) {
+    return ell_row_f<T>(m, slice, lane, [&](int c) { return x[c]; });
 }
 
 // Row sum with the first Jacobi iterate x1_j = wd_j * b_j formed on the fly (the reference's
@@ -108,35 +158,7 @@ __device__ __forceinline__ T ell_row(const EllView& m, int slice, int lane, cons
 template <class T>
 __device__ __forceinline__ T ell_row_x1(const EllView& m, int slice, int lane, const T* __restrict__ wd,
                                         const T* __restrict__ b) {
-    const int base = m.sp[slice];
-    const int w = (m.sp[slice + 1] - base) / k_wave;
-    const int* __restrict__ col = m.col + base + lane;
-    const T* __restrict__ val = static_cast<const T*>(m.val) + base + lane;
-    T s = T(0);
-    int k = 0;
-    for (; k + 4 <= w; k += 4) {
-        const int c0 = col[(k + 0) * k_wave];
-        const int c1 = col[(k + 1) * k_wave];
-        const int c2 = col[(k + 2) * k_wave];
-        const int c3 = col[(k + 3) * k_wave];
-        const T a0 = val[(k + 0) * k_wave];
-        const T a1 = val[(k + 1) * k_wave];
-        const T a2 = val[(k + 2) * k_wave];
-        const T a3 = val[(k + 3) * k_wave];
-        const T x0 = wd[c0] * b[c0];
-        const T x1 = wd[c1] * b[c1];
-        const T x2 = wd[c2] * b[c2];
-        const T x3 = wd[c3] * b[c3];
-        s = madd(a0, x0, s);
-        s = madd(a1, x1, s);
-        s = madd(a2, x2, s);
-        s = madd(a3, x3, s);
-    }
-    for (; k < w; ++k) {
-        const int c = col[k * k_wave];
-        s = madd(val[k * k_wave], wd[c] * b[c], s);
-    }
-    return s;
+    return ell_row_f<T>(m, slice, lane, [&](int c) { return wd[c] * b[c]; });
 }
 
 // Sum over groups of tpr consecutive lanes (tpr a power of two <= 64); every lane of the
@@ -344,8 +366,49 @@ __global__ void __launch_bounds__(k_block) k_spmv_dot(const SysDev* __restrict__
 
 // x += alpha p; r -= alpha q; partial r.r. Jacobi CG also forms z = dinv r and r.z; the
 // mixed solver writes r32 = (float) r for the FP32 V-cycle.
+// Search directions alternate between two buffers: iteration k reads p_old = parity ? p : p2
+// and writes p_new = parity ? p2 : p (parity = k mod 2, iterations counted from 1).
+__device__ __forceinline__ double* p_old_of(const SysDev& S, int parity) { return (parity ^ S.flip) ? S.p : S.p2; }
+__device__ __forceinline__ double* p_new_of(const SysDev& S, int parity) { return (parity ^ S.flip) ? S.p2 : S.p; }
+
+// q = A p_new with p_new = z + beta p_old formed per gathered column (the reference's separate
+// p update, krylov_kernels.inc:293-296, folded into the next product; identical arithmetic),
+// p_new written for the own rows, and the partial p_new.q. z is the level-0 V-cycle output
+// v[z_buf] (which = 1, FP64 or FP32) or the Jacobi-preconditioned residual S.z (which = 0).
+template <class TZ>
+__global__ void __launch_bounds__(k_block) k_spmv_pz(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int parity,
+                          int which, int z_buf) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const TZ* __restrict__ z = which == 0 ? reinterpret_cast<const TZ*>(S.z)
+                                          : static_cast<const TZ*>(S.lv[0].v[z_buf]);
+    const double* __restrict__ po = p_old_of(S, parity);
+    double* __restrict__ pn = p_new_of(S, parity);
+    const double beta = S.st->beta;
+    double d = 0.0;
+    if (slice < S.a64.n_slices) {
+        const double qi = ell_row_f<double>(S.a64, slice, lane, [&](int c) {
+            return static_cast<double>(z[c]) + beta * po[c];
+        });
+        if (row < S.n) {
+            const double pi = static_cast<double>(z[row]) + beta * po[row];
+            pn[row] = pi;
+            S.q[row] = qi;
+            d = pi * qi;
+        }
+    }
+    double v[1] = {d};
+    double o[1];
+    block_sums<1>(v, o);
+    if (threadIdx.x == 0) {
+        S.partials[SLOT_PQ * S.nblk0 + bk_.y] = o[0];
+    }
+}
+
 template <int Mode> // 0 AMG FP64, 1 Jacobi CG, 2 mixed AMG
-__global__ void __launch_bounds__(k_block) k_update_xr(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+__global__ void __launch_bounds__(k_block) k_update_xr(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int parity) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
     if (S.st->status != ST_RUNNING) {
         return;
@@ -354,7 +417,7 @@ __global__ void __launch_bounds__(k_block) k_update_xr(const SysDev* __restrict_
     double rr = 0.0;
     double rz = 0.0;
     if (row < S.n) {
-        S.x[row] = S.x[row] + alpha * S.p[row];
+        S.x[row] = S.x[row] + alpha * p_new_of(S, parity)[row];
         const double ri = S.r[row] - alpha * S.q[row];
         S.r[row] = ri;
         rr = ri * ri;

@@ -56,14 +56,20 @@ struct SolveState {
 // rows (ptr = slice offsets, entry k of row 64 s + lane at ptr[s] + 64 k + lane). tpr > 1: CSR
 // (ptr = row offsets) with tpr consecutive lanes per row, used for the long rows of coarse
 // operators and restrictions. A workgroup covers 256 / tpr rows.
+// Sliced-ELL options: dcol (if not null) replaces col by 16-bit offsets column - row; fmt 1
+// stores the values as BF16 scaled per row (a_ij = rscale_i * bf16_ij) with the diagonal slot
+// zero and the diagonal in FP32 in diag (rscale and diag padded to whole slices).
 struct EllView {
     int n_rows;
     int tpr;
     int n_slices;    // ELL only
-    int pad;
+    int fmt;         // 0 values of the kernel's type, 1 BF16 + row scale + FP32 diagonal
     const int* sp;   // ELL slice offsets or CSR row offsets
     const int* col;
-    const void* val; // double or float values
+    const short* dcol;
+    const void* val; // double, float or BF16 bits
+    const float* rscale;
+    const float* diag;
 };
 
 struct LevelDev {
@@ -86,12 +92,14 @@ struct SysDev {
     int nblk0;        // 256-row blocks of level 0
     int n;
     int n_levels;
-    int pad;
+    int flip;               // XOR on the launch's iteration parity: 0 unless the system joined
+                            // a running set at an even iteration
     EllView a64;            // level-0 operator in FP64
     const double* b;        // right-hand side
     double* x;              // iterate
     double* r;
     double* p;
+    double* p2;             // second search-direction buffer (alternates with p)
     double* q;
     double* z;              // Jacobi-preconditioned residual / BiCGStab scratch
     float* r32;             // FP32 copy of r (mixed V-cycle input)

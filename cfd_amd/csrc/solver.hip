@@ -6,6 +6,7 @@
 // Krylov recurrences lives on the device, and the host reads back one integer per iteration
 // (the number of systems still running) one iteration behind the launches.
 #include "solver.hpp"
+#include "kernels.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -24,6 +25,49 @@ __global__ void __launch_bounds__(k_block) k_fill(int n, double* __restrict__ x,
     if (i < n) {
         x[i] = v;
     }
+}
+
+__device__ __forceinline__ unsigned short float_to_bf16(float f) { // round to nearest even
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<unsigned short>(u >> 16);
+}
+
+// BF16 copy of a sliced-ELL FP64 operator: per row the scale s = max |off-diagonal| (1 if
+// none), off-diagonal slots bf16(a / s), the diagonal slot 0 and the diagonal in FP32. The
+// diagonal is the first slot of the row holding its own column (padding slots come after it).
+__global__ void __launch_bounds__(k_block) k_make_bf16(EllView m, unsigned short* __restrict__ vbf,
+                                                       float* __restrict__ rscale, float* __restrict__ diag) {
+    const int row = blockIdx.x * k_block + threadIdx.x;
+    const int slice = row / k_wave;
+    if (slice >= m.n_slices) {
+        return;
+    }
+    const int lane = row % k_wave;
+    const int base = m.sp[slice];
+    const int w = (m.sp[slice + 1] - base) / k_wave;
+    const ColIdx ci{m.col == nullptr ? nullptr : m.col + base + lane,
+                    m.dcol == nullptr ? nullptr : m.dcol + base + lane, row};
+    const double* val = static_cast<const double*>(m.val) + base + lane;
+    int kd = -1;
+    double d = 0.0;
+    double mx = 0.0;
+    for (int k = 0; k < w; ++k) {
+        const double v = val[k * k_wave];
+        if (kd < 0 && row < m.n_rows && ci(k * k_wave) == row) {
+            kd = k;
+            d = v;
+        } else {
+            mx = fmax(mx, fabs(v));
+        }
+    }
+    const double s = mx > 0.0 ? mx : 1.0;
+    for (int k = 0; k < w; ++k) {
+        vbf[base + lane + k * k_wave] = k == kd ? static_cast<unsigned short>(0)
+                                                : float_to_bf16(static_cast<float>(val[k * k_wave] / s));
+    }
+    rscale[row] = static_cast<float>(s);
+    diag[row] = static_cast<float>(d);
 }
 
 __global__ void __launch_bounds__(k_block) k_scale_dinv(int n, const double* __restrict__ dinv, double w,
@@ -210,7 +254,7 @@ System::System(const cfdg_csr& a) {
 
 void System::init_vectors() {
     const std::size_t n = static_cast<std::size_t>(n_);
-    for (DevBuf<double>* v : {&b_, &x_, &r_, &p_, &q_, &z_, &r0_, &v_, &s_, &t_, &ph_, &sh_, &dinv0_,
+    for (DevBuf<double>* v : {&b_, &x_, &r_, &p_, &p2_, &q_, &z_, &r0_, &v_, &s_, &t_, &ph_, &sh_, &dinv0_,
                               &wd0_64_}) {
         v->resize(n);
         v->zero(stream_.get());
@@ -240,7 +284,7 @@ void System::assemble(const double* diag, const double* upper, const double* low
         hip_check(hipMemcpyAsync(ldu + n + nf, lower, nf * sizeof(double), hipMemcpyHostToDevice, s),
                   "H2D lower");
     }
-    const long long slots = static_cast<long long>(a0_.col.size());
+    const long long slots = static_cast<long long>(a0_src_.size());
     k_ldu_gather<<<ceil_div(slots, k_block), k_block, 0, s>>>(slots, a0_src_.get(), ldu,
                                                                a0_.v64.get(), a0_.v32.get());
     hip_check(hipGetLastError(), "k_ldu_gather launch");
@@ -249,6 +293,7 @@ void System::assemble(const double* diag, const double* upper, const double* low
     hip_check(hipGetLastError(), "k_diag_inverse launch");
     hip_check(hipStreamSynchronize(s), "assemble sync");
     assembled_ = true;
+    bf16_ready_ = false;
 }
 
 void System::set_rhs(const double* b) {
@@ -271,6 +316,67 @@ void System::get_x(double* x) {
                              hipMemcpyDeviceToHost, stream_.get()),
               "D2H x");
     hip_check(hipStreamSynchronize(stream_.get()), "D2H sync");
+}
+
+void System::copy_rhs_from(const double* dev, hipStream_t s) {
+    hip_check(hipMemcpyAsync(b_.get(), dev, static_cast<std::size_t>(n_) * sizeof(double),
+                             hipMemcpyDeviceToDevice, s),
+              "D2D b");
+}
+
+void System::copy_x_to(double* dev, hipStream_t s) const {
+    hip_check(hipMemcpyAsync(dev, x_.get(), static_cast<std::size_t>(n_) * sizeof(double),
+                             hipMemcpyDeviceToDevice, s),
+              "D2D x");
+}
+
+namespace {
+
+// Stored slots of a matrix in the larger of its two device layouts: sliced ELL (64-row slices
+// padded to their longest row) or CSR.
+std::size_t layout_slots(int rows, const int* rp) {
+    std::size_t ell = 0;
+    for (int s0 = 0; s0 < rows; s0 += k_wave) {
+        int w = 0;
+        for (int i = s0; i < std::min(rows, s0 + k_wave); ++i) {
+            w = std::max(w, rp[i + 1] - rp[i]);
+        }
+        ell += static_cast<std::size_t>(k_wave) * static_cast<std::size_t>(w);
+    }
+    return std::max(ell, static_cast<std::size_t>(rp[rows]));
+}
+
+} // namespace
+
+std::size_t estimate_footprint(const cfdg_ldu_pattern& pt, const cfdg_amg_view* amg) {
+    require(pt.n >= 1 && pt.row_ptr != nullptr, "estimate_footprint: invalid pattern");
+    const std::size_t n = static_cast<std::size_t>(pt.n);
+    const std::size_t nblk0 = static_cast<std::size_t>(ceil_div(pt.n, k_block));
+    // Level 0: per slot col, dcol, FP64 / FP32 / BF16 values and the LDU source index; per row
+    // 15 FP64 vectors, 3 FP32 vectors and the BF16 scales.
+    std::size_t b = layout_slots(pt.n, pt.row_ptr) * (4 + 2 + 8 + 4 + 2 + 4);
+    b += n * (15 * 8 + 3 * 4 + 2 * 4) + (n + 2 * static_cast<std::size_t>(pt.n_faces)) * 8;
+    b += static_cast<std::size_t>(k_slots) * nblk0 * 8 + sizeof(SolveState) + 64 * 1024;
+    if (amg != nullptr && amg->levels != nullptr) {
+        for (int l = 0; l < amg->n_levels; ++l) {
+            const cfdg_amg_level_view& v = amg->levels[l];
+            const std::size_t nl = l == 0 ? n : static_cast<std::size_t>(v.a.n_rows);
+            b += nl * (5 * 8 + 5 * 4); // V-cycle vectors of both precisions
+            if (l > 0 && v.a.row_ptr != nullptr) {
+                b += layout_slots(v.a.n_rows, v.a.row_ptr) * (4 + 2 + 8 + 4) + nl * 8;
+            }
+            for (const cfdg_csr* m : {&v.p, &v.r}) {
+                if (m->row_ptr != nullptr && m->n_rows > 0) {
+                    b += layout_slots(m->n_rows, m->row_ptr) * (4 + 2 + 8 + 4) +
+                         static_cast<std::size_t>(m->n_rows + 1) * 4;
+                }
+            }
+            if (l == amg->n_levels - 1) {
+                b += nl * nl * 8;
+            }
+        }
+    }
+    return b + b / 64; // allocation granularity
 }
 
 void System::set_amg(const cfdg_amg_view& v) {
@@ -367,8 +473,24 @@ void System::ensure_level_vectors(bool fp32) {
     hip_check(hipStreamSynchronize(stream_.get()), "level vectors sync");
 }
 
-SysDev System::descriptor(bool fp32, bool use_amg) {
+void System::make_bf16() {
+    const std::size_t slots = a0_.v64.size();
+    const std::size_t rows = static_cast<std::size_t>(a0_.n_slices) * k_wave;
+    a0_.vbf.resize(slots);
+    a0_.rscale.resize(rows);
+    a0_.diag32.resize(rows);
+    k_make_bf16<<<ceil_div(static_cast<long long>(rows), k_block), k_block, 0, stream_.get()>>>(
+        a0_.view64(), a0_.vbf.get(), a0_.rscale.get(), a0_.diag32.get());
+    hip_check(hipGetLastError(), "k_make_bf16 launch");
+    hip_check(hipStreamSynchronize(stream_.get()), "bf16 sync");
+    bf16_ready_ = true;
+}
+
+SysDev System::descriptor(bool fp32, bool use_amg, bool bf16) {
     require(assembled_, "solve: the matrix has not been assembled");
+    if (bf16 && !bf16_ready_) {
+        make_bf16();
+    }
     SysDev d{};
     d.st = state_.get();
     d.partials = partials_.get();
@@ -379,6 +501,7 @@ SysDev System::descriptor(bool fp32, bool use_amg) {
     d.x = x_.get();
     d.r = r_.get();
     d.p = p_.get();
+    d.p2 = p2_.get();
     d.q = q_.get();
     d.z = z_.get();
     d.r32 = r32_.get();
@@ -405,7 +528,7 @@ SysDev System::descriptor(bool fp32, bool use_amg) {
         o.n_next = s.n_next;
         o.coarsest = coarsest ? 1 : 0;
         if (l == 0) {
-            o.a = fp32 ? a0_.view32() : a0_.view64();
+            o.a = bf16 ? a0_.viewbf() : (fp32 ? a0_.view32() : a0_.view64());
             o.b = fp32 ? static_cast<void*>(r32_.get()) : static_cast<void*>(r_.get());
             o.wd = fp32 ? static_cast<const void*>(wd0_32_.get()) : static_cast<const void*>(wd0_64_.get());
         } else {
@@ -430,7 +553,7 @@ SysDev System::descriptor(bool fp32, bool use_amg) {
 std::size_t System::footprint() const {
     std::size_t b = a0_.bytes() + a0_src_.bytes() + ldu_.bytes() + dinv0_.bytes() + wd0_64_.bytes() +
                     wd0_32_.bytes() + r32_.bytes() + partials_.bytes() + state_.bytes();
-    for (const DevBuf<double>* v : {&b_, &x_, &r_, &p_, &q_, &z_, &r0_, &v_, &s_, &t_, &ph_, &sh_}) {
+    for (const DevBuf<double>* v : {&b_, &x_, &r_, &p_, &p2_, &q_, &z_, &r0_, &v_, &s_, &t_, &ph_, &sh_}) {
         b += v->bytes();
     }
     for (const DevLevel& l : levels_) {
@@ -445,103 +568,161 @@ std::size_t System::footprint() const {
 
 namespace {
 
+// A launch table in device memory: (system, block) pairs inside the owning Tables' arena.
+struct TableRef {
+    const int2* p = nullptr;
+    [[nodiscard]] const int2* get() const { return p; }
+};
+
+struct IdsRef {
+    const int* p = nullptr;
+    [[nodiscard]] const int* get() const { return p; }
+};
+
 struct Tables {
     int n_sys = 0;
     int l_max = 1;
-    DevBuf<int> ids;
-    std::vector<DevBuf<int2>> t;   // per level: 256-row blocks of systems having the level
+    IdsRef ids;                    // descriptor index of each system (finalize workgroups)
+    std::vector<TableRef> t;       // per level: 256-row blocks of systems having the level
     std::vector<int> tn;
-    std::vector<DevBuf<int2>> c;   // per level: 4-row blocks of systems whose coarsest it is
+    std::vector<TableRef> c;       // per level: 4-row blocks of systems whose coarsest it is
     std::vector<int> cn;
-    DevBuf<int2> t0_single;        // level-0 blocks of one-level hierarchies
+    TableRef t0_single;            // level-0 blocks of one-level hierarchies
     int t0_single_n = 0;
     // per level and matrix (0 operator, 1 prolongation, 2 restriction): blocks of 256 / tpr
     // rows of that matrix for the systems whose level l is not the coarsest
-    std::vector<DevBuf<int2>> tm[3];
+    std::vector<TableRef> tm[3];
     std::vector<int> tmn[3];
     std::vector<char> a_ell; // every system's level-l operator is sliced ELL (fused presmooth)
+    DevBuf<int2> arena;      // every table above, back to back
+    PinnedBuf<int2> stage;   // host image of the arena for asynchronous uploads
 };
 
-void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hipStream_t s) {
+// Launch tables of a set of systems. Entries name a system by its index into the descriptor
+// array: its position in `sys`, or slots[position] when slots are given. A synchronous build
+// returns with the tables in device memory; an asynchronous one queues the upload on `s`
+// (the tables' previous contents must no longer be in use by queued work).
+void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hipStream_t s,
+                  const std::vector<int>* slots = nullptr, bool async = false) {
     tb.n_sys = static_cast<int>(sys.size());
     std::vector<int> ids(sys.size());
     std::iota(ids.begin(), ids.end(), 0);
-    tb.ids.upload(ids.data(), ids.size(), s);
+    if (slots != nullptr) {
+        require(slots->size() == sys.size(), "build_tables: one slot per system");
+        ids = *slots;
+    }
     tb.l_max = 1;
     if (use_amg) {
         for (System* x : sys) {
             tb.l_max = std::max(tb.l_max, x->n_levels());
         }
     }
-    tb.t.clear();
-    tb.c.clear();
-    tb.t.resize(static_cast<std::size_t>(tb.l_max));
-    tb.c.resize(static_cast<std::size_t>(tb.l_max));
-    tb.tn.assign(static_cast<std::size_t>(tb.l_max), 0);
-    tb.cn.assign(static_cast<std::size_t>(tb.l_max), 0);
+    const std::size_t L = static_cast<std::size_t>(tb.l_max);
+    // Host image: ids (two per int2), then the tables; offsets recorded per table.
+    std::vector<int2> img((ids.size() + 1) / 2, make_int2(0, 0));
+    std::memcpy(img.data(), ids.data(), ids.size() * sizeof(int));
+    std::vector<std::size_t> off_t(L), off_c(L), off_m[3];
+    tb.tn.assign(L, 0);
+    tb.cn.assign(L, 0);
     std::vector<int2> single;
     for (int l = 0; l < tb.l_max; ++l) {
-        std::vector<int2> t;
+        const std::size_t lu = static_cast<std::size_t>(l);
+        off_t[lu] = img.size();
         std::vector<int2> c;
         for (int q = 0; q < tb.n_sys; ++q) {
             System* x = sys[static_cast<std::size_t>(q)];
-            const int L = use_amg ? x->n_levels() : 1;
-            if (l >= L) {
+            const int id = ids[static_cast<std::size_t>(q)];
+            const int nl = use_amg ? x->n_levels() : 1;
+            if (l >= nl) {
                 continue;
             }
             const int rows = (use_amg && l > 0) ? x->level_rows(l) : x->n();
             const int nb = ceil_div(rows, k_block);
             for (int b = 0; b < nb; ++b) {
-                t.push_back(make_int2(q, b));
+                img.push_back(make_int2(id, b));
             }
-            if (use_amg && l == L - 1) {
+            if (use_amg && l == nl - 1) {
                 const int nc = ceil_div(rows, k_slices_per_block);
                 for (int b = 0; b < nc; ++b) {
-                    c.push_back(make_int2(q, b));
+                    c.push_back(make_int2(id, b));
                 }
                 if (l == 0) {
                     for (int b = 0; b < nb; ++b) {
-                        single.push_back(make_int2(q, b));
+                        single.push_back(make_int2(id, b));
                     }
                 }
             }
         }
-        tb.tn[static_cast<std::size_t>(l)] = static_cast<int>(t.size());
-        tb.cn[static_cast<std::size_t>(l)] = static_cast<int>(c.size());
-        tb.t[static_cast<std::size_t>(l)].upload(t.data(), t.size(), s);
-        tb.c[static_cast<std::size_t>(l)].upload(c.data(), c.size(), s);
+        tb.tn[lu] = static_cast<int>(img.size() - off_t[lu]);
+        off_c[lu] = img.size();
+        tb.cn[lu] = static_cast<int>(c.size());
+        img.insert(img.end(), c.begin(), c.end());
     }
+    const std::size_t off_single = img.size();
     tb.t0_single_n = static_cast<int>(single.size());
-    tb.t0_single.upload(single.data(), single.size(), s);
+    img.insert(img.end(), single.begin(), single.end());
+    tb.a_ell.assign(L, 1);
     for (int which = 0; which < 3; ++which) {
-        tb.tm[which].clear();
-        tb.tm[which].resize(static_cast<std::size_t>(tb.l_max));
-        tb.tmn[which].assign(static_cast<std::size_t>(tb.l_max), 0);
+        tb.tmn[which].assign(L, 0);
+        off_m[which].assign(L, img.size());
     }
-    tb.a_ell.assign(static_cast<std::size_t>(tb.l_max), 1);
-    if (!use_amg) {
-        return;
-    }
-    for (int l = 0; l + 1 < tb.l_max; ++l) {
+    for (int l = 0; use_amg && l + 1 < tb.l_max; ++l) {
+        const std::size_t lu = static_cast<std::size_t>(l);
         for (int which = 0; which < 3; ++which) {
-            std::vector<int2> t;
+            off_m[which][lu] = img.size();
             for (int q = 0; q < tb.n_sys; ++q) {
                 System* x = sys[static_cast<std::size_t>(q)];
                 if (l + 1 >= x->n_levels()) {
                     continue; // level l is this system's coarsest (or absent)
                 }
                 const int tpr = x->tpr(l, which);
-                if (which == 0 && tpr != 1) {
-                    tb.a_ell[static_cast<std::size_t>(l)] = 0;
+                // The fused first two sweeps gather wd and b per entry: worth it only for the
+                // short rows of the finest operator.
+                if (which == 0 && (tpr != 1 || l > 0)) {
+                    tb.a_ell[lu] = 0;
                 }
                 const int rows = which == 2 ? x->level_rows(l + 1) : x->level_rows(l);
                 const int nb = ceil_div(static_cast<long long>(rows) * tpr, k_block);
                 for (int b = 0; b < nb; ++b) {
-                    t.push_back(make_int2(q, b));
+                    img.push_back(make_int2(ids[static_cast<std::size_t>(q)], b));
                 }
             }
-            tb.tmn[which][static_cast<std::size_t>(l)] = static_cast<int>(t.size());
-            tb.tm[which][static_cast<std::size_t>(l)].upload(t.data(), t.size(), s);
+            tb.tmn[which][lu] = static_cast<int>(img.size() - off_m[which][lu]);
+        }
+    }
+    if (tb.arena.size() < img.size()) {
+        if (async) {
+            hip_check(hipStreamSynchronize(s), "tables: drain before growing");
+        }
+        tb.arena.resize(std::max(img.size(), 2 * tb.arena.size()));
+    }
+    if (async) {
+        if (tb.stage.size() < img.size()) {
+            tb.stage = PinnedBuf<int2>(std::max(img.size(), 2 * tb.stage.size()));
+        }
+        std::memcpy(tb.stage.get(), img.data(), img.size() * sizeof(int2));
+        hip_check(hipMemcpyAsync(tb.arena.get(), tb.stage.get(), img.size() * sizeof(int2),
+                                 hipMemcpyHostToDevice, s),
+                  "tables H2D");
+    } else {
+        hip_check(hipMemcpyAsync(tb.arena.get(), img.data(), img.size() * sizeof(int2),
+                                 hipMemcpyHostToDevice, s),
+                  "tables H2D");
+        hip_check(hipStreamSynchronize(s), "tables H2D sync");
+    }
+    const int2* base = tb.arena.get();
+    tb.ids.p = reinterpret_cast<const int*>(base);
+    tb.t.assign(L, TableRef{});
+    tb.c.assign(L, TableRef{});
+    for (std::size_t l = 0; l < L; ++l) {
+        tb.t[l].p = base + off_t[l];
+        tb.c[l].p = base + off_c[l];
+    }
+    tb.t0_single.p = base + off_single;
+    for (int which = 0; which < 3; ++which) {
+        tb.tm[which].assign(L, TableRef{});
+        for (std::size_t l = 0; l < L; ++l) {
+            tb.tm[which][l].p = base + off_m[which][l];
         }
     }
 }
@@ -669,29 +850,28 @@ void enqueue_init(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t
         fin<FIN_BICG_INIT>(d, tb, s);
         return;
     }
+    // CG start: r, z and beta = 0; p = z + 0 p_old (p_old = 0) is formed by the first
+    // iteration's product (k_spmv_pz).
     if (!pl.amg) {
         CFDG_LAUNCH(k_cg_init, g0, s, d, b0);
         fin<FIN_CG_INIT>(d, tb, s);
-        CFDG_LAUNCH(k_p_update<double>, g0, s, d, b0, 0, 0);
         return;
     }
-    const int fz = final_index(pl.sweeps);
     if (pl.fp32) {
         CFDG_LAUNCH(k_residual0<true>, g0, s, d, b0);
         fin<FIN_AMG_INIT>(d, tb, s);
         enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s);
         fin<FIN_BETA_FLEX>(d, tb, s);
-        CFDG_LAUNCH(k_p_update<float>, g0, s, d, b0, 1, fz);
     } else {
         CFDG_LAUNCH(k_residual0<false>, g0, s, d, b0);
         fin<FIN_AMG_INIT>(d, tb, s);
         enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s);
         fin<FIN_BETA>(d, tb, s);
-        CFDG_LAUNCH(k_p_update<double>, g0, s, d, b0, 1, fz);
     }
 }
 
-void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t s) {
+// One iteration; parity = iteration number mod 2 selects the search-direction buffers.
+void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t s, int parity) {
     const int g0 = tb.tn[0];
     const int2* b0 = tb.t[0].get();
     if (pl.method == CFDG_METHOD_BICGSTAB) {
@@ -708,29 +888,32 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
         fin<FIN_BICG_RES>(d, tb, s);
         return;
     }
+    const int fz = final_index(pl.sweeps);
     probe_begin(PROBE_OUTER_SPMV, s);
-    CFDG_LAUNCH(k_spmv_dot<false>, g0, s, d, b0, V_P, V_Q, V_P, SLOT_PQ, 0, 0);
+    if (!pl.amg) {
+        CFDG_LAUNCH(k_spmv_pz<double>, g0, s, d, b0, parity, 0, 0);
+    } else if (pl.fp32) {
+        CFDG_LAUNCH(k_spmv_pz<float>, g0, s, d, b0, parity, 1, fz);
+    } else {
+        CFDG_LAUNCH(k_spmv_pz<double>, g0, s, d, b0, parity, 1, fz);
+    }
     probe_end(PROBE_OUTER_SPMV, s);
     fin<FIN_ALPHA>(d, tb, s);
     if (!pl.amg) {
-        CFDG_LAUNCH(k_update_xr<1>, g0, s, d, b0);
+        CFDG_LAUNCH(k_update_xr<1>, g0, s, d, b0, parity);
         fin<FIN_CG_RES_BETA>(d, tb, s);
-        CFDG_LAUNCH(k_p_update<double>, g0, s, d, b0, 0, 0);
         return;
     }
-    const int fz = final_index(pl.sweeps);
     if (pl.fp32) {
-        CFDG_LAUNCH(k_update_xr<2>, g0, s, d, b0);
+        CFDG_LAUNCH(k_update_xr<2>, g0, s, d, b0, parity);
         fin<FIN_RES>(d, tb, s);
         enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s);
         fin<FIN_BETA_FLEX>(d, tb, s);
-        CFDG_LAUNCH(k_p_update<float>, g0, s, d, b0, 1, fz);
     } else {
-        CFDG_LAUNCH(k_update_xr<0>, g0, s, d, b0);
+        CFDG_LAUNCH(k_update_xr<0>, g0, s, d, b0, parity);
         fin<FIN_RES>(d, tb, s);
         enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s);
         fin<FIN_BETA>(d, tb, s);
-        CFDG_LAUNCH(k_p_update<double>, g0, s, d, b0, 1, fz);
     }
 }
 
@@ -760,10 +943,12 @@ public:
             gens_.push_back(sys_[i]->generation());
         }
         pl_ = Plan{opt.method, amg, fp32, sweeps};
+        precision_ = amg ? opt.precision : CFDG_PREC_FP64;
+        const bool bf16 = amg && opt.precision == CFDG_PREC_MIXED_BF16;
         build_tables(tb_, sys_, amg, s_);
         std::vector<SysDev> desc(sys_.size());
         for (std::size_t i = 0; i < sys_.size(); ++i) {
-            desc[i] = sys_[i]->descriptor(fp32, amg);
+            desc[i] = sys_[i]->descriptor(fp32, amg, bf16);
         }
         desc_.upload(desc.data(), desc.size(), s_);
         d_init_.resize(sys_.size());
@@ -794,7 +979,7 @@ public:
         }
         const bool amg = opt.use_amg && opt.method == CFDG_METHOD_CG;
         return opt.method == pl_.method && amg == pl_.amg &&
-               (amg && opt.precision != CFDG_PREC_FP64) == pl_.fp32;
+               (amg ? opt.precision : CFDG_PREC_FP64) == precision_;
     }
 
     void solve(const std::vector<cfdg_solve_control>& control, std::vector<cfdg_solve_stats>& stats,
@@ -846,7 +1031,7 @@ public:
             if (graphs) {
                 hip_check(hipGraphLaunch(iter_exec_[slot], s_), "graph launch");
             } else {
-                enqueue_iteration(pl_, d, tb_, s_);
+                enqueue_iteration(pl_, d, tb_, s_, slot);
                 k_count_running<<<1, k_block, 0, s_>>>(d, n_sys, running_dev_ + slot);
             }
             hip_check(hipEventRecord(ev_[slot].get(), s_), "event record");
@@ -918,7 +1103,7 @@ private:
         init_exec_ = capture([&] { enqueue_init(pl_, d, tb_, s_); });
         for (int slot = 0; slot < 2; ++slot) {
             iter_exec_[slot] = capture([&] {
-                enqueue_iteration(pl_, d, tb_, s_);
+                enqueue_iteration(pl_, d, tb_, s_, slot);
                 k_count_running<<<1, k_block, 0, s_>>>(d, n_sys, running_dev_ + slot);
             });
         }
@@ -941,11 +1126,238 @@ private:
     Event t1_;
     bool use_graphs_ = true;
     long long solves_ = 0;
+    cfdg_precision precision_ = CFDG_PREC_FP64;
     hipGraphExec_t init_exec_ = nullptr;
     hipGraphExec_t iter_exec_[2] = {nullptr, nullptr};
 };
 
 System::~System() = default;
+
+namespace {
+
+// Status of each listed system, written to pinned host memory at out[slot].
+__global__ void k_slot_status(const SysDev* __restrict__ sys, const int* __restrict__ ids, int n,
+                              int* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int id = ids[i];
+        out[id] = sys[id].st->status;
+    }
+}
+
+enum SlotPhase : int { SLOT_FREE = 0, SLOT_RUNNING = 1, SLOT_FINISHING = 2 };
+
+} // namespace
+
+struct SlotMachinery {
+    int k = 0;
+    Plan pl{};
+    cfdg_precision precision = CFDG_PREC_FP64;
+    hipStream_t s = nullptr;
+    DevBuf<SysDev> desc;
+    PinnedBuf<SysDev> desc_host;
+    PinnedBuf<SolveState> init_host;
+    PinnedBuf<SolveState> out_host;
+    PinnedBuf<int> status;      // two regions of k entries, one per iteration parity
+    int* status_dev = nullptr;
+    std::vector<System*> sys;   // per slot
+    std::vector<int> phase;
+    std::vector<Tables> single; // per slot: tables of the slot's system alone
+    std::vector<System*> single_of;
+    std::vector<unsigned> single_gen;
+    std::vector<std::unique_ptr<Event>> done_ev;
+    std::vector<std::chrono::steady_clock::time_point> t_start;
+    std::vector<int> running;   // slots covered by the next iteration
+    Tables merged[2];
+    int cur = 0;
+    bool dirty = false;
+    long long g = 1;            // number of the next iteration
+    std::vector<int> covered[2];
+    std::vector<long long> inflight; // iterations queued whose flags have not been read
+    Event ev[2];
+};
+
+SlotRunner::SlotRunner(int slots, const BatchOptions& opt, int sweeps, hipStream_t s)
+    : m_(std::make_unique<SlotMachinery>()) {
+    require(slots >= 1 && slots <= 4096, "slot runner: slots must be in [1, 4096]");
+    require(sweeps >= 0 && sweeps <= 64, "slot runner: invalid smooth_sweeps");
+    SlotMachinery& m = *m_;
+    m.k = slots;
+    const bool amg = opt.use_amg && opt.method == CFDG_METHOD_CG;
+    m.pl = Plan{opt.method, amg, amg && opt.precision != CFDG_PREC_FP64, sweeps};
+    m.precision = amg ? opt.precision : CFDG_PREC_FP64;
+    m.s = s;
+    const std::size_t k = static_cast<std::size_t>(slots);
+    m.desc.resize(k);
+    m.desc_host = PinnedBuf<SysDev>(k);
+    m.init_host = PinnedBuf<SolveState>(k);
+    m.out_host = PinnedBuf<SolveState>(k);
+    m.status = PinnedBuf<int>(2 * k);
+    void* dev = nullptr;
+    hip_check(hipHostGetDevicePointer(&dev, m.status.get(), 0), "hipHostGetDevicePointer");
+    m.status_dev = static_cast<int*>(dev);
+    m.sys.assign(k, nullptr);
+    m.phase.assign(k, SLOT_FREE);
+    m.single = std::vector<Tables>(k);
+    m.single_of.assign(k, nullptr);
+    m.single_gen.assign(k, 0);
+    m.t_start.resize(k);
+    for (std::size_t i = 0; i < k; ++i) {
+        m.done_ev.push_back(std::make_unique<Event>());
+    }
+}
+
+SlotRunner::~SlotRunner() {
+    if (m_ && m_->s != nullptr) {
+        (void)hipStreamSynchronize(m_->s); // queued work reads the tables and pinned buffers
+    }
+}
+
+int SlotRunner::slots() const { return m_->k; }
+
+int SlotRunner::free_slots() const {
+    return static_cast<int>(std::count(m_->phase.begin(), m_->phase.end(), SLOT_FREE));
+}
+
+bool SlotRunner::busy() const {
+    return std::any_of(m_->phase.begin(), m_->phase.end(), [](int p) { return p != SLOT_FREE; });
+}
+
+int SlotRunner::start(System* x, const cfdg_solve_control& c) {
+    SlotMachinery& m = *m_;
+    require(x != nullptr, "slot runner: null system");
+    require(c.max_iterations >= 0, "solve: max_iterations must be >= 0");
+    const auto it = std::find(m.phase.begin(), m.phase.end(), SLOT_FREE);
+    require(it != m.phase.end(), "slot runner: no free slot");
+    if (m.pl.amg) {
+        require(x->has_amg(), "solve: use_amg requested but no hierarchy was set");
+        require(x->sweeps() == m.pl.sweeps, "slot runner: system's smooth_sweeps differ from the runner's");
+    }
+    const int j = static_cast<int>(it - m.phase.begin());
+    const std::size_t ju = static_cast<std::size_t>(j);
+    const bool bf16 = m.pl.amg && m.precision == CFDG_PREC_MIXED_BF16;
+    if (bf16) {
+        x->ensure_bf16();
+    }
+    SysDev d = x->descriptor(m.pl.fp32, m.pl.amg, bf16);
+    hip_check(hipStreamSynchronize(x->stream()), "slot runner: system stream sync");
+    // The solve's first iteration is global iteration g; it must see parity 1.
+    d.flip = static_cast<int>((m.g & 1) ^ 1);
+    m.desc_host[ju] = d;
+    hip_check(hipMemcpyAsync(m.desc.get() + j, m.desc_host.get() + j, sizeof(SysDev),
+                             hipMemcpyHostToDevice, m.s),
+              "slot descriptor H2D");
+    if (m.single_of[ju] != x || m.single_gen[ju] != x->generation()) {
+        std::vector<System*> one{x};
+        std::vector<int> ids{j};
+        build_tables(m.single[ju], one, m.pl.amg, m.s, &ids, true);
+        m.single_of[ju] = x;
+        m.single_gen[ju] = x->generation();
+    }
+    SolveState& st = m.init_host[ju];
+    std::memset(&st, 0, sizeof(st));
+    st.tol = c.abs_tolerance;
+    st.max_it = c.max_iterations;
+    st.status = ST_RUNNING;
+    hip_check(hipMemcpyAsync(x->state(), &st, sizeof(SolveState), hipMemcpyHostToDevice, m.s),
+              "slot state H2D");
+    enqueue_init(m.pl, m.desc.get(), m.single[ju], m.s);
+    hip_check(hipGetLastError(), "slot init launch");
+    m.sys[ju] = x;
+    m.phase[ju] = SLOT_RUNNING;
+    m.t_start[ju] = std::chrono::steady_clock::now();
+    m.running.push_back(j);
+    m.dirty = true;
+    return j;
+}
+
+void SlotRunner::step(std::vector<Done>& done) {
+    SlotMachinery& m = *m_;
+    const SysDev* d = m.desc.get();
+    // 1. Queue the next iteration over the running solves.
+    if (!m.running.empty()) {
+        if (m.dirty) {
+            std::vector<System*> xs;
+            for (int j : m.running) {
+                xs.push_back(m.sys[static_cast<std::size_t>(j)]);
+            }
+            m.cur ^= 1; // the other table set's last users finished before the previous step
+            build_tables(m.merged[m.cur], xs, m.pl.amg, m.s, &m.running, true);
+            m.dirty = false;
+        }
+        const Tables& tb = m.merged[m.cur];
+        const int parity = static_cast<int>(m.g & 1);
+        enqueue_iteration(m.pl, d, tb, m.s, parity);
+        k_slot_status<<<ceil_div(tb.n_sys, k_block), k_block, 0, m.s>>>(
+            d, tb.ids.get(), tb.n_sys, m.status_dev + static_cast<std::size_t>(parity) * m.k);
+        hip_check(hipEventRecord(m.ev[parity].get(), m.s), "event record");
+        hip_check(hipGetLastError(), "slot iteration launch");
+        m.covered[parity] = m.running;
+        m.inflight.push_back(m.g);
+        ++m.g;
+    }
+    // 2. Read the flags of the oldest queued iteration once the newest one is queued behind
+    // it (or at once when nothing else is queued); retire the solves that ended there.
+    if (m.inflight.size() >= 2 || (m.running.empty() && !m.inflight.empty())) {
+        const long long h = m.inflight.front();
+        m.inflight.erase(m.inflight.begin());
+        const int hp = static_cast<int>(h & 1);
+        hip_check(hipEventSynchronize(m.ev[hp].get()), "event sync");
+        const int* flags = m.status.get() + static_cast<std::size_t>(hp) * m.k;
+        for (int j : m.covered[hp]) {
+            const std::size_t ju = static_cast<std::size_t>(j);
+            const int f = flags[ju];
+            if (m.phase[ju] != SLOT_RUNNING || f == ST_RUNNING || f == ST_CONVERGED_PENDING) {
+                continue;
+            }
+            const Tables& one = m.single[ju];
+            CFDG_LAUNCH(k_true_residual, one.tn[0], m.s, d, one.t[0].get());
+            fin<FIN_TRUE>(d, one, m.s);
+            hip_check(hipMemcpyAsync(m.out_host.get() + j, m.sys[ju]->state(), sizeof(SolveState),
+                                     hipMemcpyDeviceToHost, m.s),
+                      "slot state D2H");
+            hip_check(hipEventRecord(m.done_ev[ju]->get(), m.s), "event record");
+            hip_check(hipGetLastError(), "slot finish launch");
+            m.phase[ju] = SLOT_FINISHING;
+            m.running.erase(std::find(m.running.begin(), m.running.end(), j));
+            m.dirty = true;
+        }
+    }
+    // 3. Report finished solves whose state has reached the host (wait when nothing else is
+    // left to drive the device).
+    const bool idle = m.running.empty() && m.inflight.empty();
+    for (int j = 0; j < m.k; ++j) {
+        const std::size_t ju = static_cast<std::size_t>(j);
+        if (m.phase[ju] != SLOT_FINISHING) {
+            continue;
+        }
+        if (idle) {
+            hip_check(hipEventSynchronize(m.done_ev[ju]->get()), "event sync");
+        } else {
+            const hipError_t q = hipEventQuery(m.done_ev[ju]->get());
+            if (q == hipErrorNotReady) {
+                continue;
+            }
+            hip_check(q, "event query");
+        }
+        const SolveState& st = m.out_host[ju];
+        Done r{};
+        r.slot = j;
+        r.stats.iterations = std::min(st.it, st.max_it);
+        r.stats.converged = st.status == ST_CONVERGED ? 1 : 0;
+        r.stats.on_device = 1;
+        r.stats.exit_reason = st.status;
+        r.stats.residual = st.res;
+        r.stats.true_residual = st.true_res;
+        r.stats.rhs_norm = st.bnorm;
+        r.stats.solve_ms = std::chrono::duration<double, std::milli>(
+                               std::chrono::steady_clock::now() - m.t_start[ju])
+                               .count();
+        done.push_back(r);
+        m.phase[ju] = SLOT_FREE;
+        m.sys[ju] = nullptr;
+    }
+}
 
 std::unique_ptr<Runner>& System::runner_slot(cfdg_method method, bool use_amg,
                                              cfdg_precision precision) {
@@ -954,6 +1366,8 @@ std::unique_ptr<Runner>& System::runner_slot(cfdg_method method, bool use_amg,
         k = 3;
     } else if (!use_amg) {
         k = 2;
+    } else if (precision == CFDG_PREC_MIXED_BF16) {
+        k = 4;
     } else if (precision != CFDG_PREC_FP64) {
         k = 1;
     }
@@ -964,6 +1378,11 @@ void solve_batch(const std::vector<System*>& systems, const BatchOptions& opt,
                  const std::vector<cfdg_solve_control>& control,
                  std::vector<cfdg_solve_stats>& stats, hipStream_t s) {
     require(!systems.empty(), "solve_batch: no systems");
+    if (opt.use_amg && opt.method == CFDG_METHOD_CG && opt.precision == CFDG_PREC_MIXED_BF16) {
+        for (System* x : systems) {
+            x->ensure_bf16();
+        }
+    }
     Probe* probe = (systems.size() == 1 && systems[0]->probe.kind != PROBE_OFF) ? &systems[0]->probe
                                                                              : nullptr;
     if (systems.size() == 1) { // single system: keep the runner with the system
@@ -1133,10 +1552,65 @@ extern "C" cfdg_status cfdg_system_probe_read(const cfdg_system* sys, int precis
             *bytes_per_launch = fp32 ? nnz * (4.0 + 4.0) + 4.0 * 4.0 * n + 2.0 * 8.0 * n
                                      : nnz * (8.0 + 4.0) + 4.0 * 8.0 * n;
         } else if (p.kind == PROBE_OUTER_SPMV) {
-            *bytes_per_launch = nnz * (8.0 + 4.0) + 2.0 * 8.0 * n; // p in, q out
+            // z (V-cycle precision) and p_old in, p_new and q out
+            *bytes_per_launch = nnz * (8.0 + 4.0) + (fp32 ? 4.0 : 8.0) * n + 3.0 * 8.0 * n;
         } else {
             *bytes_per_launch = 0.0;
         }
+    });
+}
+
+struct cfdg_batch {
+    std::vector<System*> systems;
+    BatchOptions opt;
+    Stream stream;
+    std::unique_ptr<Runner> runner;
+};
+
+extern "C" cfdg_status cfdg_batch_create(cfdg_system* const* systems, int count, cfdg_method method,
+                                         int use_amg, cfdg_precision precision, cfdg_batch** out) {
+    return guarded([&] {
+        require(out != nullptr && systems != nullptr && count >= 1, "batch: need at least one system");
+        require(method == CFDG_METHOD_CG || method == CFDG_METHOD_BICGSTAB, "unknown method");
+        *out = nullptr;
+        auto b = std::make_unique<cfdg_batch>();
+        for (int i = 0; i < count; ++i) {
+            require(systems[i] != nullptr, "batch: null system");
+            b->systems.push_back(systems[i]->sys.get());
+        }
+        b->opt.method = method;
+        b->opt.use_amg = use_amg != 0 && method == CFDG_METHOD_CG;
+        b->opt.precision = precision;
+        *out = b.release();
+    });
+}
+
+extern "C" cfdg_status cfdg_batch_destroy(cfdg_batch* batch) {
+    return guarded([&] { delete batch; });
+}
+
+extern "C" cfdg_status cfdg_batch_solve(cfdg_batch* batch, const cfdg_solve_control* controls,
+                                        cfdg_solve_stats* stats) {
+    return guarded([&] {
+        require(batch != nullptr && controls != nullptr && stats != nullptr, "null argument");
+        const std::size_t n = batch->systems.size();
+        std::vector<cfdg_solve_control> c(controls, controls + n);
+        for (const cfdg_solve_control& x : c) {
+            check_control(&x);
+        }
+        for (System* s : batch->systems) { // uploads and fills queued on the systems' streams
+            if (batch->opt.use_amg && batch->opt.precision == CFDG_PREC_MIXED_BF16) {
+                s->ensure_bf16();
+            }
+            hip_check(hipStreamSynchronize(s->stream()), "batch: system stream sync");
+        }
+        if (!batch->runner || !batch->runner->matches(batch->systems, batch->opt)) {
+            batch->runner.reset();
+            batch->runner = std::make_unique<Runner>(batch->systems, batch->opt, batch->stream.get());
+        }
+        std::vector<cfdg_solve_stats> st;
+        batch->runner->solve(c, st, nullptr);
+        std::copy(st.begin(), st.end(), stats);
     });
 }
 

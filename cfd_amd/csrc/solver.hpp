@@ -3,7 +3,7 @@
 
 #include "ell.hpp"
 #include "hip_util.hpp"
-#include "kernels.hpp"
+#include "layout.hpp"
 
 #include <memory>
 #include <vector>
@@ -69,6 +69,9 @@ public:
     void set_rhs(const double* b);
     void set_x(const double* x);
     void get_x(double* x);
+    // Device-to-device copies of the right-hand side / iterate, queued on stream s.
+    void copy_rhs_from(const double* dev, hipStream_t s);
+    void copy_x_to(double* dev, hipStream_t s) const;
     [[nodiscard]] std::size_t footprint() const;
 
     [[nodiscard]] int n() const { return n_; }
@@ -90,7 +93,14 @@ public:
     }
 
     // Descriptor of this system for kernels of the given precision (FP64 or FP32 V-cycle).
-    [[nodiscard]] SysDev descriptor(bool fp32_vcycle, bool use_amg);
+    [[nodiscard]] SysDev descriptor(bool fp32_vcycle, bool use_amg, bool bf16 = false);
+
+    // Refreshes the BF16 copy of the level-0 operator after a new assembly.
+    void ensure_bf16() {
+        if (!bf16_ready_) {
+            make_bf16();
+        }
+    }
 
     // Generation counter, bumped when the hierarchy or vector set changes (descriptor caches).
     [[nodiscard]] unsigned generation() const { return generation_; }
@@ -109,7 +119,7 @@ private:
     DevBuf<double> dinv0_;
     DevBuf<double> wd0_64_;
     DevBuf<float> wd0_32_;
-    DevBuf<double> b_, x_, r_, p_, q_, z_, r0_, v_, s_, t_, ph_, sh_;
+    DevBuf<double> b_, x_, r_, p_, p2_, q_, z_, r0_, v_, s_, t_, ph_, sh_;
     DevBuf<float> r32_;
     DevBuf<double> partials_;
     DevBuf<SolveState> state_;
@@ -119,7 +129,9 @@ private:
     int sweeps_ = 2;
     double weight_ = 0.7;
     unsigned generation_ = 0;
-    std::unique_ptr<Runner> runners_[4];
+    std::unique_ptr<Runner> runners_[5];
+    bool bf16_ready_ = false;
+    void make_bf16();
 };
 
 // Solve options shared by every system of a batch.
@@ -135,5 +147,43 @@ struct BatchOptions {
 void solve_batch(const std::vector<System*>& systems, const BatchOptions& opt,
                  const std::vector<cfdg_solve_control>& control,
                  std::vector<cfdg_solve_stats>& stats, hipStream_t stream);
+
+// Upper bound of the device bytes a system built from this pattern and hierarchy will hold
+// (both precisions of the V-cycle, BF16 level-0 values included), from sizes and row lengths.
+std::size_t estimate_footprint(const cfdg_ldu_pattern& pattern, const cfdg_amg_view* amg);
+
+struct SlotMachinery;
+
+// Continuous batching over a fixed number of slots. Solves start and finish at iteration
+// boundaries while the others keep iterating: each step() queues one iteration that covers
+// every running solve, then reads the convergence flags of the previous iteration and retires
+// the solves that ended there. Every solve keeps its own state and reductions, so its result
+// is bit-identical to a solve of the same system on its own. One thread drives a SlotRunner.
+class SlotRunner {
+public:
+    struct Done {
+        int slot;
+        cfdg_solve_stats stats;
+    };
+    SlotRunner(int slots, const BatchOptions& opt, int sweeps, hipStream_t s);
+    ~SlotRunner();
+    SlotRunner(const SlotRunner&) = delete;
+    SlotRunner& operator=(const SlotRunner&) = delete;
+
+    [[nodiscard]] int slots() const;
+    [[nodiscard]] int free_slots() const;
+    // Solves running or finishing.
+    [[nodiscard]] bool busy() const;
+    // Starts a solve of the system's resident right-hand side from its resident iterate; its
+    // first iteration is part of the next step(). Returns the slot. The system must stay alive
+    // and untouched until the slot is reported done.
+    int start(System* sys, const cfdg_solve_control& control);
+    // One iteration boundary. Blocks until the previous iteration has finished on the device;
+    // appends the solves whose results have reached the host to `done`.
+    void step(std::vector<Done>& done);
+
+private:
+    std::unique_ptr<SlotMachinery> m_;
+};
 
 } // namespace cfdg

@@ -15,7 +15,7 @@ from __future__ import annotations
 import ctypes as C
 from dataclasses import dataclass, field
 from enum import IntEnum
-from typing import List, Optional
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -315,6 +315,51 @@ class DeviceSolver:
         v = C.c_size_t()
         L.call("cfdg_system_footprint", self._h, C.byref(v))
         return int(v.value)
+
+
+class Batch:
+    """Resident systems solved together: one kernel launch per solver stage covers every
+    member, each keeping its own reductions and stopping test. No counterpart in the
+    reference, whose DeviceSolver solves one system per call (device_solver.cpp:117-150)."""
+
+    def __init__(self, systems: Sequence[DeviceSolver], method: Method = Method.Cg,
+                 use_amg: bool = True, precision: Precision = Precision.FP64):
+        self._systems = list(systems)  # keeps the members alive while the batch exists
+        arr = (C.c_void_p * max(len(self._systems), 1))(*[s._h for s in self._systems])
+        h = C.c_void_p()
+        L.call("cfdg_batch_create", arr, len(self._systems), int(method), int(bool(use_amg)),
+               int(precision), C.byref(h))
+        self._h = h
+
+    def __len__(self) -> int:
+        return len(self._systems)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().cfdg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def solve(self, controls: Sequence[SolveControl]) -> List[SolveStats]:
+        """Solves every member from its resident right-hand side and iterate."""
+        n = len(self._systems)
+        if len(controls) != n:
+            raise L.InvalidArgument(1, "batch: one control per system")
+        cc = (L.SolveControl * n)(*[c.c() for c in controls])
+        st = (L.SolveStats * n)()
+        L.call("cfdg_batch_solve", self._h, cc, st)
+        return [SolveStats.from_c(s) for s in st]
 
 
 def assemble_convection_diffusion(n_cells: int, n_faces: int, owner, neighbour, flux, gamma,

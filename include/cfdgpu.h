@@ -200,6 +200,86 @@ cfdg_status cfdg_system_probe_read(const cfdg_system* sys, int precision, double
 cfdg_status cfdg_system_footprint(const cfdg_system* sys, size_t* bytes);
 
 /* ------------------------------------------------------------------------------------------
+ * Batched solves (no counterpart in the reference): many resident systems of different sizes
+ * solved together, one launch per solver stage for all of them. Each system keeps its own
+ * reductions, scalars and stopping test, so its result does not depend on the other members.
+ * The systems must stay alive and unchanged in structure (set_amg) while the batch exists;
+ * their matrices, right-hand sides and initial guesses may be updated between solves.
+ * ---------------------------------------------------------------------------------------- */
+
+typedef struct cfdg_batch cfdg_batch;
+
+cfdg_status cfdg_batch_create(cfdg_system* const* systems, int count, cfdg_method method,
+                              int use_amg, cfdg_precision precision, cfdg_batch** out);
+cfdg_status cfdg_batch_destroy(cfdg_batch* batch);
+/* controls and stats hold one entry per system, in creation order. */
+cfdg_status cfdg_batch_solve(cfdg_batch* batch, const cfdg_solve_control* controls,
+                             cfdg_solve_stats* stats);
+
+/* ------------------------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference): a request is one mesh's system with a run of
+ * time-step right-hand sides. Requests are admitted in arrival order while their estimated
+ * device footprint fits the memory budget; an admitted request's system and hierarchy stay
+ * resident for all of its steps, each step starting from the previous step's solution.
+ * Solves of different requests share every kernel launch (continuous batching: a solve joins
+ * or leaves the running set at any iteration boundary) and keep their own state, so every
+ * result is bit-identical to the one the request gets when served alone.
+ * ---------------------------------------------------------------------------------------- */
+
+typedef struct cfdg_server cfdg_server;
+
+typedef struct cfdg_server_options {
+    int max_active_solves;       /* solves sharing the launches (slots), default 64 */
+    cfdg_method method;          /* default CFDG_METHOD_CG */
+    int use_amg;                 /* default 1 */
+    cfdg_precision precision;    /* default CFDG_PREC_MIXED_FP32 */
+    int smooth_sweeps;           /* V-cycle sweeps every hierarchy must use, default 2 */
+    size_t memory_budget_bytes;  /* resident requests' device bytes; 0: 90% of device memory */
+} cfdg_server_options;
+
+typedef struct cfdg_server_request {
+    cfdg_ldu_pattern pattern;
+    const double* diag;          /* operator coefficients (n, n_faces, n_faces) */
+    const double* upper;
+    const double* lower;
+    const cfdg_amg_view* amg;    /* hierarchy (level 0's matrix unused); NULL without AMG */
+    int n_steps;
+    const double* const* rhs;    /* n_steps right-hand sides of n entries */
+    const double* x0;            /* initial guess of the first step; NULL: zero */
+    double rel_tolerance;        /* each step stops at |r| <= rel_tolerance |b| */
+    int max_iterations;
+    /* Outputs, written before cfdg_server_wait returns (any may be NULL): */
+    double* const* x_out;        /* n_steps solutions (entries may be NULL) */
+    cfdg_solve_stats* stats_out; /* n_steps */
+    double* step_latency_ms;     /* n_steps: from the step being ready (arrival for the first,
+                                    the previous step's result after) to its own result */
+} cfdg_server_request;
+
+typedef struct cfdg_server_counters {
+    unsigned long long submitted;
+    unsigned long long completed;
+    unsigned long long solves;
+    size_t resident_bytes;
+    size_t peak_resident_bytes;
+    size_t budget_bytes;
+    int active_solves;
+    int peak_active_solves;
+    int waiting;                 /* submitted requests not yet resident */
+} cfdg_server_counters;
+
+cfdg_status cfdg_server_default_options(cfdg_server_options* out);
+cfdg_status cfdg_server_create(const cfdg_server_options* options, cfdg_server** out);
+/* Every array the request points at must stay valid until cfdg_server_wait returns for it. */
+cfdg_status cfdg_server_submit(cfdg_server* server, const cfdg_server_request* request,
+                               unsigned long long* id);
+/* Blocks until the request has completed; latency_ms (may be NULL) receives arrival to last
+ * result. Returns the request's own status (e.g. CFDG_ERR_CAPACITY when it can never fit). */
+cfdg_status cfdg_server_wait(cfdg_server* server, unsigned long long id, double* latency_ms);
+cfdg_status cfdg_server_query(cfdg_server* server, cfdg_server_counters* out);
+/* Completes every submitted request, then stops the server. */
+cfdg_status cfdg_server_destroy(cfdg_server* server);
+
+/* ------------------------------------------------------------------------------------------
  * AMG hierarchy setup on the host (replaces fv::AmgPreconditioner::setup, amg.cpp:122-190,
  * and coarse_inverse, amg.cpp:227-240, which the reference runs through Eigen in its caller)
  * ---------------------------------------------------------------------------------------- */

@@ -130,7 +130,7 @@ def test_assembly_matches_reference_loop(oracle):
     assert np.array_equal(diag, d_ref) and np.array_equal(div, v_ref)
 
 
-@pytest.mark.parametrize("prec", [Precision.MIXED_FP32])
+@pytest.mark.parametrize("prec", [Precision.MIXED_FP32, Precision.MIXED_BF16])
 def test_mixed_precision_reaches_fp64_tolerance(oracle, prec):
     s, pat, a = _pressure_csr(32, 0.2)
     h = AmgHierarchy(a, AmgSettings(strength_decay=0.5))
@@ -156,13 +156,16 @@ def test_resident_solves_are_bitwise_reproducible():
     dev.set_rhs(s.source)
     tol = 1e-8 * np.linalg.norm(s.source)
     out = []
-    for prec in (Precision.FP64, Precision.FP64, Precision.MIXED_FP32, Precision.MIXED_FP32):
+    precs = (Precision.FP64, Precision.FP64, Precision.MIXED_FP32, Precision.MIXED_FP32,
+             Precision.MIXED_BF16, Precision.MIXED_BF16)
+    for prec in precs:
         dev.set_x(None)
         st = dev.solve_resident(Method.Cg, True, prec, SolveControl(tol, 300))
         assert st.converged
         out.append(dev.get_x())
     assert np.array_equal(out[0], out[1])
     assert np.array_equal(out[2], out[3])
+    assert np.array_equal(out[4], out[5])
 
 
 def test_empty_hierarchy_is_invalid_argument():

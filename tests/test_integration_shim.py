@@ -1,0 +1,31 @@
+"""The reference-side binding shown in INTEGRATION.md compiles against the reference's own
+cfd::gpu header and links against libcfdgpu (skipped where the reference checkout is absent)."""
+from __future__ import annotations
+
+import re
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from .conftest import ROOT
+
+REF_INCLUDE = Path("/root/reference/src/gpu/include")
+
+
+@pytest.mark.skipif(not (REF_INCLUDE / "cfd" / "gpu" / "krylov.hpp").exists() or
+                    shutil.which("g++") is None,
+                    reason="reference headers or g++ not available")
+def test_shim_compiles_and_links(tmp_path):
+    text = (ROOT / "INTEGRATION.md").read_text()
+    section = text[text.index("## 2. The binding"):]
+    code = re.search(r"```cpp\n(.*?)```", section, re.S).group(1)
+    src = tmp_path / "krylov_hip.cpp"
+    src.write_text(code)
+    out = tmp_path / "libshim.so"
+    cmd = ["g++", "-std=c++20", "-Wall", "-Werror", "-fPIC", "-shared", str(src),
+           f"-I{REF_INCLUDE}", f"-I{ROOT / 'include'}", f"-L{ROOT / 'cfd_amd' / 'lib'}",
+           "-lcfdgpu", "-Wl,--no-undefined", "-o", str(out)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr[-3000:]
