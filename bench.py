@@ -156,6 +156,86 @@ def batched(steps: int, count: int = 256) -> dict:
     }
 
 
+def served(count: int = 24, rate: float = 200.0, seed: int = 20240607) -> dict:
+    """Serving workload: requests with Poisson arrivals (rate per second, seeded), each a
+    jittered box of log-uniform 10,000..4,000,000 cells with a run of 1..20 time-step right-hand
+    sides (each the previous one plus 5% seeded noise), solved to 1e-8 from the previous step's
+    solution with the hierarchy kept resident. The same arrival sequence is replayed against
+    the mixed-precision server (continuous batching), the FP64 server, and the plain FP64 path
+    (one solve at a time). Reported: solves/s over the makespan and p50 / p99 latencies."""
+    import threading
+
+    import cfd_amd
+    from cfd_amd import AmgHierarchy, AmgSettings, Precision, cases
+
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    reqs, cells, solves = [], 0, 0
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(1e4), np.log(4e6)))))
+        m = max(4, int(round(n ** (1.0 / 3.0))))
+        nz = max(4, int(round(n / (m * m))))
+        mesh = cases.box_mesh(m, m, nz, hi=(1.0, 1.0, nz / m), jitter=0.2, seed=seed + 31 * i)
+        s = cases.pressure_system(mesh, rhs_seed=seed + 31 * i + 1)
+        pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
+        a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
+        view = AmgHierarchy(a, AmgSettings(strength_decay=0.5)).view()
+        steps = int(rng.integers(1, 21))
+        rhs = [s.source]
+        for _ in range(steps - 1):
+            noise = rng.uniform(-1.0, 1.0, s.n)
+            noise -= noise.mean()
+            rhs.append(rhs[-1] + 0.05 * np.linalg.norm(rhs[-1]) / np.linalg.norm(noise) * noise)
+        reqs.append(dict(pattern=pat, diag=s.diag, upper=s.upper, lower=s.lower, amg=view, rhs=rhs))
+        cells += s.n
+        solves += steps
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    log(f"serve: {count} requests, {cells} cells, {solves} solves, set up in "
+        f"{time.perf_counter() - t0:.1f} s")
+
+    def replay(prec, slots):
+        with cfd_amd.Server(max_active_solves=slots, precision=prec) as srv:
+            handles = [None] * count
+
+            def submitter():
+                start = time.perf_counter()
+                for i, q in enumerate(reqs):
+                    delay = arrivals[i] - (time.perf_counter() - start)
+                    if delay > 0:
+                        time.sleep(delay)
+                    handles[i] = srv.submit(**q, keep_solutions=False)
+
+            t = time.perf_counter()
+            th = threading.Thread(target=submitter)
+            th.start()
+            th.join()
+            done = [srv.wait(h) for h in handles]
+            span = time.perf_counter() - t
+            c = srv.counters()
+        lat = np.concatenate([d.step_latency_ms for d in done])
+        req_lat = np.array([d.latency_ms for d in done])
+        ok = all(st.converged for d in done for st in d.stats)
+        worst = max(st.true_residual / st.rhs_norm for d in done for st in d.stats)
+        return {"solves_per_s": round(solves / span, 2), "makespan_s": round(span, 3),
+                "p50_solve_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_solve_ms": round(float(np.percentile(lat, 99)), 3),
+                "p50_request_ms": round(float(np.percentile(req_lat, 50)), 3),
+                "p99_request_ms": round(float(np.percentile(req_lat, 99)), 3),
+                "all_converged": ok, "max_true_rel_residual": worst,
+                "peak_active_solves": c.peak_active_solves,
+                "peak_resident_gb": round(c.peak_resident_bytes / 1e9, 2)}
+
+    mixed = replay(Precision.MIXED_FP32, 64)
+    fp64 = replay(Precision.FP64, 64)
+    plain = replay(Precision.FP64, 1)
+    return {"workload": f"{count} requests, Poisson arrivals at {rate:g}/s, {cells} cells, "
+                        f"{solves} time-step solves to 1e-8",
+            "mixed_served": mixed, "fp64_served": fp64, "fp64_one_at_a_time": plain,
+            "speedup_mixed_vs_fp64_served": round(mixed["solves_per_s"] / fp64["solves_per_s"], 3),
+            "speedup_mixed_vs_fp64_one_at_a_time":
+                round(mixed["solves_per_s"] / plain["solves_per_s"], 3)}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,7 +245,21 @@ def main(argv=None) -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 10 device copies of 256 MiB (HBM counter calibration)")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(32 * 2 ** 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
+                          "copies": 10}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(batched and served systems shard across GPUs by system)"}))
@@ -322,6 +416,8 @@ def main(argv=None) -> int:
     if not args.skip_batch:
         del dev, mdev
         out["batched"] = batched(max(2, args.steps // 4))
+    if not args.skip_serve:
+        out["served"] = served()
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(view, b, tol, args.cpu_seconds)
     print(json.dumps(out), flush=True)

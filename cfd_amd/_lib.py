@@ -95,6 +95,49 @@ class LduPattern(C.Structure):
     ]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [
+        ("max_active_solves", C.c_int),
+        ("method", C.c_int),
+        ("use_amg", C.c_int),
+        ("precision", C.c_int),
+        ("smooth_sweeps", C.c_int),
+        ("memory_budget_bytes", C.c_size_t),
+    ]
+
+
+class ServerRequest(C.Structure):
+    _fields_ = [
+        ("pattern", LduPattern),
+        ("diag", C.POINTER(C.c_double)),
+        ("upper", C.POINTER(C.c_double)),
+        ("lower", C.POINTER(C.c_double)),
+        ("amg", C.POINTER(AmgView)),
+        ("n_steps", C.c_int),
+        ("rhs", C.POINTER(C.POINTER(C.c_double))),
+        ("x0", C.POINTER(C.c_double)),
+        ("rel_tolerance", C.c_double),
+        ("max_iterations", C.c_int),
+        ("x_out", C.POINTER(C.POINTER(C.c_double))),
+        ("stats_out", C.POINTER(SolveStats)),
+        ("step_latency_ms", C.POINTER(C.c_double)),
+    ]
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_ulonglong),
+        ("completed", C.c_ulonglong),
+        ("solves", C.c_ulonglong),
+        ("resident_bytes", C.c_size_t),
+        ("peak_resident_bytes", C.c_size_t),
+        ("budget_bytes", C.c_size_t),
+        ("active_solves", C.c_int),
+        ("peak_active_solves", C.c_int),
+        ("waiting", C.c_int),
+    ]
+
+
 class AmgSettings(C.Structure):
     _fields_ = [
         ("strength_threshold", C.c_double),
@@ -155,6 +198,14 @@ _SIGNATURES = {
                                C.POINTER(VP)]),
     "cfdg_batch_destroy": (ST, [VP]),
     "cfdg_batch_solve": (ST, [VP, C.POINTER(SolveControl), C.POINTER(SolveStats)]),
+    "cfdg_estimate_footprint": (ST, [C.POINTER(LduPattern), C.POINTER(AmgView),
+                                     C.POINTER(C.c_size_t)]),
+    "cfdg_server_default_options": (ST, [C.POINTER(ServerOptions)]),
+    "cfdg_server_create": (ST, [C.POINTER(ServerOptions), C.POINTER(VP)]),
+    "cfdg_server_submit": (ST, [VP, C.POINTER(ServerRequest), C.POINTER(C.c_ulonglong)]),
+    "cfdg_server_wait": (ST, [VP, C.c_ulonglong, P_DBL]),
+    "cfdg_server_query": (ST, [VP, C.POINTER(ServerCounters)]),
+    "cfdg_server_destroy": (ST, [VP]),
     "cfdg_system_probe": (ST, [VP, C.c_int]),
     "cfdg_system_probe_read": (ST, [VP, C.c_int, P_DBL, C.POINTER(C.c_longlong), P_DBL]),
     "cfdg_amg_default_settings": (ST, [C.POINTER(AmgSettings)]),

@@ -479,6 +479,14 @@ struct cfdg_server {
     std::unique_ptr<Server> s;
 };
 
+extern "C" cfdg_status cfdg_estimate_footprint(const cfdg_ldu_pattern* pattern, const cfdg_amg_view* amg,
+                                               size_t* bytes) {
+    return guarded([&] {
+        require(pattern != nullptr && bytes != nullptr, "null argument");
+        *bytes = estimate_footprint(*pattern, amg);
+    });
+}
+
 extern "C" cfdg_status cfdg_server_default_options(cfdg_server_options* out) {
     return guarded([&] {
         require(out != nullptr, "null argument");

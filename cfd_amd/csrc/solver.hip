@@ -36,6 +36,10 @@ __device__ __forceinline__ unsigned short float_to_bf16(float f) { // round to n
 // BF16 copy of a sliced-ELL FP64 operator: per row the scale s = max |off-diagonal| (1 if
 // none), off-diagonal slots bf16(a / s), the diagonal slot 0 and the diagonal in FP32. The
 // diagonal is the first slot of the row holding its own column (padding slots come after it).
+// The stored diagonal absorbs the rounding of the off-diagonals, d' = d + sum_j (a_ij - s
+// bf16(a_ij / s)), so that row sums (the operator applied to a constant) are kept: rounding
+// errors then act like a small perturbation of the operator's differences and leave the
+// near-null space of diffusion operators alone.
 __global__ void __launch_bounds__(k_block) k_make_bf16(EllView m, unsigned short* __restrict__ vbf,
                                                        float* __restrict__ rscale, float* __restrict__ diag) {
     const int row = blockIdx.x * k_block + threadIdx.x;
@@ -61,13 +65,19 @@ __global__ void __launch_bounds__(k_block) k_make_bf16(EllView m, unsigned short
             mx = fmax(mx, fabs(v));
         }
     }
-    const double s = mx > 0.0 ? mx : 1.0;
+    const float sf = static_cast<float>(mx > 0.0 ? mx : 1.0);
+    double lost = 0.0;
     for (int k = 0; k < w; ++k) {
-        vbf[base + lane + k * k_wave] = k == kd ? static_cast<unsigned short>(0)
-                                                : float_to_bf16(static_cast<float>(val[k * k_wave] / s));
+        unsigned short h = 0;
+        if (k != kd) {
+            const double v = val[k * k_wave];
+            h = float_to_bf16(static_cast<float>(v / static_cast<double>(sf)));
+            lost += v - static_cast<double>(sf) * static_cast<double>(__uint_as_float(static_cast<unsigned>(h) << 16));
+        }
+        vbf[base + lane + k * k_wave] = h;
     }
-    rscale[row] = static_cast<float>(s);
-    diag[row] = static_cast<float>(d);
+    rscale[row] = sf;
+    diag[row] = static_cast<float>(d + lost);
 }
 
 __global__ void __launch_bounds__(k_block) k_scale_dinv(int n, const double* __restrict__ dinv, double w,
@@ -1546,14 +1556,22 @@ extern "C" cfdg_status cfdg_system_probe_read(const cfdg_system* sys, int precis
         const double n = static_cast<double>(s.n());
         const double nnz = static_cast<double>(s.nnz0());
         const bool fp32 = precision != CFDG_PREC_FP64;
+        const bool bf16 = precision == CFDG_PREC_MIXED_BF16;
+        const double idx = s.index_bytes0(); // 16-bit column offsets or 32-bit columns
         if (p.kind == PROBE_L0_SWEEP) {
             // matrix (value + column index) + x_in, b, w*dinv, x_out at the V-cycle precision;
-            // the mixed sweep also reads r and q in FP64 for its two dot products.
-            *bytes_per_launch = fp32 ? nnz * (4.0 + 4.0) + 4.0 * 4.0 * n + 2.0 * 8.0 * n
-                                     : nnz * (8.0 + 4.0) + 4.0 * 8.0 * n;
+            // the mixed sweep also reads r and q in FP64 for its two dot products, and the
+            // BF16 operator its FP32 row scales and diagonal.
+            if (bf16) {
+                *bytes_per_launch = nnz * (2.0 + idx) + 2.0 * 4.0 * n + 4.0 * 4.0 * n + 2.0 * 8.0 * n;
+            } else if (fp32) {
+                *bytes_per_launch = nnz * (4.0 + idx) + 4.0 * 4.0 * n + 2.0 * 8.0 * n;
+            } else {
+                *bytes_per_launch = nnz * (8.0 + idx) + 4.0 * 8.0 * n;
+            }
         } else if (p.kind == PROBE_OUTER_SPMV) {
             // z (V-cycle precision) and p_old in, p_new and q out
-            *bytes_per_launch = nnz * (8.0 + 4.0) + (fp32 ? 4.0 : 8.0) * n + 3.0 * 8.0 * n;
+            *bytes_per_launch = nnz * (8.0 + idx) + (fp32 ? 4.0 : 8.0) * n + 3.0 * 8.0 * n;
         } else {
             *bytes_per_launch = 0.0;
         }

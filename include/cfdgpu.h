@@ -267,6 +267,10 @@ typedef struct cfdg_server_counters {
     int waiting;                 /* submitted requests not yet resident */
 } cfdg_server_counters;
 
+/* Upper bound of the device bytes a system of this pattern and hierarchy (may be NULL) holds;
+ * host-only, usable for capacity planning without a device. */
+cfdg_status cfdg_estimate_footprint(const cfdg_ldu_pattern* pattern, const cfdg_amg_view* amg,
+                                    size_t* bytes);
 cfdg_status cfdg_server_default_options(cfdg_server_options* out);
 cfdg_status cfdg_server_create(const cfdg_server_options* options, cfdg_server** out);
 /* Every array the request points at must stay valid until cfdg_server_wait returns for it. */

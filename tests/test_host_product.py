@@ -41,6 +41,22 @@ def test_no_device_is_an_error_not_a_fallback():
     pat = cases.csr_pattern(8, np.array([0, 1, 2], np.int32), np.array([1, 2, 3], np.int32))
     with pytest.raises(L.NoDevice):
         cfd_amd.DeviceSolver(pat)
+    with pytest.raises(L.NoDevice):
+        cfd_amd.Server()
+
+
+def test_footprint_estimate_bounds_the_stored_layout():
+    """The admission estimate covers at least every stored slot (value, FP32 copy, BF16 copy,
+    column offset, LDU source) and the 15 FP64 work vectors; it is host-only."""
+    mesh = cases.box_mesh(12, 10, 9, jitter=0.2, seed=3)
+    s = cases.pressure_system(mesh)
+    pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
+    plain = cfd_amd.estimate_footprint(pat)
+    assert plain >= pat.nnz * (8 + 4 + 2 + 2 + 4) + 15 * 8 * s.n
+    h = AmgHierarchy(cases.ldu_to_csr(pat, s.diag, s.upper, s.lower), AmgSettings(strength_decay=0.5))
+    sizes = h.level_sizes()
+    with_amg = cfd_amd.estimate_footprint(pat, h.view())
+    assert with_amg > plain + 8 * sizes[-1] ** 2  # coarse levels plus the dense coarsest inverse
 
 
 def test_invalid_arguments_are_reported():
