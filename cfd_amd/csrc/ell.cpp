@@ -68,7 +68,7 @@ int pick_tpr(int n_rows, long long nnz) {
     // Tuning knobs (environment): rows with at most CFDG_ELL_MAX_AVG entries on average stay
     // thread-per-row ELL; longer rows get lanes so that each lane sums about
     // CFDG_TPR_ENTRIES entries.
-    static const double ell_max = env_or("CFDG_ELL_MAX_AVG", 40.0);
+    static const double ell_max = env_or("CFDG_ELL_MAX_AVG", 12.0);
     static const double per_lane = env_or("CFDG_TPR_ENTRIES", 4.0);
     const double avg = static_cast<double>(nnz) / static_cast<double>(n_rows);
     if (avg <= ell_max) {

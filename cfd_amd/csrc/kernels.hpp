@@ -84,7 +84,18 @@ struct ColIdx {
     }
 };
 
-// Storage-order row sum of sliced-ELL slots with values val (type V) converted to T.
+template <class T, class V>
+__device__ __forceinline__ T value_at(const V* __restrict__ val, int i) {
+    if constexpr (sizeof(V) == 2) {
+        return bf16_to_float(val[i]);
+    } else {
+        return val[i];
+    }
+}
+
+// Storage-order row sum of sliced-ELL slots with values val (type V) converted to T. Loads are
+// issued four entries at a time, the tail (w mod 4, uniform over the slice's wavefront) as one
+// more group with its unused loads repeating a valid slot; only real entries are summed.
 template <class T, class V, class F>
 __device__ __forceinline__ T ell_sum(const ColIdx& ci, const V* __restrict__ val, int w, F load) {
     T s = T(0);
@@ -94,18 +105,10 @@ __device__ __forceinline__ T ell_sum(const ColIdx& ci, const V* __restrict__ val
         const int c1 = ci((k + 1) * k_wave);
         const int c2 = ci((k + 2) * k_wave);
         const int c3 = ci((k + 3) * k_wave);
-        T a0, a1, a2, a3;
-        if constexpr (sizeof(V) == 2) {
-            a0 = bf16_to_float(val[(k + 0) * k_wave]);
-            a1 = bf16_to_float(val[(k + 1) * k_wave]);
-            a2 = bf16_to_float(val[(k + 2) * k_wave]);
-            a3 = bf16_to_float(val[(k + 3) * k_wave]);
-        } else {
-            a0 = val[(k + 0) * k_wave];
-            a1 = val[(k + 1) * k_wave];
-            a2 = val[(k + 2) * k_wave];
-            a3 = val[(k + 3) * k_wave];
-        }
+        const T a0 = value_at<T>(val, (k + 0) * k_wave);
+        const T a1 = value_at<T>(val, (k + 1) * k_wave);
+        const T a2 = value_at<T>(val, (k + 2) * k_wave);
+        const T a3 = value_at<T>(val, (k + 3) * k_wave);
         const T x0 = load(c0);
         const T x1 = load(c1);
         const T x2 = load(c2);
@@ -115,14 +118,75 @@ __device__ __forceinline__ T ell_sum(const ColIdx& ci, const V* __restrict__ val
         s = madd(a2, x2, s);
         s = madd(a3, x3, s);
     }
-    for (; k < w; ++k) {
-        T a;
-        if constexpr (sizeof(V) == 2) {
-            a = bf16_to_float(val[k * k_wave]);
-        } else {
-            a = val[k * k_wave];
+    const int rem = w - k;
+    if (rem > 0) {
+        const int k1 = rem > 1 ? k + 1 : k;
+        const int k2 = rem > 2 ? k + 2 : k;
+        const int c0 = ci(k * k_wave);
+        const int c1 = ci(k1 * k_wave);
+        const int c2 = ci(k2 * k_wave);
+        const T a0 = value_at<T>(val, k * k_wave);
+        const T a1 = value_at<T>(val, k1 * k_wave);
+        const T a2 = value_at<T>(val, k2 * k_wave);
+        const T x0 = load(c0);
+        const T x1 = load(c1);
+        const T x2 = load(c2);
+        s = madd(a0, x0, s);
+        if (rem > 1) {
+            s = madd(a1, x1, s);
         }
-        s = madd(a, load(ci(k * k_wave)), s);
+        if (rem > 2) {
+            s = madd(a2, x2, s);
+        }
+    }
+    return s;
+}
+
+// Sum over the CSR entries k, k + step, ... < end of one lane, in that order, four loads at a
+// time (the lane's share of a row read by step lanes).
+template <class T, class F>
+__device__ __forceinline__ T csr_lane_sum(const int* __restrict__ col, const T* __restrict__ val, int k,
+                                          int end, int step, F load) {
+    T s = T(0);
+    for (; k + 3 * step < end; k += 4 * step) {
+        const int c0 = col[k];
+        const int c1 = col[k + step];
+        const int c2 = col[k + 2 * step];
+        const int c3 = col[k + 3 * step];
+        const T a0 = val[k];
+        const T a1 = val[k + step];
+        const T a2 = val[k + 2 * step];
+        const T a3 = val[k + 3 * step];
+        const T x0 = load(c0);
+        const T x1 = load(c1);
+        const T x2 = load(c2);
+        const T x3 = load(c3);
+        s = madd(a0, x0, s);
+        s = madd(a1, x1, s);
+        s = madd(a2, x2, s);
+        s = madd(a3, x3, s);
+    }
+    if (k < end) {
+        const bool has1 = k + step < end;
+        const bool has2 = k + 2 * step < end;
+        const int k1 = has1 ? k + step : k;
+        const int k2 = has2 ? k + 2 * step : k;
+        const int c0 = col[k];
+        const int c1 = col[k1];
+        const int c2 = col[k2];
+        const T a0 = val[k];
+        const T a1 = val[k1];
+        const T a2 = val[k2];
+        const T x0 = load(c0);
+        const T x1 = load(c1);
+        const T x2 = load(c2);
+        s = madd(a0, x0, s);
+        if (has1) {
+            s = madd(a1, x1, s);
+        }
+        if (has2) {
+            s = madd(a2, x2, s);
+        }
     }
     return s;
 }
@@ -181,12 +245,8 @@ __device__ __forceinline__ T mat_row(const EllView& m, int row, int sub, const T
     }
     T s = T(0);
     if (row < m.n_rows) {
-        const int* __restrict__ col = m.col;
-        const T* __restrict__ val = static_cast<const T*>(m.val);
-        const int end = m.sp[row + 1];
-        for (int k = m.sp[row] + sub; k < end; k += m.tpr) {
-            s = madd(val[k], x[col[k]], s);
-        }
+        s = csr_lane_sum<T>(m.col, static_cast<const T*>(m.val), m.sp[row] + sub, m.sp[row + 1], m.tpr,
+                            [&](int c) { return x[c]; });
     }
     return group_sum(s, m.tpr);
 }
@@ -200,13 +260,8 @@ __device__ __forceinline__ T mat_row_x1(const EllView& m, int row, int sub, cons
     }
     T s = T(0);
     if (row < m.n_rows) {
-        const int* __restrict__ col = m.col;
-        const T* __restrict__ val = static_cast<const T*>(m.val);
-        const int end = m.sp[row + 1];
-        for (int k = m.sp[row] + sub; k < end; k += m.tpr) {
-            const int c = col[k];
-            s = madd(val[k], wd[c] * b[c], s);
-        }
+        s = csr_lane_sum<T>(m.col, static_cast<const T*>(m.val), m.sp[row] + sub, m.sp[row + 1], m.tpr,
+                            [&](int c) { return wd[c] * b[c]; });
     }
     return group_sum(s, m.tpr);
 }

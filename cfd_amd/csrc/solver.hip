@@ -9,6 +9,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -951,6 +952,7 @@ public:
                 require(sys_[i]->sweeps() == sweeps, "solve_batch: systems differ in smooth_sweeps");
             }
             gens_.push_back(sys_[i]->generation());
+            uids_.push_back(sys_[i]->uid());
         }
         pl_ = Plan{opt.method, amg, fp32, sweeps};
         precision_ = amg ? opt.precision : CFDG_PREC_FP64;
@@ -983,7 +985,7 @@ public:
             return false;
         }
         for (std::size_t i = 0; i < sys_.size(); ++i) {
-            if (sys_[i]->generation() != gens_[i]) {
+            if (sys_[i]->generation() != gens_[i] || sys_[i]->uid() != uids_[i]) {
                 return false;
             }
         }
@@ -1121,6 +1123,7 @@ private:
 
     std::vector<System*> sys_;
     std::vector<unsigned> gens_;
+    std::vector<unsigned long long> uids_;
     hipStream_t s_;
     Plan pl_{};
     Tables tb_;
@@ -1142,6 +1145,11 @@ private:
 };
 
 System::~System() = default;
+
+unsigned long long System::next_uid() {
+    static std::atomic<unsigned long long> counter{0};
+    return ++counter;
+}
 
 namespace {
 
@@ -1173,8 +1181,9 @@ struct SlotMachinery {
     std::vector<System*> sys;   // per slot
     std::vector<int> phase;
     std::vector<Tables> single; // per slot: tables of the slot's system alone
-    std::vector<System*> single_of;
+    std::vector<unsigned long long> single_of; // uid of the system the single tables describe
     std::vector<unsigned> single_gen;
+    std::vector<long long> epoch; // per slot: number of the solve occupying it
     std::vector<std::unique_ptr<Event>> done_ev;
     std::vector<std::chrono::steady_clock::time_point> t_start;
     std::vector<int> running;   // slots covered by the next iteration
@@ -1182,7 +1191,8 @@ struct SlotMachinery {
     int cur = 0;
     bool dirty = false;
     long long g = 1;            // number of the next iteration
-    std::vector<int> covered[2];
+    std::vector<std::pair<int, long long>> covered[2]; // (slot, epoch) of each covered solve
+    long long solves_started = 0;
     std::vector<long long> inflight; // iterations queued whose flags have not been read
     Event ev[2];
 };
@@ -1209,8 +1219,9 @@ SlotRunner::SlotRunner(int slots, const BatchOptions& opt, int sweeps, hipStream
     m.sys.assign(k, nullptr);
     m.phase.assign(k, SLOT_FREE);
     m.single = std::vector<Tables>(k);
-    m.single_of.assign(k, nullptr);
+    m.single_of.assign(k, 0);
     m.single_gen.assign(k, 0);
+    m.epoch.assign(k, -1);
     m.t_start.resize(k);
     for (std::size_t i = 0; i < k; ++i) {
         m.done_ev.push_back(std::make_unique<Event>());
@@ -1257,11 +1268,11 @@ int SlotRunner::start(System* x, const cfdg_solve_control& c) {
     hip_check(hipMemcpyAsync(m.desc.get() + j, m.desc_host.get() + j, sizeof(SysDev),
                              hipMemcpyHostToDevice, m.s),
               "slot descriptor H2D");
-    if (m.single_of[ju] != x || m.single_gen[ju] != x->generation()) {
+    if (m.single_of[ju] != x->uid() || m.single_gen[ju] != x->generation()) {
         std::vector<System*> one{x};
         std::vector<int> ids{j};
         build_tables(m.single[ju], one, m.pl.amg, m.s, &ids, true);
-        m.single_of[ju] = x;
+        m.single_of[ju] = x->uid();
         m.single_gen[ju] = x->generation();
     }
     SolveState& st = m.init_host[ju];
@@ -1274,6 +1285,7 @@ int SlotRunner::start(System* x, const cfdg_solve_control& c) {
     enqueue_init(m.pl, m.desc.get(), m.single[ju], m.s);
     hip_check(hipGetLastError(), "slot init launch");
     m.sys[ju] = x;
+    m.epoch[ju] = m.solves_started++;
     m.phase[ju] = SLOT_RUNNING;
     m.t_start[ju] = std::chrono::steady_clock::now();
     m.running.push_back(j);
@@ -1302,7 +1314,10 @@ void SlotRunner::step(std::vector<Done>& done) {
             d, tb.ids.get(), tb.n_sys, m.status_dev + static_cast<std::size_t>(parity) * m.k);
         hip_check(hipEventRecord(m.ev[parity].get(), m.s), "event record");
         hip_check(hipGetLastError(), "slot iteration launch");
-        m.covered[parity] = m.running;
+        m.covered[parity].clear();
+        for (int j : m.running) {
+            m.covered[parity].emplace_back(j, m.epoch[static_cast<std::size_t>(j)]);
+        }
         m.inflight.push_back(m.g);
         ++m.g;
     }
@@ -1314,10 +1329,12 @@ void SlotRunner::step(std::vector<Done>& done) {
         const int hp = static_cast<int>(h & 1);
         hip_check(hipEventSynchronize(m.ev[hp].get()), "event sync");
         const int* flags = m.status.get() + static_cast<std::size_t>(hp) * m.k;
-        for (int j : m.covered[hp]) {
+        for (const auto& [j, ep] : m.covered[hp]) {
             const std::size_t ju = static_cast<std::size_t>(j);
             const int f = flags[ju];
-            if (m.phase[ju] != SLOT_RUNNING || f == ST_RUNNING || f == ST_CONVERGED_PENDING) {
+            // Flags of a slot's earlier solve say nothing about the one that occupies it now.
+            if (m.phase[ju] != SLOT_RUNNING || m.epoch[ju] != ep || f == ST_RUNNING ||
+                f == ST_CONVERGED_PENDING) {
                 continue;
             }
             const Tables& one = m.single[ju];

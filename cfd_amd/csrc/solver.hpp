@@ -106,6 +106,8 @@ public:
 
     // Generation counter, bumped when the hierarchy or vector set changes (descriptor caches).
     [[nodiscard]] unsigned generation() const { return generation_; }
+    // Process-wide unique identity of this system (addresses of destroyed systems are reused).
+    [[nodiscard]] unsigned long long uid() const { return uid_; }
 
 private:
     void init_vectors();
@@ -131,6 +133,8 @@ private:
     int sweeps_ = 2;
     double weight_ = 0.7;
     unsigned generation_ = 0;
+    unsigned long long uid_ = next_uid();
+    static unsigned long long next_uid();
     std::unique_ptr<Runner> runners_[5];
     bool bf16_ready_ = false;
     void make_bf16();
