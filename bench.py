@@ -378,11 +378,15 @@ def main(argv=None) -> int:
     probe_spmv = dev.probe_read(Precision.FP64)
     dev.probe(0)
 
-    def roof(p, name):
+    # HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE) for this workload
+    traffic_path = ROOT / "profiles" / "traffic.json"
+    traffic = json.loads(traffic_path.read_text())["hbm_bytes_per_launch"] if traffic_path.exists() else {}
+
+    def roof(p, name, key):
         ms, launches, nbytes = p
         gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get(key), "kernel": name,
                 "bytes_per_launch": int(nbytes), "avg_us": round(ms * 1e3, 3),
                 "launches": int(launches)}
     log(f"mixed: {args.steps / t_mixed:.2f} solves/s, it {its_m[-1]}, true rel res "
@@ -450,9 +454,12 @@ def main(argv=None) -> int:
                                    "true_rel_residual": st_mom.true_residual / np.linalg.norm(m_sys.source)},
         "setup_ms": round(hier.setup_ms, 1),
         "footprint_bytes": dev.footprint(),
-        "roofline": roof(probe_m, "k_jacobi<float,2>: last level-0 FP32 Jacobi sweep + r.z, z.q"),
-        "roofline_fp64_sweep": roof(probe_d, "k_jacobi<double,1>: last level-0 FP64 Jacobi sweep + r.z"),
-        "roofline_fp64_spmv": roof(probe_spmv, "k_spmv_dot<false>: FP64 q = A p + p.q"),
+        "roofline": roof(probe_m, "k_jacobi<float,2>: last level-0 FP32 Jacobi sweep + r.z, z.q",
+                         "k_jacobi<float, 2>@8192"),
+        "roofline_fp64_sweep": roof(probe_d, "k_jacobi<double,1>: last level-0 FP64 Jacobi sweep + r.z",
+                                    "k_jacobi<double, 1>@8192"),
+        "roofline_fp64_spmv": roof(probe_spmv, "k_spmv_pz<double>: FP64 q = A p_new, p_new = z + beta p_old, p.q",
+                                   "k_spmv_pz<double>@8192"),
         "working_set_mb": round(dev.footprint() / 1e6, 1),
         "cpu_baseline": None,
     }

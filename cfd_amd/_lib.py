@@ -103,6 +103,7 @@ class ServerOptions(C.Structure):
         ("precision", C.c_int),
         ("smooth_sweeps", C.c_int),
         ("memory_budget_bytes", C.c_size_t),
+        ("staging_threads", C.c_int),
     ]
 
 

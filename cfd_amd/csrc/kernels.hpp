@@ -74,10 +74,21 @@ __device__ __forceinline__ float bf16_to_float(unsigned short h) {
     return __uint_as_float(static_cast<unsigned>(h) << 16);
 }
 
+// Device buffers seen in the global address space: loads through such pointers compile to
+// global_load with a scalar base instead of flat loads (which also count against the LDS
+// counter and need a full 64-bit address per lane). Descriptor pointers are generic.
+template <class T>
+using gmem_t = __attribute__((address_space(1))) T;
+
+template <class T>
+__device__ __forceinline__ gmem_t<T>* gm(T* p) {
+    return (gmem_t<T>*)p;
+}
+
 // Column index of slot k: int32 column or row + 16-bit offset.
 struct ColIdx {
-    const int* __restrict__ col;
-    const short* __restrict__ dcol;
+    const gmem_t<const int>* __restrict__ col;
+    const gmem_t<const short>* __restrict__ dcol;
     int row;
     __device__ __forceinline__ int operator()(int k) const {
         return dcol != nullptr ? row + static_cast<int>(dcol[k]) : col[k];
@@ -85,7 +96,7 @@ struct ColIdx {
 };
 
 template <class T, class V>
-__device__ __forceinline__ T value_at(const V* __restrict__ val, int i) {
+__device__ __forceinline__ T value_at(const gmem_t<const V>* __restrict__ val, int i) {
     if constexpr (sizeof(V) == 2) {
         return bf16_to_float(val[i]);
     } else {
@@ -97,7 +108,7 @@ __device__ __forceinline__ T value_at(const V* __restrict__ val, int i) {
 // issued four entries at a time, the tail (w mod 4, uniform over the slice's wavefront) as one
 // more group with its unused loads repeating a valid slot; only real entries are summed.
 template <class T, class V, class F>
-__device__ __forceinline__ T ell_sum(const ColIdx& ci, const V* __restrict__ val, int w, F load) {
+__device__ __forceinline__ T ell_sum(const ColIdx& ci, const gmem_t<const V>* __restrict__ val, int w, F load) {
     T s = T(0);
     int k = 0;
     for (; k + 4 <= w; k += 4) {
@@ -105,10 +116,10 @@ __device__ __forceinline__ T ell_sum(const ColIdx& ci, const V* __restrict__ val
         const int c1 = ci((k + 1) * k_wave);
         const int c2 = ci((k + 2) * k_wave);
         const int c3 = ci((k + 3) * k_wave);
-        const T a0 = value_at<T>(val, (k + 0) * k_wave);
-        const T a1 = value_at<T>(val, (k + 1) * k_wave);
-        const T a2 = value_at<T>(val, (k + 2) * k_wave);
-        const T a3 = value_at<T>(val, (k + 3) * k_wave);
+        const T a0 = value_at<T, V>(val, (k + 0) * k_wave);
+        const T a1 = value_at<T, V>(val, (k + 1) * k_wave);
+        const T a2 = value_at<T, V>(val, (k + 2) * k_wave);
+        const T a3 = value_at<T, V>(val, (k + 3) * k_wave);
         const T x0 = load(c0);
         const T x1 = load(c1);
         const T x2 = load(c2);
@@ -125,9 +136,9 @@ __device__ __forceinline__ T ell_sum(const ColIdx& ci, const V* __restrict__ val
         const int c0 = ci(k * k_wave);
         const int c1 = ci(k1 * k_wave);
         const int c2 = ci(k2 * k_wave);
-        const T a0 = value_at<T>(val, k * k_wave);
-        const T a1 = value_at<T>(val, k1 * k_wave);
-        const T a2 = value_at<T>(val, k2 * k_wave);
+        const T a0 = value_at<T, V>(val, k * k_wave);
+        const T a1 = value_at<T, V>(val, k1 * k_wave);
+        const T a2 = value_at<T, V>(val, k2 * k_wave);
         const T x0 = load(c0);
         const T x1 = load(c1);
         const T x2 = load(c2);
@@ -145,8 +156,9 @@ __device__ __forceinline__ T ell_sum(const ColIdx& ci, const V* __restrict__ val
 // Sum over the CSR entries k, k + step, ... < end of one lane, in that order, four loads at a
 // time (the lane's share of a row read by step lanes).
 template <class T, class F>
-__device__ __forceinline__ T csr_lane_sum(const int* __restrict__ col, const T* __restrict__ val, int k,
-                                          int end, int step, F load) {
+__device__ __forceinline__ T csr_lane_sum(const gmem_t<const int>* __restrict__ col,
+                                          const gmem_t<const T>* __restrict__ val, int k, int end, int step,
+                                          F load) {
     T s = T(0);
     for (; k + 3 * step < end; k += 4 * step) {
         const int c0 = col[k];
@@ -198,30 +210,31 @@ __device__ __forceinline__ T ell_row_f(const EllView& m, int slice, int lane, F 
     const int base = m.sp[slice];
     const int w = (m.sp[slice + 1] - base) / k_wave;
     const int row = slice * k_wave + lane;
-    const ColIdx ci{m.col == nullptr ? nullptr : m.col + base + lane,
-                    m.dcol == nullptr ? nullptr : m.dcol + base + lane, row};
+    const ColIdx ci{m.col == nullptr ? nullptr : gm(m.col) + base + lane,
+                    m.dcol == nullptr ? nullptr : gm(m.dcol) + base + lane, row};
     if constexpr (sizeof(T) == 4) {
         if (m.fmt == 1) {
-            const unsigned short* val = static_cast<const unsigned short*>(m.val) + base + lane;
-            const T s = ell_sum<T>(ci, val, w, load);
-            const T dx = row < m.n_rows ? m.diag[row] * load(row) : T(0);
-            return madd(m.rscale[row], s, dx);
+            const auto val = gm(static_cast<const unsigned short*>(m.val)) + base + lane;
+            const T s = ell_sum<T, unsigned short>(ci, val, w, load);
+            const T dx = row < m.n_rows ? gm(m.diag)[row] * load(row) : T(0);
+            return madd(gm(m.rscale)[row], s, dx);
         }
     }
-    return ell_sum<T>(ci, static_cast<const T*>(m.val) + base + lane, w, load);
+    return ell_sum<T, T>(ci, gm(static_cast<const T*>(m.val)) + base + lane, w, load);
 }
 
 // Row sum of an ELL row: sum_k val_k * x[col_k] in storage order (any column / value format).
 template <class T>
 __device__ __forceinline__ T ell_row(const EllView& m, int slice, int lane, const T* __restrict__ x) {
-    return ell_row_f<T>(m, slice, lane, [&](int c) { return x[c]; });
+    const auto gx = gm(x);
+    return ell_row_f<T>(m, slice, lane, [&](int c) { return gx[c]; });
 }
 
 // Row sum with the first Jacobi iterate x1_j = wd_j * b_j formed on the fly (the reference's
 // first sweep multiplies A by x = 0, so x1 = (w dinv) b exactly).
 template <class T>
-__device__ __forceinline__ T ell_row_x1(const EllView& m, int slice, int lane, const T* __restrict__ wd,
-                                        const T* __restrict__ b) {
+__device__ __forceinline__ T ell_row_x1(const EllView& m, int slice, int lane, const gmem_t<const T>* __restrict__ wd,
+                                        const gmem_t<const T>* __restrict__ b) {
     return ell_row_f<T>(m, slice, lane, [&](int c) { return wd[c] * b[c]; });
 }
 
@@ -245,23 +258,24 @@ __device__ __forceinline__ T mat_row(const EllView& m, int row, int sub, const T
     }
     T s = T(0);
     if (row < m.n_rows) {
-        s = csr_lane_sum<T>(m.col, static_cast<const T*>(m.val), m.sp[row] + sub, m.sp[row + 1], m.tpr,
-                            [&](int c) { return x[c]; });
+        const auto gx = gm(x);
+        s = csr_lane_sum<T>(gm(m.col), gm(static_cast<const T*>(m.val)), m.sp[row] + sub, m.sp[row + 1],
+                            m.tpr, [&](int c) { return gx[c]; });
     }
     return group_sum(s, m.tpr);
 }
 
 template <class T>
-__device__ __forceinline__ T mat_row_x1(const EllView& m, int row, int sub, const T* __restrict__ wd,
-                                        const T* __restrict__ b) {
+__device__ __forceinline__ T mat_row_x1(const EllView& m, int row, int sub, const gmem_t<const T>* __restrict__ wd,
+                                        const gmem_t<const T>* __restrict__ b) {
     if (m.tpr == 1) {
         const int slice = row / k_wave;
         return slice < m.n_slices ? ell_row_x1<T>(m, slice, row % k_wave, wd, b) : T(0);
     }
     T s = T(0);
     if (row < m.n_rows) {
-        s = csr_lane_sum<T>(m.col, static_cast<const T*>(m.val), m.sp[row] + sub, m.sp[row + 1], m.tpr,
-                            [&](int c) { return wd[c] * b[c]; });
+        s = csr_lane_sum<T>(gm(m.col), gm(static_cast<const T*>(m.val)), m.sp[row] + sub, m.sp[row + 1],
+                            m.tpr, [&](int c) { return wd[c] * b[c]; });
     }
     return group_sum(s, m.tpr);
 }
@@ -323,13 +337,13 @@ __global__ void __launch_bounds__(k_block) k_residual0(const SysDev* __restrict_
     if (slice < S.a64.n_slices) {
         const double ax = ell_row<double>(S.a64, slice, lane, S.x);
         if (row < S.n) {
-            const double bi = S.b[row];
+            const double bi = gm(S.b)[row];
             const double ri = bi - ax;
-            S.r[row] = ri;
-            S.p[row] = 0.0;
-            S.q[row] = 0.0;
+            gm(S.r)[row] = ri;
+            gm(S.p)[row] = 0.0;
+            gm(S.q)[row] = 0.0;
             if (Mixed) {
-                S.r32[row] = static_cast<float>(ri);
+                gm(S.r32)[row] = static_cast<float>(ri);
             }
             rr = ri * ri;
             bb = bi * bi;
@@ -356,12 +370,12 @@ __global__ void __launch_bounds__(k_block) k_cg_init(const SysDev* __restrict__ 
     if (slice < S.a64.n_slices) {
         const double ax = ell_row<double>(S.a64, slice, lane, S.x);
         if (row < S.n) {
-            const double bi = S.b[row];
+            const double bi = gm(S.b)[row];
             const double ri = bi - ax;
-            const double zi = ri * S.dinv[row];
-            S.r[row] = ri;
-            S.z[row] = zi;
-            S.p[row] = 0.0;
+            const double zi = ri * gm(S.dinv)[row];
+            gm(S.r)[row] = ri;
+            gm(S.z)[row] = zi;
+            gm(S.p)[row] = 0.0;
             rr = ri * ri;
             rz = ri * zi;
             bb = bi * bi;
@@ -437,10 +451,9 @@ __global__ void __launch_bounds__(k_block) k_spmv_pz(const SysDev* __restrict__ 
     if (S.st->status != ST_RUNNING) {
         return;
     }
-    const TZ* __restrict__ z = which == 0 ? reinterpret_cast<const TZ*>(S.z)
-                                          : static_cast<const TZ*>(S.lv[0].v[z_buf]);
-    const double* __restrict__ po = p_old_of(S, parity);
-    double* __restrict__ pn = p_new_of(S, parity);
+    const auto z = gm(which == 0 ? reinterpret_cast<const TZ*>(S.z) : static_cast<const TZ*>(S.lv[0].v[z_buf]));
+    const auto po = gm(static_cast<const double*>(p_old_of(S, parity)));
+    const auto pn = gm(p_new_of(S, parity));
     const double beta = S.st->beta;
     double d = 0.0;
     if (slice < S.a64.n_slices) {
@@ -450,7 +463,7 @@ __global__ void __launch_bounds__(k_block) k_spmv_pz(const SysDev* __restrict__ 
         if (row < S.n) {
             const double pi = static_cast<double>(z[row]) + beta * po[row];
             pn[row] = pi;
-            S.q[row] = qi;
+            gm(S.q)[row] = qi;
             d = pi * qi;
         }
     }
@@ -472,17 +485,17 @@ __global__ void __launch_bounds__(k_block) k_update_xr(const SysDev* __restrict_
     double rr = 0.0;
     double rz = 0.0;
     if (row < S.n) {
-        S.x[row] = S.x[row] + alpha * p_new_of(S, parity)[row];
-        const double ri = S.r[row] - alpha * S.q[row];
-        S.r[row] = ri;
+        gm(S.x)[row] = gm(S.x)[row] + alpha * p_new_of(S, parity)[row];
+        const double ri = gm(S.r)[row] - alpha * gm(S.q)[row];
+        gm(S.r)[row] = ri;
         rr = ri * ri;
         if (Mode == 1) {
-            const double zi = ri * S.dinv[row];
-            S.z[row] = zi;
+            const double zi = ri * gm(S.dinv)[row];
+            gm(S.z)[row] = zi;
             rz = ri * zi;
         }
         if (Mode == 2) {
-            S.r32[row] = static_cast<float>(ri);
+            gm(S.r32)[row] = static_cast<float>(ri);
         }
     }
     if (Mode == 1) {
@@ -516,7 +529,7 @@ __global__ void __launch_bounds__(k_block) k_p_update(const SysDev* __restrict__
                              : static_cast<const TZ*>(S.lv[0].v[z_buf]);
     const double beta = S.st->beta;
     if (row < S.n) {
-        S.p[row] = static_cast<double>(z[row]) + beta * S.p[row];
+        gm(S.p)[row] = static_cast<double>(z[row]) + beta * gm(S.p)[row];
     }
 }
 
@@ -527,7 +540,7 @@ __global__ void __launch_bounds__(k_block) k_true_residual(const SysDev* __restr
     if (slice < S.a64.n_slices) {
         const double ax = ell_row<double>(S.a64, slice, lane, S.x);
         if (row < S.n) {
-            const double ri = S.b[row] - ax;
+            const double ri = gm(S.b)[row] - ax;
             t = ri * ri;
         }
     }
@@ -554,12 +567,12 @@ __global__ void __launch_bounds__(k_block) k_bicg_init(const SysDev* __restrict_
     if (slice < S.a64.n_slices) {
         const double ax = ell_row<double>(S.a64, slice, lane, S.x);
         if (row < S.n) {
-            const double bi = S.b[row];
+            const double bi = gm(S.b)[row];
             const double ri = bi - ax;
-            S.r[row] = ri;
+            gm(S.r)[row] = ri;
             S.r0[row] = ri;
-            S.p[row] = 0.0;
-            S.v[row] = 0.0;
+            gm(S.p)[row] = 0.0;
+            gm(S.v)[row] = 0.0;
             rr = ri * ri;
             bb = bi * bi;
         }
@@ -583,9 +596,9 @@ __global__ void __launch_bounds__(k_block) k_bicg_p(const SysDev* __restrict__ s
     const double beta = S.st->beta;
     const double omega = S.st->omega;
     if (row < S.n) {
-        const double pi = S.r[row] + beta * (S.p[row] - omega * S.v[row]);
-        S.p[row] = pi;
-        S.ph[row] = pi * S.dinv[row];
+        const double pi = gm(S.r)[row] + beta * (gm(S.p)[row] - omega * gm(S.v)[row]);
+        gm(S.p)[row] = pi;
+        gm(S.ph)[row] = pi * gm(S.dinv)[row];
     }
 }
 
@@ -598,9 +611,9 @@ __global__ void __launch_bounds__(k_block) k_bicg_s(const SysDev* __restrict__ s
     const double alpha = S.st->alpha;
     double ss = 0.0;
     if (row < S.n) {
-        const double si = S.r[row] - alpha * S.v[row];
+        const double si = gm(S.r)[row] - alpha * gm(S.v)[row];
         S.s[row] = si;
-        S.sh[row] = si * S.dinv[row];
+        gm(S.sh)[row] = si * gm(S.dinv)[row];
         ss = si * si;
     }
     double v[1] = {ss};
@@ -618,7 +631,7 @@ __global__ void __launch_bounds__(k_block) k_bicg_early_x(const SysDev* __restri
         return;
     }
     if (row < S.n) {
-        S.x[row] = S.x[row] + S.st->alpha * S.ph[row];
+        gm(S.x)[row] = gm(S.x)[row] + S.st->alpha * gm(S.ph)[row];
     }
 }
 
@@ -633,9 +646,9 @@ __global__ void __launch_bounds__(k_block) k_bicg_xr(const SysDev* __restrict__ 
     double rr = 0.0;
     double r0r = 0.0;
     if (row < S.n) {
-        S.x[row] = S.x[row] + (alpha * S.ph[row] + omega * S.sh[row]);
+        gm(S.x)[row] = gm(S.x)[row] + (alpha * gm(S.ph)[row] + omega * gm(S.sh)[row]);
         const double ri = S.s[row] - omega * S.t[row];
-        S.r[row] = ri;
+        gm(S.r)[row] = ri;
         rr = ri * ri;
         r0r = S.r0[row] * ri;
     }
@@ -675,12 +688,12 @@ __global__ void __launch_bounds__(k_block) k_bicg_xr(const SysDev* __restrict__ 
 template <class T>
 __global__ void __launch_bounds__(k_block) k_presmooth2(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
-    const T* b = static_cast<const T*>(L.b);
-    const T* wd = static_cast<const T*>(L.wd);
+    const auto b = gm(static_cast<const T*>(L.b));
+    const auto wd = gm(static_cast<const T*>(L.wd));
     const T ax = mat_row_x1<T>(M, row, sub, wd, b);
     if (sub == 0 && row < L.n) {
         const T x1 = wd[row] * b[row];
-        static_cast<T*>(L.v[out])[row] = x1 + wd[row] * (b[row] - ax);
+        gm(static_cast<T*>(L.v[out]))[row] = x1 + wd[row] * (b[row] - ax);
     }
 }
 
@@ -695,9 +708,9 @@ __global__ void __launch_bounds__(k_block) k_jacobi_zero(const SysDev* __restric
     if (L.coarsest || row >= L.n) {
         return;
     }
-    const T* b = static_cast<const T*>(L.b);
-    const T* wd = static_cast<const T*>(L.wd);
-    static_cast<T*>(L.v[out])[row] = wd[row] * b[row];
+    const auto b = gm(static_cast<const T*>(L.b));
+    const auto wd = gm(static_cast<const T*>(L.wd));
+    gm(static_cast<T*>(L.v[out]))[row] = wd[row] * b[row];
 }
 
 // v[out] = 0 (smoothing-free V-cycles; row-wise).
@@ -711,7 +724,7 @@ __global__ void __launch_bounds__(k_block) k_zero_level(const SysDev* __restrict
     if (L.coarsest || row >= L.n) {
         return;
     }
-    static_cast<T*>(L.v[out])[row] = T(0);
+    gm(static_cast<T*>(L.v[out]))[row] = T(0);
 }
 
 // Jacobi sweep v[out] = v[in] + wd (b - A v[in]). Dots (level 0 only, sliced ELL): 0 none;
@@ -723,19 +736,19 @@ __global__ void __launch_bounds__(k_block) k_jacobi(const SysDev* __restrict__ s
     double d1 = 0.0;
     double d2 = 0.0;
     const T* x = static_cast<const T*>(L.v[in]);
-    const T* b = static_cast<const T*>(L.b);
-    const T* wd = static_cast<const T*>(L.wd);
+    const auto b = gm(static_cast<const T*>(L.b));
+    const auto wd = gm(static_cast<const T*>(L.wd));
     const T ax = mat_row<T>(M, row, sub, x);
     if (sub == 0 && row < L.n) {
         const T z = x[row] + wd[row] * (b[row] - ax);
-        static_cast<T*>(L.v[out])[row] = z;
+        gm(static_cast<T*>(L.v[out]))[row] = z;
         if (Dots == 1) {
             d1 = static_cast<double>(b[row]) * static_cast<double>(z);
         }
         if (Dots == 2) {
             const double zd = static_cast<double>(z);
-            d1 = S.r[row] * zd;
-            d2 = zd * S.q[row];
+            d1 = gm(S.r)[row] * zd;
+            d2 = zd * gm(S.q)[row];
         }
     }
     if (Dots == 1) {
@@ -764,7 +777,7 @@ __global__ void __launch_bounds__(k_block) k_residual(const SysDev* __restrict__
     const T* x = static_cast<const T*>(L.v[in]);
     const T ax = mat_row<T>(M, row, sub, x);
     if (sub == 0 && row < L.n) {
-        static_cast<T*>(L.v[out])[row] = static_cast<const T*>(L.b)[row] - ax;
+        gm(static_cast<T*>(L.v[out]))[row] = gm(static_cast<const T*>(L.b))[row] - ax;
     }
 }
 
@@ -774,7 +787,7 @@ __global__ void __launch_bounds__(k_block) k_restrict(const SysDev* __restrict__
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, r)
     const T y = mat_row<T>(M, row, sub, static_cast<const T*>(L.v[in]));
     if (sub == 0 && row < L.n_next) {
-        static_cast<T*>(S.lv[level + 1].b)[row] = y;
+        gm(static_cast<T*>(S.lv[level + 1].b))[row] = y;
     }
 }
 
@@ -808,14 +821,14 @@ __global__ void __launch_bounds__(k_block) k_coarse(const SysDev* __restrict__ s
     }
     const int n = L.n;
     const double* m = L.cinv + static_cast<long long>(i) * n;
-    const T* b = static_cast<const T*>(L.b);
+    const auto b = gm(static_cast<const T*>(L.b));
     double s = 0.0;
     for (int j = lane; j < n; j += k_wave) {
         s += m[j] * static_cast<double>(b[j]);
     }
     s = wave_sum(s);
     if (lane == 0) {
-        static_cast<T*>(L.v[out])[i] = static_cast<T>(s);
+        gm(static_cast<T*>(L.v[out]))[i] = static_cast<T>(s);
     }
 }
 

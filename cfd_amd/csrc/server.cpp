@@ -3,11 +3,11 @@
 // stay resident for all of their steps, and share every kernel launch with the other running
 // solves through a SlotRunner (continuous batching).
 //
-// Two host threads: the engine thread drives the device (one iteration boundary per step() of
-// the slot runner, starting a request's next step in the slot its previous step freed); the
-// staging thread builds admitted requests' systems on the device (upload, hierarchy, right-hand
-// sides) and copies finished requests' results back, so that neither kind of host work stalls
-// the launches. No counterpart in the reference, whose DeviceSolver serves one caller's solve
+// Host threads: the engine thread drives the device (one iteration boundary per step() of the
+// slot runner, starting a request's next step in the slot its previous step freed); staging
+// threads build admitted requests' systems on the device (upload, hierarchy, right-hand sides)
+// and copy finished requests' results back, so that neither kind of host work stalls the
+// launches, and several requests are staged at once. No counterpart in the reference, whose DeviceSolver serves one caller's solve
 // at a time (src/gpu/src/device_solver.cpp:117-150).
 #include "solver.hpp"
 
@@ -80,6 +80,7 @@ public:
                     o.precision == CFDG_PREC_MIXED_BF16,
                 "server: unknown precision");
         require(o.smooth_sweeps >= 0 && o.smooth_sweeps <= 64, "server: invalid smooth_sweeps");
+        require(o.staging_threads >= 1 && o.staging_threads <= 64, "server: staging_threads must be in [1, 64]");
         require_device();
         budget_ = o.memory_budget_bytes;
         if (budget_ == 0) {
@@ -96,7 +97,9 @@ public:
         runner_ = std::make_unique<SlotRunner>(o.max_active_solves, bo, o.smooth_sweeps, stream_.get());
         owner_.assign(static_cast<std::size_t>(o.max_active_solves), nullptr);
         engine_ = std::thread([this] { engine_loop(); });
-        stager_ = std::thread([this] { stager_loop(); });
+        for (int i = 0; i < o.staging_threads; ++i) {
+            stagers_.emplace_back([this] { stager_loop(); });
+        }
     }
 
     ~Server() {
@@ -107,7 +110,9 @@ public:
         }
         cv_.notify_all();
         engine_.join();
-        stager_.join();
+        for (std::thread& t : stagers_) {
+            t.join();
+        }
     }
 
     Server(const Server&) = delete;
@@ -468,7 +473,7 @@ private:
     std::unique_ptr<SlotRunner> runner_;
     std::vector<Request*> owner_; // per slot, engine thread only
     std::thread engine_;
-    std::thread stager_;
+    std::vector<std::thread> stagers_;
 };
 
 } // namespace cfdg
@@ -497,6 +502,7 @@ extern "C" cfdg_status cfdg_server_default_options(cfdg_server_options* out) {
         out->precision = CFDG_PREC_MIXED_FP32;
         out->smooth_sweeps = 2;
         out->memory_budget_bytes = 0;
+        out->staging_threads = 4;
     });
 }
 

@@ -51,8 +51,8 @@ __global__ void __launch_bounds__(k_block) k_make_bf16(EllView m, unsigned short
     const int lane = row % k_wave;
     const int base = m.sp[slice];
     const int w = (m.sp[slice + 1] - base) / k_wave;
-    const ColIdx ci{m.col == nullptr ? nullptr : m.col + base + lane,
-                    m.dcol == nullptr ? nullptr : m.dcol + base + lane, row};
+    const ColIdx ci{m.col == nullptr ? nullptr : gm(m.col) + base + lane,
+                    m.dcol == nullptr ? nullptr : gm(m.dcol) + base + lane, row};
     const double* val = static_cast<const double*>(m.val) + base + lane;
     int kd = -1;
     double d = 0.0;

@@ -61,7 +61,7 @@ class ServedRequest:
 class Server:
     def __init__(self, max_active_solves: int = 64, method: Method = Method.Cg,
                  use_amg: bool = True, precision: Precision = Precision.MIXED_FP32,
-                 smooth_sweeps: int = 2, memory_budget_bytes: int = 0):
+                 smooth_sweeps: int = 2, memory_budget_bytes: int = 0, staging_threads: int = 4):
         o = L.ServerOptions()
         L.call("cfdg_server_default_options", C.byref(o))
         o.max_active_solves = int(max_active_solves)
@@ -70,6 +70,7 @@ class Server:
         o.precision = int(precision)
         o.smooth_sweeps = int(smooth_sweeps)
         o.memory_budget_bytes = int(memory_budget_bytes)
+        o.staging_threads = int(staging_threads)
         h = C.c_void_p()
         L.call("cfdg_server_create", C.byref(o), C.byref(h))
         self._h = h

@@ -235,6 +235,7 @@ typedef struct cfdg_server_options {
     cfdg_precision precision;    /* default CFDG_PREC_MIXED_FP32 */
     int smooth_sweeps;           /* V-cycle sweeps every hierarchy must use, default 2 */
     size_t memory_budget_bytes;  /* resident requests' device bytes; 0: 90% of device memory */
+    int staging_threads;         /* host threads uploading and retiring requests, default 4 */
 } cfdg_server_options;
 
 typedef struct cfdg_server_request {
