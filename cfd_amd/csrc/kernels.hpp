@@ -85,14 +85,18 @@ __device__ __forceinline__ gmem_t<T>* gm(T* p) {
     return (gmem_t<T>*)p;
 }
 
-// Column index of slot k: int32 column or row + 16-bit offset.
-struct ColIdx {
+// Column index of slot k: 32-bit columns (ColAbs) or row + 16-bit offset (ColOff). The format
+// is chosen once per matrix (a uniform branch), never per slot: a per-lane pointer test would
+// turn every column load into a divergent branch with its own wait.
+struct ColAbs {
     const gmem_t<const int>* __restrict__ col;
+    __device__ __forceinline__ int operator()(int k) const { return col[k]; }
+};
+
+struct ColOff {
     const gmem_t<const short>* __restrict__ dcol;
     int row;
-    __device__ __forceinline__ int operator()(int k) const {
-        return dcol != nullptr ? row + static_cast<int>(dcol[k]) : col[k];
-    }
+    __device__ __forceinline__ int operator()(int k) const { return row + static_cast<int>(dcol[k]); }
 };
 
 template <class T, class V>
@@ -107,8 +111,8 @@ __device__ __forceinline__ T value_at(const gmem_t<const V>* __restrict__ val, i
 // Storage-order row sum of sliced-ELL slots with values val (type V) converted to T. Loads are
 // issued four entries at a time, the tail (w mod 4, uniform over the slice's wavefront) as one
 // more group with its unused loads repeating a valid slot; only real entries are summed.
-template <class T, class V, class F>
-__device__ __forceinline__ T ell_sum(const ColIdx& ci, const gmem_t<const V>* __restrict__ val, int w, F load) {
+template <class T, class V, class C, class F>
+__device__ __forceinline__ T ell_sum(const C& ci, const gmem_t<const V>* __restrict__ val, int w, F load) {
     T s = T(0);
     int k = 0;
     for (; k + 4 <= w; k += 4) {
@@ -210,17 +214,21 @@ __device__ __forceinline__ T ell_row_f(const EllView& m, int slice, int lane, F 
     const int base = m.sp[slice];
     const int w = (m.sp[slice + 1] - base) / k_wave;
     const int row = slice * k_wave + lane;
-    const ColIdx ci{m.col == nullptr ? nullptr : gm(m.col) + base + lane,
-                    m.dcol == nullptr ? nullptr : gm(m.dcol) + base + lane, row};
-    if constexpr (sizeof(T) == 4) {
-        if (m.fmt == 1) {
-            const auto val = gm(static_cast<const unsigned short*>(m.val)) + base + lane;
-            const T s = ell_sum<T, unsigned short>(ci, val, w, load);
-            const T dx = row < m.n_rows ? gm(m.diag)[row] * load(row) : T(0);
-            return madd(gm(m.rscale)[row], s, dx);
+    auto sum = [&](const auto& ci) -> T {
+        if constexpr (sizeof(T) == 4) {
+            if (m.fmt == 1) {
+                const auto val = gm(static_cast<const unsigned short*>(m.val)) + base + lane;
+                const T s = ell_sum<T, unsigned short>(ci, val, w, load);
+                const T dx = row < m.n_rows ? gm(m.diag)[row] * load(row) : T(0);
+                return madd(gm(m.rscale)[row], s, dx);
+            }
         }
+        return ell_sum<T, T>(ci, gm(static_cast<const T*>(m.val)) + base + lane, w, load);
+    };
+    if (m.dcol != nullptr) {
+        return sum(ColOff{gm(m.dcol) + base + lane, row});
     }
-    return ell_sum<T, T>(ci, gm(static_cast<const T*>(m.val)) + base + lane, w, load);
+    return sum(ColAbs{gm(m.col) + base + lane});
 }
 
 // Row sum of an ELL row: sum_k val_k * x[col_k] in storage order (any column / value format).

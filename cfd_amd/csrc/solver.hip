@@ -51,15 +51,17 @@ __global__ void __launch_bounds__(k_block) k_make_bf16(EllView m, unsigned short
     const int lane = row % k_wave;
     const int base = m.sp[slice];
     const int w = (m.sp[slice + 1] - base) / k_wave;
-    const ColIdx ci{m.col == nullptr ? nullptr : gm(m.col) + base + lane,
-                    m.dcol == nullptr ? nullptr : gm(m.dcol) + base + lane, row};
+    // Column of slot k (setup kernel, run once per assembly: a per-slot test is fine here).
+    auto column = [&](int k) {
+        return m.dcol != nullptr ? row + static_cast<int>(m.dcol[base + lane + k]) : m.col[base + lane + k];
+    };
     const double* val = static_cast<const double*>(m.val) + base + lane;
     int kd = -1;
     double d = 0.0;
     double mx = 0.0;
     for (int k = 0; k < w; ++k) {
         const double v = val[k * k_wave];
-        if (kd < 0 && row < m.n_rows && ci(k * k_wave) == row) {
+        if (kd < 0 && row < m.n_rows && column(k * k_wave) == row) {
             kd = k;
             d = v;
         } else {
