@@ -193,6 +193,19 @@ def stress(n_side: int = 160, reps: int = 3) -> dict:
             "true_rel_residual": st.true_residual / bnorm,
             "rel_error_vs_fp64": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
             "solves_per_s": round(1.0 / float(np.median(times[1:])), 3)}
+    # Level-0 FP32 sweep at a working set larger than the 256 MB Infinity Cache.
+    dev.probe(1)
+    for _ in range(2):
+        dev.set_x(None)
+        dev.solve_resident(Method.Cg, True, Precision.MIXED_FP32, ctl)
+    ms, launches, nbytes = dev.probe_read(Precision.MIXED_FP32)
+    dev.probe(0)
+    gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    out["roofline_l0_fp32_sweep"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                     "bytes_per_launch": int(nbytes), "avg_us": round(ms * 1e3, 3),
+                                     "launches": int(launches),
+                                     "working_set_mb": round(dev.footprint() / 1e6, 1)}
     out["workload"] = (f"{n_side}^3 jittered box ({p.n} cells), rho 1000 in a sphere of radius "
                        "0.2 / 1 outside, harmonic faces, SA-AMG-PCG to 1e-10")
     return out

@@ -1576,7 +1576,9 @@ extern "C" cfdg_status cfdg_system_probe_read(const cfdg_system* sys, int precis
         const double nnz = static_cast<double>(s.nnz0());
         const bool fp32 = precision != CFDG_PREC_FP64;
         const bool bf16 = precision == CFDG_PREC_MIXED_BF16;
-        const double idx = s.index_bytes0(); // 16-bit column offsets or 32-bit columns
+        // Column indices count 4 bytes per stored entry whatever their stored width (16-bit
+        // offsets here), values their stored width; vectors once per launch.
+        const double idx = 4.0;
         if (p.kind == PROBE_L0_SWEEP) {
             // matrix (value + column index) + x_in, b, w*dinv, x_out at the V-cycle precision;
             // the mixed sweep also reads r and q in FP64 for its two dot products, and the

@@ -59,8 +59,6 @@ public:
     // Cached solve machinery (launch tables, descriptors, graphs) per method / precision.
     std::unique_ptr<Runner>& runner_slot(cfdg_method method, bool use_amg, cfdg_precision precision);
     [[nodiscard]] long long nnz0() const { return nnz0_; }
-    // Bytes per column index of the level-0 operator (2: 16-bit offsets, 4: 32-bit columns).
-    [[nodiscard]] int index_bytes0() const { return a0_.dcol.size() > 0 ? 2 : 4; }
     explicit System(const cfdg_ldu_pattern& pattern); // values assembled later from LDU
     explicit System(const cfdg_csr& a);               // values given
     System(const System&) = delete;
