@@ -459,6 +459,7 @@ def main(argv=None) -> int:
         "true_rel_residual_fp64": st_d.true_residual / bnorm,
         "mixed_vs_fp64_rel_error": err,
         "mixed_bf16": {"solves_per_s": round(args.steps / t_bf16, 3), "iterations": int(its_b[-1]),
+                       "device_ms": round(float(np.median(dms_b)), 3),
                        "p99_ms": round(1e3 * float(np.percentile(lat_b, 99)), 3),
                        "true_rel_residual": st_b.true_residual / bnorm,
                        "rel_error_vs_fp64": float(np.linalg.norm(x_bf16 - x_fp64) / np.linalg.norm(x_fp64))},

@@ -6,20 +6,44 @@
 
 namespace cfdg {
 
-HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci) {
+namespace {
+
+// Rounds of slice s: the longest row of the slice divided by the lanes per row, rounded up.
+int slice_rounds(int s, int n_rows, const int* rp, int lanes) {
+    const int rows = k_wave / lanes;
+    int w = 0;
+    const int r_end = std::min(n_rows, (s + 1) * rows);
+    for (int r = s * rows; r < r_end; ++r) {
+        w = std::max(w, rp[r + 1] - rp[r]);
+    }
+    return (w + lanes - 1) / lanes;
+}
+
+} // namespace
+
+long long ell_slots(int n_rows, const int* rp, int lanes) {
+    const int rows = k_wave / lanes;
+    const int n_slices = (n_rows + rows - 1) / rows;
+    long long total = 0;
+    for (int s = 0; s < n_slices; ++s) {
+        total += static_cast<long long>(slice_rounds(s, n_rows, rp, lanes)) * k_wave;
+    }
+    return total;
+}
+
+HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci, int lanes) {
+    require(lanes >= 1 && lanes <= k_wave && (lanes & (lanes - 1)) == 0,
+            "csr_to_ell: lanes per row must be a power of two <= 64");
     HostEll e;
     e.n_rows = n_rows;
     e.n_cols = n_cols;
-    e.n_slices = (n_rows + k_wave - 1) / k_wave;
+    e.lanes = lanes;
+    const int rows = k_wave / lanes;
+    e.n_slices = (n_rows + rows - 1) / rows;
     e.sp.assign(static_cast<std::size_t>(e.n_slices) + 1, 0);
     long long total = 0;
     for (int s = 0; s < e.n_slices; ++s) {
-        int w = 0;
-        const int r_end = std::min(n_rows, (s + 1) * k_wave);
-        for (int r = s * k_wave; r < r_end; ++r) {
-            w = std::max(w, rp[r + 1] - rp[r]);
-        }
-        total += static_cast<long long>(w) * k_wave;
+        total += static_cast<long long>(slice_rounds(s, n_rows, rp, lanes)) * k_wave;
         if (total > std::numeric_limits<int>::max()) {
             throw InvalidArgument("matrix too large for 32-bit sliced-ELL offsets");
         }
@@ -32,7 +56,8 @@ HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci) {
         const int base = e.sp[static_cast<std::size_t>(s)];
         const int w = (e.sp[static_cast<std::size_t>(s) + 1] - base) / k_wave;
         for (int lane = 0; lane < k_wave; ++lane) {
-            const int r = s * k_wave + lane;
+            const int r = s * rows + lane / lanes;
+            const int sub = lane % lanes;
             const int len = r < n_rows ? rp[r + 1] - rp[r] : 0;
             // rows past the end point at the last row of a square matrix (a short offset)
             int pad = (n_rows == n_cols && n_rows > 0) ? n_rows - 1 : 0;
@@ -41,9 +66,10 @@ HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci) {
             }
             for (int k = 0; k < w; ++k) {
                 const std::size_t slot = static_cast<std::size_t>(base) + static_cast<std::size_t>(k) * k_wave + lane;
-                if (k < len) {
-                    e.col[slot] = ci[rp[r] + k];
-                    e.src[slot] = rp[r] + k;
+                const int j = k * lanes + sub;
+                if (j < len) {
+                    e.col[slot] = ci[rp[r] + j];
+                    e.src[slot] = rp[r] + j;
                 } else {
                     e.col[slot] = pad;
                     e.src[slot] = -1;
@@ -85,7 +111,8 @@ void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hi
     n_rows = e.n_rows;
     n_cols = e.n_cols;
     n_slices = e.n_slices;
-    tpr = 1;
+    tpr = e.lanes;
+    csr = 0;
     sp.upload(e.sp.data(), e.sp.size(), s);
     // Square matrices whose columns all lie within 32767 of the row store 16-bit offsets.
     bool short_ok = e.n_rows == e.n_cols;
@@ -96,7 +123,7 @@ void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hi
             const int base = e.sp[static_cast<std::size_t>(sl)];
             const int end = e.sp[static_cast<std::size_t>(sl) + 1];
             for (int k = base; k < end; ++k) {
-                const int row = sl * k_wave + (k - base) % k_wave;
+                const int row = sl * (k_wave / e.lanes) + ((k - base) % k_wave) / e.lanes;
                 const int off = e.col[static_cast<std::size_t>(k)] - row;
                 if (off < -32768 || off > 32767) {
                     short_ok = false;
@@ -128,14 +155,18 @@ void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hi
 void DevEll::upload_csr(int rows, int cols, const int* rp, const int* ci, const double* va,
                         hipStream_t s, int tpr_override) {
     const int t = tpr_override > 0 ? tpr_override : pick_tpr(rows, rp[rows]);
-    if (t == 1) {
-        upload(csr_to_ell(rows, cols, rp, ci), va, true, true, s);
+    // Sliced ELL (coalesced, 16-bit columns) unless its padding would exceed the CSR entries by
+    // more than CFDG_ELL_MAX_PAD (default 1.5).
+    static const double max_pad = env_or("CFDG_ELL_MAX_PAD", 1.5);
+    if (t == 1 || static_cast<double>(ell_slots(rows, rp, t)) <= max_pad * static_cast<double>(rp[rows])) {
+        upload(csr_to_ell(rows, cols, rp, ci, t), va, true, true, s);
         return;
     }
     n_rows = rows;
     n_cols = cols;
     n_slices = 0;
     tpr = t;
+    csr = 1;
     const std::size_t nnz = static_cast<std::size_t>(rp[rows]);
     sp.upload(rp, static_cast<std::size_t>(rows) + 1, s);
     dcol.resize(0); // CSR rows always carry 32-bit columns

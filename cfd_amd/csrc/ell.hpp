@@ -12,15 +12,20 @@ struct HostEll {
     int n_rows = 0;
     int n_cols = 0;
     int n_slices = 0;
+    int lanes = 1;        // lanes per row
     std::vector<int> sp;  // n_slices + 1
     std::vector<int> col; // sp[n_slices]
     std::vector<int> src; // CSR position per slot, -1 for padding
 };
 
-// Slices of 64 rows; slot (64 k + lane) of slice s holds entry k of row 64 s + lane in CSR
-// order, or padding (value 0) pointing at a column that the row touches anyway: the row itself
-// for square matrices, else the row's last column (0 for rows past the end).
-HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci);
+// Slices of 64 / lanes rows; lane l of slice s serves row s 64 / lanes + l / lanes and slot
+// (64 k + l) holds that row's entry k lanes + l mod lanes in CSR order, or padding (value 0)
+// pointing at a column that the row touches anyway: the row itself for square matrices, else
+// the row's last column (rows past the end: the last row, or column 0).
+HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci, int lanes = 1);
+
+// Stored slots of the sliced-ELL layout with the given lanes per row (no conversion).
+long long ell_slots(int n_rows, const int* rp, int lanes);
 
 template <class T>
 std::vector<T> ell_values(const HostEll& e, const double* va) {
@@ -43,6 +48,7 @@ struct DevEll {
     int n_cols = 0;
     int n_slices = 0;
     int tpr = 1;
+    int csr = 0;
     DevBuf<int> sp;
     DevBuf<int> col;
     DevBuf<short> dcol;          // 16-bit column offsets (replaces col when present)
@@ -62,6 +68,7 @@ struct DevEll {
         e.tpr = tpr;
         e.n_slices = n_slices;
         e.fmt = fmt;
+        e.csr = csr;
         e.sp = sp.get();
         e.col = col.size() > 0 ? col.get() : nullptr;
         e.dcol = dcol.size() > 0 ? dcol.get() : nullptr;

@@ -213,7 +213,8 @@ template <class T, class F>
 __device__ __forceinline__ T ell_row_f(const EllView& m, int slice, int lane, F load) {
     const int base = m.sp[slice];
     const int w = (m.sp[slice + 1] - base) / k_wave;
-    const int row = slice * k_wave + lane;
+    const int lg = __builtin_ctz(m.tpr); // lanes per row = 2^lg
+    const int row = slice * (k_wave >> lg) + (lane >> lg);
     auto sum = [&](const auto& ci) -> T {
         if constexpr (sizeof(T) == 4) {
             if (m.fmt == 1) {
@@ -260,9 +261,12 @@ __device__ __forceinline__ T group_sum(T v, int tpr) {
 // result is complete in every lane of the row's group. Rows past the end give 0.
 template <class T>
 __device__ __forceinline__ T mat_row(const EllView& m, int row, int sub, const T* __restrict__ x) {
-    if (m.tpr == 1) {
-        const int slice = row / k_wave;
-        return slice < m.n_slices ? ell_row<T>(m, slice, row % k_wave, x) : T(0);
+    if (m.csr == 0) {
+        const int lg = __builtin_ctz(m.tpr);
+        const int slice = row >> (6 - lg); // 64 / tpr rows per slice
+        const T s = slice < m.n_slices ? ell_row<T>(m, slice, static_cast<int>(threadIdx.x) & (k_wave - 1), x)
+                                       : T(0);
+        return m.tpr > 1 ? group_sum(s, m.tpr) : s;
     }
     T s = T(0);
     if (row < m.n_rows) {
@@ -276,9 +280,13 @@ __device__ __forceinline__ T mat_row(const EllView& m, int row, int sub, const T
 template <class T>
 __device__ __forceinline__ T mat_row_x1(const EllView& m, int row, int sub, const gmem_t<const T>* __restrict__ wd,
                                         const gmem_t<const T>* __restrict__ b) {
-    if (m.tpr == 1) {
-        const int slice = row / k_wave;
-        return slice < m.n_slices ? ell_row_x1<T>(m, slice, row % k_wave, wd, b) : T(0);
+    if (m.csr == 0) {
+        const int lg = __builtin_ctz(m.tpr);
+        const int slice = row >> (6 - lg);
+        const T s = slice < m.n_slices
+                        ? ell_row_x1<T>(m, slice, static_cast<int>(threadIdx.x) & (k_wave - 1), wd, b)
+                        : T(0);
+        return m.tpr > 1 ? group_sum(s, m.tpr) : s;
     }
     T s = T(0);
     if (row < m.n_rows) {
@@ -799,15 +807,33 @@ __global__ void __launch_bounds__(k_block) k_restrict(const SysDev* __restrict__
     }
 }
 
-// v_l[io] += P_l v_{l+1}[coarse_buf].
-template <class T>
+// v_l[io] += P_l v_{l+1}[coarse_buf]. X1: the level's single pre-sweep from zero was not
+// stored; its result wd b is formed here (v[io] = wd b + P v_{l+1}, the same sums).
+template <class T, bool X1 = false>
 __global__ void __launch_bounds__(k_block) k_prolong(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int io,
                           int coarse_buf) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, p)
     const T t = mat_row<T>(M, row, sub, static_cast<const T*>(S.lv[level + 1].v[coarse_buf]));
     if (sub == 0 && row < L.n) {
-        T* x = static_cast<T*>(L.v[io]);
-        x[row] = x[row] + t;
+        const auto x = gm(static_cast<T*>(L.v[io]));
+        if constexpr (X1) {
+            x[row] = gm(static_cast<const T*>(L.wd))[row] * gm(static_cast<const T*>(L.b))[row] + t;
+        } else {
+            x[row] = x[row] + t;
+        }
+    }
+}
+
+// v[out] = b - A x1 with x1 = wd b, the single Jacobi sweep from zero, formed per gathered
+// column instead of stored (one pre-sweep per side).
+template <class T>
+__global__ void __launch_bounds__(k_block) k_residual_x1(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
+    CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
+    const auto b = gm(static_cast<const T*>(L.b));
+    const auto wd = gm(static_cast<const T*>(L.wd));
+    const T ax = mat_row_x1<T>(M, row, sub, wd, b);
+    if (sub == 0 && row < L.n) {
+        gm(static_cast<T*>(L.v[out]))[row] = b[row] - ax;
     }
 }
 
@@ -1083,8 +1109,11 @@ __global__ void __launch_bounds__(k_block) k_ldu_gather(long long n_slots, const
     }
 }
 
-__global__ void __launch_bounds__(k_block) k_diag_inverse(int n, const double* __restrict__ diag, double w, double* __restrict__ dinv,
-                               double* __restrict__ wd64, float* __restrict__ wd32) {
+// dinv = 1 / a_ii (1.0 for a zero diagonal), wd64 = w64 dinv (FP64 smoother), wd32 = w32 dinv
+// (mixed smoother).
+__global__ void __launch_bounds__(k_block) k_diag_inverse(int n, const double* __restrict__ diag, double w64, double w32,
+                                                          double* __restrict__ dinv, double* __restrict__ wd64,
+                                                          float* __restrict__ wd32) {
     const int i = blockIdx.x * k_block + threadIdx.x;
     if (i >= n) {
         return;
@@ -1092,10 +1121,9 @@ __global__ void __launch_bounds__(k_block) k_diag_inverse(int n, const double* _
     const double d = diag[i];
     const double di = d != 0.0 ? 1.0 / d : 1.0;
     dinv[i] = di;
-    const double wdi = w * di;
-    wd64[i] = wdi;
+    wd64[i] = w64 * di;
     if (wd32 != nullptr) {
-        wd32[i] = static_cast<float>(wdi);
+        wd32[i] = static_cast<float>(w32 * di);
     }
 }
 

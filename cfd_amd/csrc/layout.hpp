@@ -52,18 +52,23 @@ struct SolveState {
     int pad;
 };
 
-// A sparse matrix as the kernels see it. tpr == 1: sliced ELL, one thread per row, slices of 64
-// rows (ptr = slice offsets, entry k of row 64 s + lane at ptr[s] + 64 k + lane). tpr > 1: CSR
-// (ptr = row offsets) with tpr consecutive lanes per row, used for the long rows of coarse
-// operators and restrictions. A workgroup covers 256 / tpr rows.
-// Sliced-ELL options: dcol (if not null) replaces col by 16-bit offsets column - row; fmt 1
-// stores the values as BF16 scaled per row (a_ij = rscale_i * bf16_ij) with the diagonal slot
-// zero and the diagonal in FP32 in diag (rscale and diag padded to whole slices).
+// A sparse matrix as the kernels see it; tpr (a power of two) lanes read each row and a
+// workgroup covers 256 / tpr rows.
+// Sliced ELL (csr == 0): a slice is one wavefront, 64 / tpr rows of tpr lanes each (ptr = slice
+// offsets); lane l of slice s serves row s 64 / tpr + l / tpr, and its round k (slot
+// ptr[s] + 64 k + l) holds the row's entry k tpr + l mod tpr in CSR order, or padding (value 0,
+// a column the row touches anyway). With tpr == 1 a lane owns a row and sums it in CSR order.
+// CSR (csr == 1, ptr = row offsets): lane j of a row's group sums its entries j, j + tpr, ...
+// Both give the same sums for the same tpr. dcol (if not null) replaces col by 16-bit offsets
+// column - row; fmt 1 stores the values as BF16 scaled per row (a_ij = rscale_i * bf16_ij) with
+// the diagonal slot zero and the diagonal in FP32 in diag (rscale and diag padded to whole
+// slices; tpr == 1 only).
 struct EllView {
     int n_rows;
     int tpr;
     int n_slices;    // ELL only
     int fmt;         // 0 values of the kernel's type, 1 BF16 + row scale + FP32 diagonal
+    int csr;         // 1: CSR rows, 0: sliced ELL
     const int* sp;   // ELL slice offsets or CSR row offsets
     const int* col;
     const short* dcol;

@@ -83,14 +83,72 @@ __global__ void __launch_bounds__(k_block) k_make_bf16(EllView m, unsigned short
     diag[row] = static_cast<float>(d + lost);
 }
 
-__global__ void __launch_bounds__(k_block) k_scale_dinv(int n, const double* __restrict__ dinv, double w,
-                                                        double* __restrict__ wd64, float* __restrict__ wd32) {
+__global__ void __launch_bounds__(k_block) k_scale_dinv(int n, const double* __restrict__ dinv, double w64,
+                                                        double w32, double* __restrict__ wd64,
+                                                        float* __restrict__ wd32) {
     const int i = blockIdx.x * k_block + threadIdx.x;
     if (i < n) {
-        const double v = w * dinv[i];
-        wd64[i] = v;
-        wd32[i] = static_cast<float>(v);
+        wd64[i] = w64 * dinv[i];
+        wd32[i] = static_cast<float>(w32 * dinv[i]);
     }
+}
+
+// Largest eigenvalue of D^-1 A (D = diag A) by 15 power iterations from a fixed start vector;
+// rows in parallel, norms in a fixed order, so the estimate is reproducible.
+double estimate_lambda(const cfdg_csr& a) {
+    const int n = a.n_rows;
+    std::vector<double> dinv(static_cast<std::size_t>(n), 1.0);
+    for (int i = 0; i < n; ++i) {
+        for (int k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
+            if (a.col[k] == i && a.val[k] != 0.0) {
+                dinv[static_cast<std::size_t>(i)] = 1.0 / a.val[k];
+            }
+        }
+    }
+    std::vector<double> x(static_cast<std::size_t>(n));
+    std::vector<double> y(static_cast<std::size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        x[static_cast<std::size_t>(i)] = 1.0 + 0.5 * std::sin(0.7 * i + 0.3);
+    }
+    double lambda = 0.0;
+    for (int it = 0; it < 15; ++it) {
+#pragma omp parallel for schedule(static) if (n > 20000)
+        for (int i = 0; i < n; ++i) {
+            double t = 0.0;
+            for (int k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
+                t += a.val[k] * x[static_cast<std::size_t>(a.col[k])];
+            }
+            y[static_cast<std::size_t>(i)] = dinv[static_cast<std::size_t>(i)] * t;
+        }
+        double xx = 0.0;
+        double yy = 0.0;
+        for (int i = 0; i < n; ++i) {
+            xx += x[static_cast<std::size_t>(i)] * x[static_cast<std::size_t>(i)];
+            yy += y[static_cast<std::size_t>(i)] * y[static_cast<std::size_t>(i)];
+        }
+        lambda = xx > 0.0 ? std::sqrt(yy / xx) : 0.0;
+        if (!(lambda > 0.0) || !std::isfinite(lambda)) {
+            return 2.0;
+        }
+        const double inv = 1.0 / std::sqrt(yy);
+        for (int i = 0; i < n; ++i) {
+            x[static_cast<std::size_t>(i)] = y[static_cast<std::size_t>(i)] * inv;
+        }
+    }
+    return lambda;
+}
+
+// Damping of a mixed-precision Jacobi sweep on a level with D^-1 A eigenvalue estimate lambda:
+// the centre of the interval [h / ratio, h], h = 1.1 lambda (the first-degree Chebyshev
+// smoother, which damps the upper part of the spectrum uniformly).
+double mixed_weight(double lambda) {
+    static const double ratio = [] {
+        const char* e = std::getenv("CFDG_MIXED_RATIO");
+        const double r = e != nullptr ? std::atof(e) : 10.0;
+        return r > 1.0 ? r : 10.0;
+    }();
+    const double h = 1.1 * lambda;
+    return 2.0 / (h + h / ratio);
 }
 
 // Level-0 dot products from a finished V-cycle output v[buf] (used when no Jacobi sweep on
@@ -186,6 +244,16 @@ void check_csr(const cfdg_csr& m, int rows, int cols, const char* what) {
 }
 
 } // namespace
+
+// Smoother of the mixed-precision V-cycle: sweeps per side (CFDG_MIXED_SWEEPS, default 1) and
+// the ratio of the damping interval (CFDG_MIXED_RATIO, default 10; see mixed_weight).
+int mixed_sweeps() {
+    static const int v = [] {
+        const char* e = std::getenv("CFDG_MIXED_SWEEPS");
+        return e != nullptr ? std::max(0, std::min(64, std::atoi(e))) : 1;
+    }();
+    return v;
+}
 
 void require_device() {
     int count = 0;
@@ -301,7 +369,7 @@ void System::assemble(const double* diag, const double* upper, const double* low
     k_ldu_gather<<<ceil_div(slots, k_block), k_block, 0, s>>>(slots, a0_src_.get(), ldu,
                                                                a0_.v64.get(), a0_.v32.get());
     hip_check(hipGetLastError(), "k_ldu_gather launch");
-    k_diag_inverse<<<ceil_div(n_, k_block), k_block, 0, s>>>(n_, ldu, weight_, dinv0_.get(),
+    k_diag_inverse<<<ceil_div(n_, k_block), k_block, 0, s>>>(n_, ldu, weight_, weight0_32_, dinv0_.get(),
                                                              wd0_64_.get(), wd0_32_.get());
     hip_check(hipGetLastError(), "k_diag_inverse launch");
     hip_check(hipStreamSynchronize(s), "assemble sync");
@@ -427,11 +495,12 @@ void System::set_amg(const cfdg_amg_view& v) {
             const cfdg_csr& a = v.levels[l].a;
             d.a.upload_csr(d.n, d.n, a.row_ptr, a.col, a.val, s);
             const std::vector<double> dinv = inverse_diagonal(a);
+            const double w32 = mixed_weight(estimate_lambda(a));
             std::vector<double> wd(dinv.size());
             std::vector<float> wd32(dinv.size());
             for (std::size_t i = 0; i < dinv.size(); ++i) {
                 wd[i] = w * dinv[i];
-                wd32[i] = static_cast<float>(wd[i]);
+                wd32[i] = static_cast<float>(w32 * dinv[i]);
             }
             d.wd64.upload(wd.data(), wd.size(), s);
             d.wd32.upload(wd32.data(), wd32.size(), s);
@@ -445,13 +514,18 @@ void System::set_amg(const cfdg_amg_view& v) {
             d.cinv.upload(v.coarse_inverse, static_cast<std::size_t>(d.n) * static_cast<std::size_t>(d.n), s);
         }
     }
+    // Level 0's operator is the assembled one (it may differ from the hierarchy's level 0, which
+    // only serves the estimate); diffusion operators have lambda(D^-1 A) <= 2.
+    const cfdg_csr& a0 = v.levels[0].a;
+    const double lambda0 = (L > 1 && a0.row_ptr != nullptr && a0.n_rows == n_) ? estimate_lambda(a0) : 2.0;
     levels_ = std::move(lv);
     sweeps_ = v.smooth_sweeps;
     weight_ = w;
+    weight0_32_ = mixed_weight(lambda0);
     has_amg_ = true;
     ++generation_;
-    // Level-0 smoother weights follow the (possibly new) weight.
-    k_scale_dinv<<<ceil_div(n_, k_block), k_block, 0, s>>>(n_, dinv0_.get(), weight_, wd0_64_.get(),
+    // Level-0 smoother weights follow the (possibly new) weights.
+    k_scale_dinv<<<ceil_div(n_, k_block), k_block, 0, s>>>(n_, dinv0_.get(), weight_, weight0_32_, wd0_64_.get(),
                                                             wd0_32_.get());
     hip_check(hipGetLastError(), "k_scale_dinv launch");
     hip_check(hipStreamSynchronize(s), "set_amg sync");
@@ -795,7 +869,12 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
         if (sweeps == 0) {
             CFDG_LAUNCH(k_zero_level<T>, g, s, d, blk, l, 0);
         } else if (sweeps == 1) {
-            CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 0);
+            // x1 = wd b is formed inside the residual and the prolongation (bitwise the same)
+            cur[li] = 0;
+            CFDG_LAUNCH(k_residual_x1<T>, ga, s, d, ba, l, 1);
+            CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1);
+            CFDG_LAUNCH(k_coarse<T>, tb.cn[li + 1], s, d, tb.c[li + 1].get(), l + 1, fin);
+            continue;
         } else {
             if (tb.a_ell[li] != 0) {
                 CFDG_LAUNCH(k_presmooth2<T>, ga, s, d, ba, l, 0);
@@ -819,7 +898,11 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
         const int ga = tb.tmn[0][li];
         const int2* ba = tb.tm[0][li].get();
         const int c0 = cur[li];
-        CFDG_LAUNCH(k_prolong<T>, tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
+        if (sweeps == 1) {
+            CFDG_LAUNCH((k_prolong<T, true>), tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
+        } else {
+            CFDG_LAUNCH((k_prolong<T, false>), tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
+        }
         int c = c0;
         for (int k = 0; k < sweeps; ++k) {
             const int o = c == 1 ? c0 : 1;
@@ -949,9 +1032,9 @@ public:
             if (amg) {
                 require(sys_[i]->has_amg(), "solve: use_amg requested but no hierarchy was set");
                 if (i == 0) {
-                    sweeps = sys_[i]->sweeps();
+                    sweeps = sys_[i]->sweeps(fp32);
                 }
-                require(sys_[i]->sweeps() == sweeps, "solve_batch: systems differ in smooth_sweeps");
+                require(sys_[i]->sweeps(fp32) == sweeps, "solve_batch: systems differ in smooth_sweeps");
             }
             gens_.push_back(sys_[i]->generation());
             uids_.push_back(sys_[i]->uid());
@@ -1206,7 +1289,8 @@ SlotRunner::SlotRunner(int slots, const BatchOptions& opt, int sweeps, hipStream
     SlotMachinery& m = *m_;
     m.k = slots;
     const bool amg = opt.use_amg && opt.method == CFDG_METHOD_CG;
-    m.pl = Plan{opt.method, amg, amg && opt.precision != CFDG_PREC_FP64, sweeps};
+    const bool fp32 = amg && opt.precision != CFDG_PREC_FP64;
+    m.pl = Plan{opt.method, amg, fp32, fp32 ? mixed_sweeps() : sweeps};
     m.precision = amg ? opt.precision : CFDG_PREC_FP64;
     m.s = s;
     const std::size_t k = static_cast<std::size_t>(slots);
@@ -1254,7 +1338,7 @@ int SlotRunner::start(System* x, const cfdg_solve_control& c) {
     require(it != m.phase.end(), "slot runner: no free slot");
     if (m.pl.amg) {
         require(x->has_amg(), "solve: use_amg requested but no hierarchy was set");
-        require(x->sweeps() == m.pl.sweeps, "slot runner: system's smooth_sweeps differ from the runner's");
+        require(x->sweeps(m.pl.fp32) == m.pl.sweeps, "slot runner: system's smooth_sweeps differ from the runner's");
     }
     const int j = static_cast<int>(it - m.phase.begin());
     const std::size_t ju = static_cast<std::size_t>(j);

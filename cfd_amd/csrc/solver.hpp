@@ -50,6 +50,9 @@ struct Probe {
 
 class Runner;
 
+// Sweeps per side of the mixed-precision V-cycle (process-wide setting).
+int mixed_sweeps();
+
 // A linear system resident in HBM: level-0 operator (FP64 + FP32 values), right-hand side,
 // iterate, Krylov work vectors, and optionally an AMG hierarchy.
 class System {
@@ -78,7 +81,9 @@ public:
     [[nodiscard]] int n_faces() const { return nf_; }
     [[nodiscard]] int n_levels() const { return has_amg_ ? static_cast<int>(levels_.size()) : 0; }
     [[nodiscard]] bool has_amg() const { return has_amg_; }
-    [[nodiscard]] int sweeps() const { return sweeps_; }
+    // V-cycle sweeps per side: the hierarchy's (FP64, the reference algorithm) or the mixed
+    // smoother's (FP32 V-cycle).
+    [[nodiscard]] int sweeps(bool fp32 = false) const { return fp32 ? mixed_sweeps() : sweeps_; }
     [[nodiscard]] double weight() const { return weight_; }
     [[nodiscard]] hipStream_t stream() const { return stream_.get(); }
     [[nodiscard]] SolveState* state() const { return state_.get(); }
@@ -130,6 +135,7 @@ private:
     bool assembled_ = false;
     int sweeps_ = 2;
     double weight_ = 0.7;
+    double weight0_32_ = 0.7; // level-0 damping of the mixed smoother
     unsigned generation_ = 0;
     unsigned long long uid_ = next_uid();
     static unsigned long long next_uid();
