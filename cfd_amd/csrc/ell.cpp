@@ -119,19 +119,21 @@ void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hi
     std::vector<short> d;
     if (short_ok) {
         d.resize(e.col.size());
-        for (int sl = 0; sl < e.n_slices && short_ok; ++sl) {
+        int fits = 1;
+#pragma omp parallel for schedule(static) reduction(min : fits) if (e.n_slices > 256)
+        for (int sl = 0; sl < e.n_slices; ++sl) {
             const int base = e.sp[static_cast<std::size_t>(sl)];
             const int end = e.sp[static_cast<std::size_t>(sl) + 1];
             for (int k = base; k < end; ++k) {
                 const int row = sl * (k_wave / e.lanes) + ((k - base) % k_wave) / e.lanes;
                 const int off = e.col[static_cast<std::size_t>(k)] - row;
                 if (off < -32768 || off > 32767) {
-                    short_ok = false;
-                    break;
+                    fits = 0;
                 }
                 d[static_cast<std::size_t>(k)] = static_cast<short>(off);
             }
         }
+        short_ok = fits != 0;
     }
     if (short_ok && !std::getenv("CFDG_NO_SHORT_COLS")) {
         dcol.upload(d.data(), d.size(), s);

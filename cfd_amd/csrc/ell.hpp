@@ -30,9 +30,11 @@ long long ell_slots(int n_rows, const int* rp, int lanes);
 template <class T>
 std::vector<T> ell_values(const HostEll& e, const double* va) {
     std::vector<T> v(e.col.size());
-    for (std::size_t k = 0; k < v.size(); ++k) {
-        const int s = e.src[k];
-        v[k] = s >= 0 ? static_cast<T>(va[s]) : T(0);
+    const long long n = static_cast<long long>(v.size());
+#pragma omp parallel for schedule(static) if (n > (1 << 16))
+    for (long long k = 0; k < n; ++k) {
+        const int s = e.src[static_cast<std::size_t>(k)];
+        v[static_cast<std::size_t>(k)] = s >= 0 ? static_cast<T>(va[s]) : T(0);
     }
     return v;
 }

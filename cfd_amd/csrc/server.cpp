@@ -306,9 +306,21 @@ private:
         for (std::size_t k = 0; k < steps; ++k) {
             hip_check(hipMemcpy(r.rhs_dev.get() + k * n, r.rhs[k], n * sizeof(double), hipMemcpyHostToDevice),
                       "H2D right-hand side");
+            // |b| over 64 fixed chunks summed in order (reproducible for any thread count)
+            constexpr int chunks = 64;
+            double part[chunks];
+            const double* b = r.rhs[k];
+#pragma omp parallel for schedule(static) if (n > 100000)
+            for (int c = 0; c < chunks; ++c) {
+                double t = 0.0;
+                for (std::size_t i = n * c / chunks; i < n * (c + 1) / chunks; ++i) {
+                    t += b[i] * b[i];
+                }
+                part[c] = t;
+            }
             double s = 0.0;
-            for (std::size_t i = 0; i < n; ++i) {
-                s += r.rhs[k][i] * r.rhs[k][i];
+            for (int c = 0; c < chunks; ++c) {
+                s += part[c];
             }
             r.tol[k] = q.rel_tolerance * std::sqrt(s);
         }

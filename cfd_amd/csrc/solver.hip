@@ -93,8 +93,10 @@ __global__ void __launch_bounds__(k_block) k_scale_dinv(int n, const double* __r
     }
 }
 
-// Largest eigenvalue of D^-1 A (D = diag A) by 15 power iterations from a fixed start vector;
-// rows in parallel, norms in a fixed order, so the estimate is reproducible.
+// Largest eigenvalue of D^-1 A (D = diag A) by 20 power iterations from a fixed pseudo-random
+// start vector (a smooth start has almost no component along the oscillatory top eigenvector of
+// a regular grid and underestimates it); rows in parallel, norms in a fixed order, so the
+// estimate is reproducible.
 double estimate_lambda(const cfdg_csr& a) {
     const int n = a.n_rows;
     std::vector<double> dinv(static_cast<std::size_t>(n), 1.0);
@@ -107,11 +109,15 @@ double estimate_lambda(const cfdg_csr& a) {
     }
     std::vector<double> x(static_cast<std::size_t>(n));
     std::vector<double> y(static_cast<std::size_t>(n));
-    for (int i = 0; i < n; ++i) {
-        x[static_cast<std::size_t>(i)] = 1.0 + 0.5 * std::sin(0.7 * i + 0.3);
+    for (int i = 0; i < n; ++i) { // splitmix64 of the index, mapped to [-1, 1)
+        unsigned long long z = 0x9E3779B97F4A7C15ULL * static_cast<unsigned long long>(i + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        x[static_cast<std::size_t>(i)] = static_cast<double>(z >> 11) * 0x1.0p-52 - 1.0;
     }
     double lambda = 0.0;
-    for (int it = 0; it < 15; ++it) {
+    for (int it = 0; it < 20; ++it) {
 #pragma omp parallel for schedule(static) if (n > 20000)
         for (int i = 0; i < n; ++i) {
             double t = 0.0;
@@ -120,17 +126,35 @@ double estimate_lambda(const cfdg_csr& a) {
             }
             y[static_cast<std::size_t>(i)] = dinv[static_cast<std::size_t>(i)] * t;
         }
+        // norms over 64 fixed chunks summed in order (reproducible for any thread count)
+        constexpr int chunks = 64;
+        double cx[chunks];
+        double cy[chunks];
+#pragma omp parallel for schedule(static) if (n > 20000)
+        for (int c = 0; c < chunks; ++c) {
+            const long long lo = static_cast<long long>(n) * c / chunks;
+            const long long hi = static_cast<long long>(n) * (c + 1) / chunks;
+            double sx = 0.0;
+            double sy = 0.0;
+            for (long long i = lo; i < hi; ++i) {
+                sx += x[static_cast<std::size_t>(i)] * x[static_cast<std::size_t>(i)];
+                sy += y[static_cast<std::size_t>(i)] * y[static_cast<std::size_t>(i)];
+            }
+            cx[c] = sx;
+            cy[c] = sy;
+        }
         double xx = 0.0;
         double yy = 0.0;
-        for (int i = 0; i < n; ++i) {
-            xx += x[static_cast<std::size_t>(i)] * x[static_cast<std::size_t>(i)];
-            yy += y[static_cast<std::size_t>(i)] * y[static_cast<std::size_t>(i)];
+        for (int c = 0; c < chunks; ++c) {
+            xx += cx[c];
+            yy += cy[c];
         }
         lambda = xx > 0.0 ? std::sqrt(yy / xx) : 0.0;
         if (!(lambda > 0.0) || !std::isfinite(lambda)) {
             return 2.0;
         }
         const double inv = 1.0 / std::sqrt(yy);
+#pragma omp parallel for schedule(static) if (n > 20000)
         for (int i = 0; i < n; ++i) {
             x[static_cast<std::size_t>(i)] = y[static_cast<std::size_t>(i)] * inv;
         }
@@ -138,18 +162,43 @@ double estimate_lambda(const cfdg_csr& a) {
     return lambda;
 }
 
-// Damping of a mixed-precision Jacobi sweep on a level with D^-1 A eigenvalue estimate lambda:
-// the centre of the interval [h / ratio, h], h = 1.1 lambda (the first-degree Chebyshev
-// smoother, which damps the upper part of the spectrum uniformly).
-double mixed_weight(double lambda) {
+// Gershgorin bound of the eigenvalues of D^-1 A: max_i sum_j |a_ij| / |a_ii| (2 for the
+// diffusion operators with zero row sums, which is also their largest eigenvalue on fine grids).
+double gershgorin(const cfdg_csr& a) {
+    double g = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : g) if (a.n_rows > 20000)
+    for (int i = 0; i < a.n_rows; ++i) {
+        double d = 0.0;
+        double s = 0.0;
+        for (int k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
+            s += std::fabs(a.val[k]);
+            if (a.col[k] == i) {
+                d = std::fabs(a.val[k]);
+            }
+        }
+        if (d > 0.0) {
+            g = std::max(g, s / d);
+        }
+    }
+    return g > 0.0 ? g : 2.0;
+}
+
+// Damping of a mixed-precision Jacobi sweep on a level whose D^-1 A spectrum lies below h: the
+// centre of [h / ratio, h] (the first-degree Chebyshev smoother, which damps the upper part of
+// the spectrum uniformly). A sweep diverges on the top mode only if h (1 + 1 / ratio) is below
+// the largest eigenvalue.
+double mixed_weight(double h) {
     static const double ratio = [] {
         const char* e = std::getenv("CFDG_MIXED_RATIO");
         const double r = e != nullptr ? std::atof(e) : 10.0;
         return r > 1.0 ? r : 10.0;
     }();
-    const double h = 1.1 * lambda;
     return 2.0 / (h + h / ratio);
 }
+
+// Upper end h of a coarse level's spectrum: the power-iteration estimate (4-8% low on the box
+// hierarchies) with a 15% margin.
+double coarse_spectrum_top(const cfdg_csr& a) { return 1.15 * estimate_lambda(a); }
 
 // Level-0 dot products from a finished V-cycle output v[buf] (used when no Jacobi sweep on
 // level 0 could carry them: one-level hierarchies and smoothing-free cycles).
@@ -219,6 +268,7 @@ __global__ void k_init_states(const SysDev* __restrict__ sys, int n_sys, const S
 // inverse_diagonal (krylov_openmp.cpp:147-157): 1 / a_ii, 1.0 for a zero or missing diagonal.
 std::vector<double> inverse_diagonal(const cfdg_csr& m) {
     std::vector<double> d(static_cast<std::size_t>(m.n_rows), 1.0);
+#pragma omp parallel for schedule(static) if (m.n_rows > 20000)
     for (int i = 0; i < m.n_rows; ++i) {
         for (int k = m.row_ptr[i]; k < m.row_ptr[i + 1]; ++k) {
             if (m.col[k] == i && m.val[k] != 0.0) {
@@ -305,8 +355,11 @@ System::System(const cfdg_ldu_pattern& pt) {
     }
     const HostEll e = csr_to_ell(n_, n_, pt.row_ptr, pt.col);
     std::vector<int> src(e.src.size());
-    for (std::size_t k = 0; k < src.size(); ++k) {
-        src[k] = e.src[k] >= 0 ? csr_src[static_cast<std::size_t>(e.src[k])] : -1;
+    const long long n_src = static_cast<long long>(src.size());
+#pragma omp parallel for schedule(static) if (n_src > (1 << 16))
+    for (long long k = 0; k < n_src; ++k) {
+        const int c = e.src[static_cast<std::size_t>(k)];
+        src[static_cast<std::size_t>(k)] = c >= 0 ? csr_src[static_cast<std::size_t>(c)] : -1;
     }
     hipStream_t s = stream_.get();
     a0_.upload(e, nullptr, false, false, s);
@@ -495,7 +548,7 @@ void System::set_amg(const cfdg_amg_view& v) {
             const cfdg_csr& a = v.levels[l].a;
             d.a.upload_csr(d.n, d.n, a.row_ptr, a.col, a.val, s);
             const std::vector<double> dinv = inverse_diagonal(a);
-            const double w32 = mixed_weight(estimate_lambda(a));
+            const double w32 = mixed_weight(coarse_spectrum_top(a));
             std::vector<double> wd(dinv.size());
             std::vector<float> wd32(dinv.size());
             for (std::size_t i = 0; i < dinv.size(); ++i) {
@@ -515,9 +568,9 @@ void System::set_amg(const cfdg_amg_view& v) {
         }
     }
     // Level 0's operator is the assembled one (it may differ from the hierarchy's level 0, which
-    // only serves the estimate); diffusion operators have lambda(D^-1 A) <= 2.
+    // only serves the bound); diffusion operators have lambda(D^-1 A) <= 2.
     const cfdg_csr& a0 = v.levels[0].a;
-    const double lambda0 = (L > 1 && a0.row_ptr != nullptr && a0.n_rows == n_) ? estimate_lambda(a0) : 2.0;
+    const double lambda0 = (L > 1 && a0.row_ptr != nullptr && a0.n_rows == n_) ? gershgorin(a0) : 2.0;
     levels_ = std::move(lv);
     sweeps_ = v.smooth_sweeps;
     weight_ = w;
@@ -681,6 +734,7 @@ struct Tables {
     std::vector<TableRef> tm[3];
     std::vector<int> tmn[3];
     std::vector<char> a_ell; // every system's level-l operator is sliced ELL (fused presmooth)
+    std::vector<char> x1;    // the level's residual forms x1 = wd b per gather (short rows or small)
     DevBuf<int2> arena;      // every table above, back to back
     PinnedBuf<int2> stage;   // host image of the arena for asynchronous uploads
 };
@@ -749,6 +803,7 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
     tb.t0_single_n = static_cast<int>(single.size());
     img.insert(img.end(), single.begin(), single.end());
     tb.a_ell.assign(L, 1);
+    tb.x1.assign(L, 1);
     for (int which = 0; which < 3; ++which) {
         tb.tmn[which].assign(L, 0);
         off_m[which].assign(L, img.size());
@@ -767,6 +822,11 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
                 // short rows of the finest operator.
                 if (which == 0 && (tpr != 1 || l > 0)) {
                     tb.a_ell[lu] = 0;
+                }
+                // Forming x1 per gather doubles the gathers of a row; for many long rows a
+                // stored x1 (one more row-wise launch) is cheaper.
+                if (which == 0 && tpr != 1 && static_cast<long long>(x->level_rows(l)) * tpr >= (1LL << 19)) {
+                    tb.x1[lu] = 0;
                 }
                 const int rows = which == 2 ? x->level_rows(l + 1) : x->level_rows(l);
                 const int nb = ceil_div(static_cast<long long>(rows) * tpr, k_block);
@@ -869,9 +929,15 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
         if (sweeps == 0) {
             CFDG_LAUNCH(k_zero_level<T>, g, s, d, blk, l, 0);
         } else if (sweeps == 1) {
-            // x1 = wd b is formed inside the residual and the prolongation (bitwise the same)
+            // x1 = wd b is formed inside the residual (or stored first, for many long rows) and
+            // inside the prolongation; bitwise the same as a stored sweep
             cur[li] = 0;
-            CFDG_LAUNCH(k_residual_x1<T>, ga, s, d, ba, l, 1);
+            if (tb.x1[li] != 0) {
+                CFDG_LAUNCH(k_residual_x1<T>, ga, s, d, ba, l, 1);
+            } else {
+                CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 0);
+                CFDG_LAUNCH(k_residual<T>, ga, s, d, ba, l, 0, 1);
+            }
             CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1);
             CFDG_LAUNCH(k_coarse<T>, tb.cn[li + 1], s, d, tb.c[li + 1].get(), l + 1, fin);
             continue;
