@@ -14,6 +14,24 @@ from .krylov import (AmgLevelView, AmgView, Batch, CsrView, DeviceSolver, LduPat
                      Precision, SolveControl, SolveStats, assemble_convection_diffusion,
                      backend_name, device_available, device_description, solve, solve_amg_cg)
 
+
+def _apply_tuning() -> None:
+    """Installs cfd_amd/tuning.json (written by python -m cfd_amd.autotune on the target GPU)
+    unless CFDG_NO_TUNING is set."""
+    import json
+    import os
+    from pathlib import Path
+
+    path = Path(__file__).resolve().parent / "tuning.json"
+    if os.environ.get("CFDG_NO_TUNING") or not path.exists():
+        return
+    from .autotune import apply
+
+    apply(json.loads(path.read_text()))
+
+
+_apply_tuning()
+
 __all__ = [
     "AmgHierarchy", "AmgSettings", "AmgLevelView", "AmgView", "Batch", "CsrView", "DeviceSolver",
     "LduPattern", "Method", "Precision", "SolveControl", "SolveStats",

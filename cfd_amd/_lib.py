@@ -139,6 +139,17 @@ class ServerCounters(C.Structure):
     ]
 
 
+class LayoutRule(C.Structure):
+    _fields_ = [
+        ("min_rows", C.c_int),
+        ("max_rows", C.c_int),
+        ("min_mean", C.c_double),
+        ("max_mean", C.c_double),
+        ("lanes", C.c_int),
+        ("csr", C.c_int),
+    ]
+
+
 class AmgSettings(C.Structure):
     _fields_ = [
         ("strength_threshold", C.c_double),
@@ -199,6 +210,10 @@ _SIGNATURES = {
                                C.POINTER(VP)]),
     "cfdg_batch_destroy": (ST, [VP]),
     "cfdg_batch_solve": (ST, [VP, C.POINTER(SolveControl), C.POINTER(SolveStats)]),
+    "cfdg_set_layout_rules": (ST, [C.POINTER(LayoutRule), C.c_int]),
+    "cfdg_layout_of": (ST, [C.c_int, P_INT, P_INT, P_INT]),
+    "cfdg_set_mixed_smoother": (ST, [C.c_int, C.c_double]),
+    "cfdg_spmv_time": (ST, [C.POINTER(Csr), C.c_int, C.c_int, C.c_int, C.c_int, P_DBL]),
     "cfdg_estimate_footprint": (ST, [C.POINTER(LduPattern), C.POINTER(AmgView),
                                      C.POINTER(C.c_size_t)]),
     "cfdg_server_default_options": (ST, [C.POINTER(ServerOptions)]),

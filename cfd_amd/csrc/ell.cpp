@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <limits>
+#include <mutex>
 
 namespace cfdg {
 
@@ -154,13 +155,48 @@ void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hi
     }
 }
 
-void DevEll::upload_csr(int rows, int cols, const int* rp, const int* ci, const double* va,
-                        hipStream_t s, int tpr_override) {
-    const int t = tpr_override > 0 ? tpr_override : pick_tpr(rows, rp[rows]);
+namespace {
+std::mutex g_rules_mutex;
+std::vector<cfdg_layout_rule> g_rules;
+} // namespace
+
+void set_layout_rules(const cfdg_layout_rule* rules, int count) {
+    std::vector<cfdg_layout_rule> r;
+    for (int i = 0; i < count; ++i) {
+        const cfdg_layout_rule& x = rules[i];
+        require(x.lanes >= 1 && x.lanes <= k_wave && (x.lanes & (x.lanes - 1)) == 0,
+                "layout rule: lanes must be a power of two <= 64");
+        require(x.csr == 0 || x.csr == 1, "layout rule: csr must be 0 or 1");
+        require(x.min_rows <= x.max_rows && x.min_mean <= x.max_mean, "layout rule: empty range");
+        r.push_back(x);
+    }
+    std::lock_guard<std::mutex> lk(g_rules_mutex);
+    g_rules = std::move(r);
+}
+
+Layout choose_layout(int rows, const int* rp) {
+    const double mean = rows > 0 ? static_cast<double>(rp[rows]) / rows : 0.0;
+    {
+        std::lock_guard<std::mutex> lk(g_rules_mutex);
+        for (const cfdg_layout_rule& x : g_rules) {
+            if (rows >= x.min_rows && rows <= x.max_rows && mean >= x.min_mean && mean <= x.max_mean) {
+                return Layout{x.lanes, x.csr};
+            }
+        }
+    }
+    const int t = pick_tpr(rows, rp[rows]);
     // Sliced ELL (coalesced, 16-bit columns) unless its padding would exceed the CSR entries by
     // more than CFDG_ELL_MAX_PAD (default 1.5).
     static const double max_pad = env_or("CFDG_ELL_MAX_PAD", 1.5);
-    if (t == 1 || static_cast<double>(ell_slots(rows, rp, t)) <= max_pad * static_cast<double>(rp[rows])) {
+    const bool ell = t == 1 || static_cast<double>(ell_slots(rows, rp, t)) <= max_pad * static_cast<double>(rp[rows]);
+    return Layout{t, ell ? 0 : 1};
+}
+
+void DevEll::upload_csr(int rows, int cols, const int* rp, const int* ci, const double* va,
+                        hipStream_t s, const Layout* forced) {
+    const Layout lay = forced != nullptr ? *forced : choose_layout(rows, rp);
+    const int t = lay.lanes;
+    if (lay.csr == 0) {
         upload(csr_to_ell(rows, cols, rp, ci, t), va, true, true, s);
         return;
     }
@@ -175,8 +211,10 @@ void DevEll::upload_csr(int rows, int cols, const int* rp, const int* ci, const 
     col.upload(ci, nnz, s);
     v64.upload(va, nnz, s);
     std::vector<float> f(nnz);
-    for (std::size_t k = 0; k < nnz; ++k) {
-        f[k] = static_cast<float>(va[k]);
+    const long long m = static_cast<long long>(nnz);
+#pragma omp parallel for schedule(static) if (m > (1 << 16))
+    for (long long k = 0; k < m; ++k) {
+        f[static_cast<std::size_t>(k)] = static_cast<float>(va[k]);
     }
     v32.upload(f.data(), nnz, s);
 }

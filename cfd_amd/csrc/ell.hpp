@@ -43,6 +43,18 @@ std::vector<T> ell_values(const HostEll& e, const double* va) {
 // quarter of the mean row length, at most 64.
 int pick_tpr(int n_rows, long long nnz);
 
+// Layout of a matrix on the device: lanes per row and sliced ELL (csr = 0) or CSR (csr = 1).
+struct Layout {
+    int lanes;
+    int csr;
+};
+
+// The layout for a matrix with these rows and entries: the first rule of the tuned table that
+// matches (set_layout_rules), else the built-in rule (pick_tpr, then sliced ELL unless its
+// padding exceeds 1.5 times the entries).
+Layout choose_layout(int n_rows, const int* rp);
+void set_layout_rules(const cfdg_layout_rule* rules, int count);
+
 // Device copy of a sparse matrix (sliced ELL or CSR, see EllView) with FP64 and / or FP32
 // values.
 struct DevEll {
@@ -61,9 +73,9 @@ struct DevEll {
     DevBuf<float> diag32;        // per padded row
 
     void upload(const HostEll& e, const double* va, bool fp64, bool fp32, hipStream_t s);
-    // Chooses the layout from the row lengths (tpr_override > 0 forces it).
+    // Chooses the layout from the row lengths (choose_layout), or uses the one given.
     void upload_csr(int n_rows, int n_cols, const int* rp, const int* ci, const double* va,
-                    hipStream_t s, int tpr_override = 0);
+                    hipStream_t s, const Layout* forced = nullptr);
     [[nodiscard]] EllView view(const void* v, int fmt = 0) const {
         EllView e{};
         e.n_rows = n_rows;

@@ -188,11 +188,7 @@ double gershgorin(const cfdg_csr& a) {
 // the spectrum uniformly). A sweep diverges on the top mode only if h (1 + 1 / ratio) is below
 // the largest eigenvalue.
 double mixed_weight(double h) {
-    static const double ratio = [] {
-        const char* e = std::getenv("CFDG_MIXED_RATIO");
-        const double r = e != nullptr ? std::atof(e) : 10.0;
-        return r > 1.0 ? r : 10.0;
-    }();
+    const double ratio = mixed_ratio();
     return 2.0 / (h + h / ratio);
 }
 
@@ -285,23 +281,51 @@ void check_csr(const cfdg_csr& m, int rows, int cols, const char* what) {
     require(m.nnz >= 0 && m.row_ptr != nullptr, what);
     require(m.nnz == 0 || (m.col != nullptr && m.val != nullptr), what);
     require(m.row_ptr[0] == 0 && m.row_ptr[rows] == m.nnz, what);
+    int ok = 1;
+#pragma omp parallel for schedule(static) reduction(min : ok) if (rows > 20000)
     for (int i = 0; i < rows; ++i) {
-        require(m.row_ptr[i + 1] >= m.row_ptr[i], what);
+        ok = std::min(ok, m.row_ptr[i + 1] >= m.row_ptr[i] ? 1 : 0);
     }
+    require(ok != 0, what);
+#pragma omp parallel for schedule(static) reduction(min : ok) if (m.nnz > 100000)
     for (int k = 0; k < m.nnz; ++k) {
-        require(m.col[k] >= 0 && m.col[k] < cols, what);
+        ok = std::min(ok, (m.col[k] >= 0 && m.col[k] < cols) ? 1 : 0);
     }
+    require(ok != 0, what);
 }
 
 } // namespace
 
-// Smoother of the mixed-precision V-cycle: sweeps per side (CFDG_MIXED_SWEEPS, default 1) and
-// the ratio of the damping interval (CFDG_MIXED_RATIO, default 10; see mixed_weight).
+// Smoother of the mixed-precision V-cycle: sweeps per side (default 1, or CFDG_MIXED_SWEEPS)
+// and the ratio of the damping interval (default 10, or CFDG_MIXED_RATIO; see mixed_weight),
+// process-wide, settable with cfdg_set_mixed_smoother.
+namespace {
+std::atomic<int> g_mixed_sweeps{-1};
+std::atomic<double> g_mixed_ratio{-1.0};
+} // namespace
+
 int mixed_sweeps() {
-    static const int v = [] {
+    int v = g_mixed_sweeps.load();
+    if (v < 0) {
         const char* e = std::getenv("CFDG_MIXED_SWEEPS");
-        return e != nullptr ? std::max(0, std::min(64, std::atoi(e))) : 1;
-    }();
+        v = e != nullptr ? std::max(0, std::min(64, std::atoi(e))) : 1;
+        int expected = -1;
+        g_mixed_sweeps.compare_exchange_strong(expected, v);
+        v = g_mixed_sweeps.load();
+    }
+    return v;
+}
+
+double mixed_ratio() {
+    double v = g_mixed_ratio.load();
+    if (v < 0.0) {
+        const char* e = std::getenv("CFDG_MIXED_RATIO");
+        const double r = e != nullptr ? std::atof(e) : 10.0;
+        v = r > 1.0 ? r : 10.0;
+        double expected = -1.0;
+        g_mixed_ratio.compare_exchange_strong(expected, v);
+        v = g_mixed_ratio.load();
+    }
     return v;
 }
 
@@ -335,9 +359,12 @@ System::System(const cfdg_ldu_pattern& pt) {
     require(pt.n_faces == 0 || (pt.upper_pos != nullptr && pt.lower_pos != nullptr),
             "LduPattern: null face positions");
     require(pt.row_ptr[0] == 0 && pt.row_ptr[pt.n] == pt.nnz, "LduPattern: row_ptr mismatch");
+    int cols_ok = 1;
+#pragma omp parallel for schedule(static) reduction(min : cols_ok) if (pt.nnz > 100000)
     for (int k = 0; k < pt.nnz; ++k) {
-        require(pt.col[k] >= 0 && pt.col[k] < pt.n, "LduPattern: column out of range");
+        cols_ok = std::min(cols_ok, (pt.col[k] >= 0 && pt.col[k] < pt.n) ? 1 : 0);
     }
+    require(cols_ok != 0, "LduPattern: column out of range");
     n_ = pt.n;
     nnz0_ = pt.nnz;
     nf_ = pt.n_faces;
@@ -1141,6 +1168,9 @@ public:
             }
         }
         const bool amg = opt.use_amg && opt.method == CFDG_METHOD_CG;
+        if (amg && sys_[0]->sweeps(pl_.fp32) != pl_.sweeps) {
+            return false; // the mixed smoother's sweeps were changed
+        }
         return opt.method == pl_.method && amg == pl_.amg &&
                (amg ? opt.precision : CFDG_PREC_FP64) == precision_;
     }
@@ -1746,6 +1776,99 @@ extern "C" cfdg_status cfdg_system_probe_read(const cfdg_system* sys, int precis
         } else {
             *bytes_per_launch = 0.0;
         }
+    });
+}
+
+namespace {
+
+// y = A x for one matrix in any layout (layout timing).
+template <class T>
+__global__ void __launch_bounds__(k_block) k_spmv_view(EllView m, const T* __restrict__ x, T* __restrict__ y) {
+    const int tpr = m.tpr;
+    const int row = static_cast<int>(blockIdx.x) * (k_block / tpr) + static_cast<int>(threadIdx.x) / tpr;
+    const int sub = static_cast<int>(threadIdx.x) & (tpr - 1);
+    const T v = mat_row<T>(m, row, sub, x);
+    if (sub == 0 && row < m.n_rows) {
+        y[row] = v;
+    }
+}
+
+} // namespace
+
+extern "C" cfdg_status cfdg_set_mixed_smoother(int sweeps, double ratio) {
+    return guarded([&] {
+        require(sweeps >= 0 && sweeps <= 64, "mixed smoother: sweeps must be in [0, 64]");
+        require(ratio > 1.0, "mixed smoother: ratio must exceed 1");
+        g_mixed_sweeps.store(sweeps);
+        g_mixed_ratio.store(ratio);
+    });
+}
+
+extern "C" cfdg_status cfdg_layout_of(int n_rows, const int* row_ptr, int* lanes, int* csr) {
+    return guarded([&] {
+        require(n_rows >= 0 && row_ptr != nullptr && lanes != nullptr && csr != nullptr, "null argument");
+        const Layout l = choose_layout(n_rows, row_ptr);
+        *lanes = l.lanes;
+        *csr = l.csr;
+    });
+}
+
+extern "C" cfdg_status cfdg_set_layout_rules(const cfdg_layout_rule* rules, int count) {
+    return guarded([&] {
+        require(count >= 0 && (count == 0 || rules != nullptr), "layout rules: null table");
+        set_layout_rules(rules, count);
+    });
+}
+
+extern "C" cfdg_status cfdg_spmv_time(const cfdg_csr* a, int lanes, int csr, int fp32, int reps, double* avg_ms) {
+    return guarded([&] {
+        require(a != nullptr && avg_ms != nullptr && reps >= 1, "null argument");
+        require(lanes >= 1 && lanes <= k_wave && (lanes & (lanes - 1)) == 0, "lanes must be a power of two <= 64");
+        require(csr == 0 || csr == 1, "csr must be 0 or 1");
+        require_device();
+        const int cols = a->n_cols == 0 ? a->n_rows : a->n_cols;
+        check_csr(*a, a->n_rows, cols, "spmv_time: invalid CSR matrix");
+        Stream stream;
+        hipStream_t s = stream.get();
+        DevEll m;
+        const Layout lay{lanes, csr};
+        m.upload_csr(a->n_rows, cols, a->row_ptr, a->col, a->val, s, &lay);
+        const std::vector<double> ones(static_cast<std::size_t>(cols), 1.0);
+        const std::vector<float> ones32(static_cast<std::size_t>(cols), 1.0f);
+        DevBuf<double> x64;
+        DevBuf<double> y64;
+        DevBuf<float> x32;
+        DevBuf<float> y32;
+        x64.upload(ones.data(), ones.size(), s);
+        x32.upload(ones32.data(), ones32.size(), s);
+        y64.resize(static_cast<std::size_t>(a->n_rows));
+        y32.resize(static_cast<std::size_t>(a->n_rows));
+        const int grid = ceil_div(static_cast<long long>(a->n_rows) * lanes, k_block);
+        auto launch = [&] {
+            if (grid == 0) {
+                return;
+            }
+            if (fp32 != 0) {
+                k_spmv_view<float><<<grid, k_block, 0, s>>>(m.view32(), x32.get(), y32.get());
+            } else {
+                k_spmv_view<double><<<grid, k_block, 0, s>>>(m.view64(), x64.get(), y64.get());
+            }
+        };
+        for (int i = 0; i < 3; ++i) {
+            launch();
+        }
+        hip_check(hipGetLastError(), "spmv launch");
+        Event e0;
+        Event e1;
+        hip_check(hipEventRecord(e0.get(), s), "event record");
+        for (int i = 0; i < reps; ++i) {
+            launch();
+        }
+        hip_check(hipEventRecord(e1.get(), s), "event record");
+        hip_check(hipEventSynchronize(e1.get()), "event sync");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0.get(), e1.get()), "event elapsed");
+        *avg_ms = static_cast<double>(ms) / reps;
     });
 }
 

@@ -50,8 +50,10 @@ struct Probe {
 
 class Runner;
 
-// Sweeps per side of the mixed-precision V-cycle (process-wide setting).
+// Sweeps per side of the mixed-precision V-cycle and the ratio of its damping interval
+// (process-wide settings).
 int mixed_sweeps();
+double mixed_ratio();
 
 // A linear system resident in HBM: level-0 operator (FP64 + FP32 values), right-hand side,
 // iterate, Krylov work vectors, and optionally an AMG hierarchy.

@@ -268,6 +268,35 @@ typedef struct cfdg_server_counters {
     int waiting;                 /* submitted requests not yet resident */
 } cfdg_server_counters;
 
+/* ------------------------------------------------------------------------------------------
+ * Layout tuning (no counterpart in the reference). Coarse operators and transfer matrices are
+ * stored as sliced ELL or CSR with `lanes` lanes per row. The library's built-in rule picks
+ * them from the mean row length; a tuned table (written by python -m cfd_amd.autotune from
+ * timings of cfdg_spmv_time on this GPU) overrides it for the matrices it covers. The table is
+ * process-wide and applies to hierarchies uploaded after it is set.
+ * ---------------------------------------------------------------------------------------- */
+
+typedef struct cfdg_layout_rule {
+    int min_rows;
+    int max_rows;
+    double min_mean;  /* mean entries per row */
+    double max_mean;
+    int lanes;        /* power of two <= 64 */
+    int csr;          /* 0 sliced ELL, 1 CSR */
+} cfdg_layout_rule;
+
+/* count 0 restores the built-in rule. */
+cfdg_status cfdg_set_layout_rules(const cfdg_layout_rule* rules, int count);
+/* The layout the library picks for a matrix with these row offsets (rules, then built-in). */
+cfdg_status cfdg_layout_of(int n_rows, const int* row_ptr, int* lanes, int* csr);
+/* Smoother of the mixed-precision V-cycle: damped-Jacobi sweeps per side (default 1) and the
+ * ratio of the damping interval [h / ratio, h] (default 10), h from a per-level estimate of the
+ * largest eigenvalue of D^-1 A. The ratio applies to hierarchies set afterwards. */
+cfdg_status cfdg_set_mixed_smoother(int sweeps, double ratio);
+/* Average device time of y = A x (fp32 != 0: FP32 values, else FP64) over reps launches with
+ * the given layout. */
+cfdg_status cfdg_spmv_time(const cfdg_csr* a, int lanes, int csr, int fp32, int reps, double* avg_ms);
+
 /* Upper bound of the device bytes a system of this pattern and hierarchy (may be NULL) holds;
  * host-only, usable for capacity planning without a device. */
 cfdg_status cfdg_estimate_footprint(const cfdg_ldu_pattern* pattern, const cfdg_amg_view* amg,

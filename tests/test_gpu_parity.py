@@ -173,3 +173,29 @@ def test_empty_hierarchy_is_invalid_argument():
 
     with pytest.raises(InvalidArgument):
         cfd_amd.solve_amg_cg(cfd_amd.AmgView(), np.zeros(1), np.zeros(1), SolveControl())
+
+
+@pytest.mark.parametrize("prec", [Precision.FP64, Precision.MIXED_FP32, Precision.MIXED_BF16])
+def test_precision_stress_sphere_against_oracle(oracle, prec):
+    """The precision-stress case (rho 1000 inside a seeded sphere, harmonic face averaging) at
+    32^3, solved to 1e-10: every precision reaches the tolerance in FP64 terms and its solution
+    agrees with the oracle's FP64 AMG-PCG solution to the accuracy the tolerance implies."""
+    mesh, s = cases.config3(32)
+    pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
+    a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
+    h = AmgHierarchy(a, AmgSettings(strength_decay=0.5))
+    b = s.source.copy()
+    tol = 1e-10 * np.linalg.norm(b)
+    x = np.zeros(s.n)
+    st = cfd_amd.solve_amg_cg(h.view(), b, x, SolveControl(tol, 1000), precision=prec)
+    oh = oracle.Hierarchy(oracle.Csr(a.n, a.n, a.row_ptr, a.col, a.val), decay=0.5)
+    xo = np.zeros(s.n)
+    so = oh.amg_cg(b, xo, tol, 1000)
+    print(f"{prec.name}: it={st.iterations} oracle it={so.iterations} err={_rel(x, xo):.2e}")
+    assert st.converged and so.converged
+    assert st.true_residual <= 1.05 * tol
+    if prec == Precision.FP64:
+        assert abs(st.iterations - so.iterations) <= 1 and _rel(x, xo) <= 1e-10
+    else:
+        assert _rel(x, xo) <= 1e-6  # both residuals are 1e-10 |b|; the operator's conditioning
+        assert st.iterations <= so.iterations + 6
