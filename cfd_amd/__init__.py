@@ -24,6 +24,7 @@ def _apply_tuning() -> None:
 
     path = Path(__file__).resolve().parent / "tuning.json"
     if os.environ.get("CFDG_NO_TUNING") or not path.exists():
+        _lib.call("cfdg_set_layout_rules", None, 0)
         return
     from .autotune import apply
 

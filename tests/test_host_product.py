@@ -185,3 +185,42 @@ def test_setup_unsorted_input_is_sorted():
     h2 = AmgHierarchy(CsrView(n, len(va), rp, ci2, va2))
     assert h1.n_levels > 1 and h1.level_sizes() == h2.level_sizes()
     assert np.array_equal(h1.matrix(1).val, h2.matrix(1).val)
+
+
+def test_layout_rules_and_built_in_choice():
+    """The built-in layout rule (no device needed) and a tuned rule overriding it."""
+    def layout(rp):
+        lanes, csr = C.c_int(), C.c_int()
+        L.call("cfdg_layout_of", len(rp) - 1, L.iptr(np.asarray(rp, np.int32)), C.byref(lanes), C.byref(csr))
+        return lanes.value, csr.value
+
+    short = np.arange(0, 7 * 1000 + 1, 7)     # 1000 rows of 7 entries: one lane per row, ELL
+    long_ = np.arange(0, 31 * 1000 + 1, 31)   # 31 entries: lanes for about 4 entries each
+    L.call("cfdg_set_layout_rules", None, 0)
+    assert layout(short) == (1, 0)
+    assert layout(long_) == (4, 0)
+    rule = (L.LayoutRule * 1)(L.LayoutRule(500, 2000, 20.0, 40.0, 16, 1))
+    try:
+        L.call("cfdg_set_layout_rules", rule, 1)
+        assert layout(long_) == (16, 1)
+        assert layout(short) == (1, 0)  # outside the rule's class
+    finally:
+        cfd_amd._apply_tuning()  # back to the table installed at import (if any)
+    bad = (L.LayoutRule * 1)(L.LayoutRule(0, 10, 0.0, 1.0, 3, 0))  # lanes not a power of two
+    with pytest.raises(L.InvalidArgument):
+        L.call("cfdg_set_layout_rules", bad, 1)
+    with pytest.raises(L.InvalidArgument):
+        L.call("cfdg_set_mixed_smoother", 1, 0.5)
+
+
+def test_tuning_table_is_well_formed():
+    import json
+
+    path = ROOT / "cfd_amd" / "tuning.json"
+    if not path.exists():
+        pytest.skip("no tuning table")
+    t = json.loads(path.read_text())
+    for r in t["layout_rules"]:
+        assert r["min_rows"] <= r["max_rows"] and r["min_mean"] <= r["max_mean"]
+        assert r["lanes"] in (1, 2, 4, 8, 16, 32, 64) and r["csr"] in (0, 1)
+    assert t["mixed_smoother"]["sweeps"] in (1, 2)
