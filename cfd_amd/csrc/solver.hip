@@ -93,75 +93,6 @@ __global__ void __launch_bounds__(k_block) k_scale_dinv(int n, const double* __r
     }
 }
 
-// Largest eigenvalue of D^-1 A (D = diag A) by 20 power iterations from a fixed pseudo-random
-// start vector (a smooth start has almost no component along the oscillatory top eigenvector of
-// a regular grid and underestimates it); rows in parallel, norms in a fixed order, so the
-// estimate is reproducible.
-double estimate_lambda(const cfdg_csr& a) {
-    const int n = a.n_rows;
-    std::vector<double> dinv(static_cast<std::size_t>(n), 1.0);
-    for (int i = 0; i < n; ++i) {
-        for (int k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
-            if (a.col[k] == i && a.val[k] != 0.0) {
-                dinv[static_cast<std::size_t>(i)] = 1.0 / a.val[k];
-            }
-        }
-    }
-    std::vector<double> x(static_cast<std::size_t>(n));
-    std::vector<double> y(static_cast<std::size_t>(n));
-    for (int i = 0; i < n; ++i) { // splitmix64 of the index, mapped to [-1, 1)
-        unsigned long long z = 0x9E3779B97F4A7C15ULL * static_cast<unsigned long long>(i + 1);
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-        z ^= z >> 31;
-        x[static_cast<std::size_t>(i)] = static_cast<double>(z >> 11) * 0x1.0p-52 - 1.0;
-    }
-    double lambda = 0.0;
-    for (int it = 0; it < 20; ++it) {
-#pragma omp parallel for schedule(static) if (n > 20000)
-        for (int i = 0; i < n; ++i) {
-            double t = 0.0;
-            for (int k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
-                t += a.val[k] * x[static_cast<std::size_t>(a.col[k])];
-            }
-            y[static_cast<std::size_t>(i)] = dinv[static_cast<std::size_t>(i)] * t;
-        }
-        // norms over 64 fixed chunks summed in order (reproducible for any thread count)
-        constexpr int chunks = 64;
-        double cx[chunks];
-        double cy[chunks];
-#pragma omp parallel for schedule(static) if (n > 20000)
-        for (int c = 0; c < chunks; ++c) {
-            const long long lo = static_cast<long long>(n) * c / chunks;
-            const long long hi = static_cast<long long>(n) * (c + 1) / chunks;
-            double sx = 0.0;
-            double sy = 0.0;
-            for (long long i = lo; i < hi; ++i) {
-                sx += x[static_cast<std::size_t>(i)] * x[static_cast<std::size_t>(i)];
-                sy += y[static_cast<std::size_t>(i)] * y[static_cast<std::size_t>(i)];
-            }
-            cx[c] = sx;
-            cy[c] = sy;
-        }
-        double xx = 0.0;
-        double yy = 0.0;
-        for (int c = 0; c < chunks; ++c) {
-            xx += cx[c];
-            yy += cy[c];
-        }
-        lambda = xx > 0.0 ? std::sqrt(yy / xx) : 0.0;
-        if (!(lambda > 0.0) || !std::isfinite(lambda)) {
-            return 2.0;
-        }
-        const double inv = 1.0 / std::sqrt(yy);
-#pragma omp parallel for schedule(static) if (n > 20000)
-        for (int i = 0; i < n; ++i) {
-            x[static_cast<std::size_t>(i)] = y[static_cast<std::size_t>(i)] * inv;
-        }
-    }
-    return lambda;
-}
-
 // Gershgorin bound of the eigenvalues of D^-1 A: max_i sum_j |a_ij| / |a_ii| (2 for the
 // diffusion operators with zero row sums, which is also their largest eigenvalue on fine grids).
 double gershgorin(const cfdg_csr& a) {
@@ -192,9 +123,6 @@ double mixed_weight(double h) {
     return 2.0 / (h + h / ratio);
 }
 
-// Upper end h of a coarse level's spectrum: the power-iteration estimate (4-8% low on the box
-// hierarchies) with a 15% margin.
-double coarse_spectrum_top(const cfdg_csr& a) { return 1.15 * estimate_lambda(a); }
 
 // Level-0 dot products from a finished V-cycle output v[buf] (used when no Jacobi sweep on
 // level 0 could carry them: one-level hierarchies and smoothing-free cycles).
@@ -369,17 +297,29 @@ System::System(const cfdg_ldu_pattern& pt) {
     nnz0_ = pt.nnz;
     nf_ = pt.n_faces;
     std::vector<int> csr_src(static_cast<std::size_t>(pt.nnz), -1);
-    auto place = [&](int pos, int src) {
-        require(pos >= 0 && pos < pt.nnz, "LduPattern: position out of range");
-        csr_src[static_cast<std::size_t>(pos)] = src;
-    };
+    int pos_ok = 1;
+    const int nnz = pt.nnz;
+#pragma omp parallel for schedule(static) reduction(min : pos_ok) if (n_ > 20000)
     for (int i = 0; i < n_; ++i) {
-        place(pt.diag_pos[i], i);
+        const int p = pt.diag_pos[i];
+        if (p >= 0 && p < nnz) {
+            csr_src[static_cast<std::size_t>(p)] = i;
+        } else {
+            pos_ok = 0;
+        }
     }
+#pragma omp parallel for schedule(static) reduction(min : pos_ok) if (nf_ > 20000)
     for (int f = 0; f < nf_; ++f) {
-        place(pt.upper_pos[f], n_ + f);
-        place(pt.lower_pos[f], n_ + nf_ + f);
+        const int pu = pt.upper_pos[f];
+        const int pl = pt.lower_pos[f];
+        if (pu >= 0 && pu < nnz && pl >= 0 && pl < nnz) {
+            csr_src[static_cast<std::size_t>(pu)] = n_ + f;
+            csr_src[static_cast<std::size_t>(pl)] = n_ + nf_ + f;
+        } else {
+            pos_ok = 0;
+        }
     }
+    require(pos_ok != 0, "LduPattern: position out of range");
     const HostEll e = csr_to_ell(n_, n_, pt.row_ptr, pt.col);
     std::vector<int> src(e.src.size());
     const long long n_src = static_cast<long long>(src.size());
@@ -540,6 +480,87 @@ std::size_t estimate_footprint(const cfdg_ldu_pattern& pt, const cfdg_amg_view* 
     return b + b / 64; // allocation granularity
 }
 
+namespace {
+
+// x_i = pseudo-random in [-1, 1) from the index (splitmix64), the power iteration's start.
+__global__ void __launch_bounds__(k_block) k_power_start(int n, double* __restrict__ x) {
+    const int i = blockIdx.x * k_block + threadIdx.x;
+    if (i < n) {
+        unsigned long long z = 0x9E3779B97F4A7C15ULL * static_cast<unsigned long long>(i + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        x[i] = static_cast<double>(z >> 11) * 0x1.0p-52 - 1.0;
+    }
+}
+
+// y = c D^-1 A x (D^-1 = wd / w) and the block partials of y.y; 256 / tpr rows per block.
+__global__ void __launch_bounds__(k_block) k_power_step(EllView m, const double* __restrict__ wd, double c,
+                                                        const double* __restrict__ x, double* __restrict__ y,
+                                                        double* __restrict__ partials) {
+    const int tpr = m.tpr;
+    const int row = static_cast<int>(blockIdx.x) * (k_block / tpr) + static_cast<int>(threadIdx.x) / tpr;
+    const int sub = static_cast<int>(threadIdx.x) & (tpr - 1);
+    const double ax = mat_row<double>(m, row, sub, x);
+    double yy = 0.0;
+    if (sub == 0 && row < m.n_rows) {
+        const double v = c * wd[row] * ax;
+        y[row] = v;
+        yy = v * v;
+    }
+    double in[1] = {yy};
+    double out[1];
+    block_sums<1>(in, out);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = out[0];
+    }
+}
+
+// Largest eigenvalue of D^-1 A for an uploaded square operator (wd = w D^-1): 20 power
+// iterations from the pseudo-random start; the norms are block partials summed on the host in
+// block order, so the estimate is reproducible.
+double device_lambda(const DevEll& a, const DevBuf<double>& wd, double w, hipStream_t s) {
+    const int n = a.n_rows;
+    if (n <= 0) {
+        return 2.0;
+    }
+    const int blocks = ceil_div(static_cast<long long>(n) * a.tpr, k_block);
+    DevBuf<double> x;
+    DevBuf<double> y;
+    DevBuf<double> part;
+    x.resize(static_cast<std::size_t>(n));
+    y.resize(static_cast<std::size_t>(n));
+    part.resize(static_cast<std::size_t>(blocks));
+    std::vector<double> host(static_cast<std::size_t>(blocks));
+    k_power_start<<<ceil_div(n, k_block), k_block, 0, s>>>(n, x.get());
+    hip_check(hipGetLastError(), "k_power_start launch");
+    double c = 1.0 / w;  // first step: the start vector unnormalised
+    double lambda = 0.0;
+    for (int it = 0; it < 21; ++it) {
+        k_power_step<<<blocks, k_block, 0, s>>>(a.view64(), wd.get(), c, x.get(), y.get(), part.get());
+        hip_check(hipGetLastError(), "k_power_step launch");
+        hip_check(hipMemcpyAsync(host.data(), part.get(), host.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+                  "D2H power partials");
+        hip_check(hipStreamSynchronize(s), "power sync");
+        double yy = 0.0;
+        for (double v : host) {
+            yy += v;
+        }
+        const double norm = std::sqrt(yy);
+        if (!(norm > 0.0) || !std::isfinite(norm)) {
+            return 2.0;
+        }
+        if (it > 0) {
+            lambda = norm; // the input had unit norm
+        }
+        c = 1.0 / (w * norm); // next input: y / |y|
+        std::swap(x, y);
+    }
+    return lambda > 0.0 ? lambda : 2.0;
+}
+
+} // namespace
+
 void System::set_amg(const cfdg_amg_view& v) {
     require(v.n_levels >= 1 && v.levels != nullptr,
             "DeviceSolver::set_amg: empty hierarchy");
@@ -575,14 +596,20 @@ void System::set_amg(const cfdg_amg_view& v) {
             const cfdg_csr& a = v.levels[l].a;
             d.a.upload_csr(d.n, d.n, a.row_ptr, a.col, a.val, s);
             const std::vector<double> dinv = inverse_diagonal(a);
-            const double w32 = mixed_weight(coarse_spectrum_top(a));
             std::vector<double> wd(dinv.size());
-            std::vector<float> wd32(dinv.size());
+#pragma omp parallel for schedule(static) if (dinv.size() > 20000)
             for (std::size_t i = 0; i < dinv.size(); ++i) {
                 wd[i] = w * dinv[i];
-                wd32[i] = static_cast<float>(w32 * dinv[i]);
             }
             d.wd64.upload(wd.data(), wd.size(), s);
+            // Upper end of the spectrum of D^-1 A: the power-iteration estimate (4-8% low on the
+            // box hierarchies) with a 15% margin.
+            const double w32 = mixed_weight(1.15 * device_lambda(d.a, d.wd64, w, s));
+            std::vector<float> wd32(dinv.size());
+#pragma omp parallel for schedule(static) if (dinv.size() > 20000)
+            for (std::size_t i = 0; i < dinv.size(); ++i) {
+                wd32[i] = static_cast<float>(w32 * dinv[i]);
+            }
             d.wd32.upload(wd32.data(), wd32.size(), s);
         }
         if (!coarsest) {
