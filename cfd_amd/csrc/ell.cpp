@@ -32,6 +32,43 @@ long long ell_slots(int n_rows, const int* rp, int lanes) {
     return total;
 }
 
+std::vector<int> ell_slice_offsets(int n_rows, const int* rp, int lanes) {
+    const int rows = k_wave / lanes;
+    const int n_slices = (n_rows + rows - 1) / rows;
+    std::vector<int> sp(static_cast<std::size_t>(n_slices) + 1, 0);
+    std::vector<long long> w(static_cast<std::size_t>(n_slices));
+#pragma omp parallel for schedule(static) if (n_slices > 4096)
+    for (int s = 0; s < n_slices; ++s) {
+        w[static_cast<std::size_t>(s)] = static_cast<long long>(slice_rounds(s, n_rows, rp, lanes)) * k_wave;
+    }
+    long long total = 0;
+    for (int s = 0; s < n_slices; ++s) {
+        total += w[static_cast<std::size_t>(s)];
+        if (total > std::numeric_limits<int>::max()) {
+            throw InvalidArgument("matrix too large for 32-bit sliced-ELL offsets");
+        }
+        sp[static_cast<std::size_t>(s) + 1] = static_cast<int>(total);
+    }
+    return sp;
+}
+
+bool short_offsets_fit(int n_rows, int n_cols, const int* rp, const int* ci) {
+    if (n_rows != n_cols || std::getenv("CFDG_NO_SHORT_COLS") != nullptr) {
+        return false;
+    }
+    int fits = 1;
+#pragma omp parallel for schedule(static) reduction(min : fits) if (n_rows > 20000)
+    for (int r = 0; r < n_rows; ++r) {
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+            const int off = ci[k] - r;
+            if (off < -32768 || off > 32767) {
+                fits = 0;
+            }
+        }
+    }
+    return fits != 0;
+}
+
 HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci, int lanes) {
     require(lanes >= 1 && lanes <= k_wave && (lanes & (lanes - 1)) == 0,
             "csr_to_ell: lanes per row must be a power of two <= 64");
@@ -197,7 +234,11 @@ void DevEll::upload_csr(int rows, int cols, const int* rp, const int* ci, const 
     const Layout lay = forced != nullptr ? *forced : choose_layout(rows, rp);
     const int t = lay.lanes;
     if (lay.csr == 0) {
-        upload(csr_to_ell(rows, cols, rp, ci, t), va, true, true, s);
+        if (std::getenv("CFDG_HOST_ELL") != nullptr) { // host conversion (same layout), for checks
+            upload(csr_to_ell(rows, cols, rp, ci, t), va, true, true, s);
+        } else {
+            build_on_device(rows, cols, rp, ci, va, t, s);
+        }
         return;
     }
     n_rows = rows;

@@ -27,6 +27,12 @@ HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci, int lan
 // Stored slots of the sliced-ELL layout with the given lanes per row (no conversion).
 long long ell_slots(int n_rows, const int* rp, int lanes);
 
+// Slice offsets (n_slices + 1) of the sliced-ELL layout with the given lanes per row.
+std::vector<int> ell_slice_offsets(int n_rows, const int* rp, int lanes);
+
+// True when every entry of a square CSR matrix lies within 32767 of its row (16-bit offsets).
+bool short_offsets_fit(int n_rows, int n_cols, const int* rp, const int* ci);
+
 template <class T>
 std::vector<T> ell_values(const HostEll& e, const double* va) {
     std::vector<T> v(e.col.size());
@@ -73,6 +79,10 @@ struct DevEll {
     DevBuf<float> diag32;        // per padded row
 
     void upload(const HostEll& e, const double* va, bool fp64, bool fp32, hipStream_t s);
+    // Sliced ELL with the given lanes built on the device from the CSR arrays (the same layout
+    // as upload(csr_to_ell(...)), FP64 and FP32 values).
+    void build_on_device(int n_rows, int n_cols, const int* rp, const int* ci, const double* va,
+                         int lanes, hipStream_t s);
     // Chooses the layout from the row lengths (choose_layout), or uses the one given.
     void upload_csr(int n_rows, int n_cols, const int* rp, const int* ci, const double* va,
                     hipStream_t s, const Layout* forced = nullptr);

@@ -28,6 +28,54 @@ __global__ void __launch_bounds__(k_block) k_fill(int n, double* __restrict__ x,
     }
 }
 
+// Sliced-ELL fill from CSR, one thread per (slice, lane): slot sp[s] + 64 k + lane holds entry
+// k lanes + lane mod lanes of row s 64 / lanes + lane / lanes, or padding (value 0, a column the
+// row touches anyway), exactly as csr_to_ell. Short: 16-bit offsets column - row. src != nullptr:
+// also the CSR position map (LDU patterns); va may be null (values assembled later).
+template <bool Short>
+__global__ void __launch_bounds__(k_block) k_ell_fill(int n_rows, int n_cols, int n_slices, int lanes,
+                                                      const int* __restrict__ sp, const int* __restrict__ rp,
+                                                      const int* __restrict__ ci, const double* __restrict__ va,
+                                                      const int* __restrict__ csr_src, int* __restrict__ col,
+                                                      short* __restrict__ dcol, double* __restrict__ v64,
+                                                      float* __restrict__ v32, int* __restrict__ src) {
+    const long long t = static_cast<long long>(blockIdx.x) * k_block + threadIdx.x;
+    const int slice = static_cast<int>(t / k_wave);
+    if (slice >= n_slices) {
+        return;
+    }
+    const int lane = static_cast<int>(t % k_wave);
+    const int r = slice * (k_wave / lanes) + lane / lanes;
+    const int sub = lane % lanes;
+    const int start = r < n_rows ? rp[r] : 0;
+    const int len = r < n_rows ? rp[r + 1] - start : 0;
+    int pad = (n_rows == n_cols && n_rows > 0) ? n_rows - 1 : 0;
+    if (r < n_rows) {
+        pad = n_rows == n_cols ? r : (len > 0 ? ci[start + len - 1] : 0);
+    }
+    const int base = sp[slice];
+    const int w = (sp[slice + 1] - base) / k_wave;
+    for (int k = 0; k < w; ++k) {
+        const int j = k * lanes + sub;
+        const long long slot = static_cast<long long>(base) + static_cast<long long>(k) * k_wave + lane;
+        const bool real = j < len;
+        const int c = real ? ci[start + j] : pad;
+        if constexpr (Short) {
+            dcol[slot] = static_cast<short>(c - r);
+        } else {
+            col[slot] = c;
+        }
+        if (va != nullptr) {
+            const double v = real ? va[start + j] : 0.0;
+            v64[slot] = v;
+            v32[slot] = static_cast<float>(v);
+        }
+        if (src != nullptr) {
+            src[slot] = real ? csr_src[start + j] : -1;
+        }
+    }
+}
+
 __device__ __forceinline__ unsigned short float_to_bf16(float f) { // round to nearest even
     unsigned u = __float_as_uint(f);
     u += 0x7FFFu + ((u >> 16) & 1u);
@@ -320,19 +368,61 @@ System::System(const cfdg_ldu_pattern& pt) {
         }
     }
     require(pos_ok != 0, "LduPattern: position out of range");
-    const HostEll e = csr_to_ell(n_, n_, pt.row_ptr, pt.col);
-    std::vector<int> src(e.src.size());
-    const long long n_src = static_cast<long long>(src.size());
-#pragma omp parallel for schedule(static) if (n_src > (1 << 16))
-    for (long long k = 0; k < n_src; ++k) {
-        const int c = e.src[static_cast<std::size_t>(k)];
-        src[static_cast<std::size_t>(k)] = c >= 0 ? csr_src[static_cast<std::size_t>(c)] : -1;
-    }
     hipStream_t s = stream_.get();
-    a0_.upload(e, nullptr, false, false, s);
-    a0_.v64.resize(e.col.size());
-    a0_.v32.resize(e.col.size());
-    a0_src_.upload(src.data(), src.size(), s);
+    if (std::getenv("CFDG_HOST_ELL") != nullptr) { // host conversion (same layout), for checks
+        const HostEll e = csr_to_ell(n_, n_, pt.row_ptr, pt.col);
+        std::vector<int> src(e.src.size());
+        for (std::size_t k = 0; k < src.size(); ++k) {
+            src[k] = e.src[k] >= 0 ? csr_src[static_cast<std::size_t>(e.src[k])] : -1;
+        }
+        a0_.upload(e, nullptr, false, false, s);
+        a0_.v64.resize(e.col.size());
+        a0_.v32.resize(e.col.size());
+        a0_src_.upload(src.data(), src.size(), s);
+        ldu_.resize(static_cast<std::size_t>(n_) + 2 * static_cast<std::size_t>(nf_));
+        init_vectors();
+        return;
+    }
+    // Sliced ELL (one lane per row) and its slot -> LDU index map, built on the device.
+    const std::vector<int> offsets = ell_slice_offsets(n_, pt.row_ptr, 1);
+    const std::size_t slots = static_cast<std::size_t>(offsets.back());
+    a0_.n_rows = n_;
+    a0_.n_cols = n_;
+    a0_.n_slices = static_cast<int>(offsets.size()) - 1;
+    a0_.tpr = 1;
+    a0_.csr = 0;
+    a0_.sp.upload(offsets.data(), offsets.size(), s);
+    {
+        DevBuf<int> rp_d;
+        DevBuf<int> ci_d;
+        DevBuf<int> src_d;
+        rp_d.upload(pt.row_ptr, static_cast<std::size_t>(n_) + 1, s);
+        ci_d.upload(pt.col, static_cast<std::size_t>(pt.nnz), s);
+        src_d.upload(csr_src.data(), csr_src.size(), s);
+        const bool short_cols = short_offsets_fit(n_, n_, pt.row_ptr, pt.col);
+        if (short_cols) {
+            a0_.dcol.resize(slots);
+            a0_.col.resize(0);
+        } else {
+            a0_.col.resize(slots);
+            a0_.dcol.resize(0);
+        }
+        a0_src_.resize(slots);
+        const int grid = ceil_div(static_cast<long long>(a0_.n_slices) * k_wave, k_block);
+        if (short_cols) {
+            k_ell_fill<true><<<grid, k_block, 0, s>>>(n_, n_, a0_.n_slices, 1, a0_.sp.get(), rp_d.get(), ci_d.get(),
+                                                      nullptr, src_d.get(), nullptr, a0_.dcol.get(), nullptr,
+                                                      nullptr, a0_src_.get());
+        } else {
+            k_ell_fill<false><<<grid, k_block, 0, s>>>(n_, n_, a0_.n_slices, 1, a0_.sp.get(), rp_d.get(), ci_d.get(),
+                                                       nullptr, src_d.get(), a0_.col.get(), nullptr, nullptr,
+                                                       nullptr, a0_src_.get());
+        }
+        hip_check(hipGetLastError(), "k_ell_fill launch");
+        hip_check(hipStreamSynchronize(s), "pattern build sync");
+    }
+    a0_.v64.resize(slots);
+    a0_.v32.resize(slots);
     ldu_.resize(static_cast<std::size_t>(n_) + 2 * static_cast<std::size_t>(nf_));
     init_vectors();
 }
@@ -560,6 +650,49 @@ double device_lambda(const DevEll& a, const DevBuf<double>& wd, double w, hipStr
 }
 
 } // namespace
+
+void DevEll::build_on_device(int rows, int cols, const int* rp, const int* ci, const double* va, int lanes,
+                             hipStream_t s) {
+    const std::vector<int> offsets = ell_slice_offsets(rows, rp, lanes);
+    n_rows = rows;
+    n_cols = cols;
+    n_slices = static_cast<int>(offsets.size()) - 1;
+    tpr = lanes;
+    csr = 0;
+    const std::size_t slots = static_cast<std::size_t>(offsets.back());
+    const std::size_t nnz = static_cast<std::size_t>(rp[rows]);
+    sp.upload(offsets.data(), offsets.size(), s);
+    DevBuf<int> rp_d;
+    DevBuf<int> ci_d;
+    DevBuf<double> va_d;
+    rp_d.upload(rp, static_cast<std::size_t>(rows) + 1, s);
+    ci_d.upload(ci, nnz, s);
+    va_d.upload(va, nnz, s);
+    const bool short_cols = short_offsets_fit(rows, cols, rp, ci);
+    if (short_cols) {
+        dcol.resize(slots);
+        col.resize(0);
+    } else {
+        col.resize(slots);
+        dcol.resize(0);
+    }
+    v64.resize(slots);
+    v32.resize(slots);
+    const int grid = ceil_div(static_cast<long long>(n_slices) * k_wave, k_block);
+    if (grid > 0) {
+        if (short_cols) {
+            k_ell_fill<true><<<grid, k_block, 0, s>>>(rows, cols, n_slices, lanes, sp.get(), rp_d.get(), ci_d.get(),
+                                                      va_d.get(), nullptr, nullptr, dcol.get(), v64.get(),
+                                                      v32.get(), nullptr);
+        } else {
+            k_ell_fill<false><<<grid, k_block, 0, s>>>(rows, cols, n_slices, lanes, sp.get(), rp_d.get(), ci_d.get(),
+                                                       va_d.get(), nullptr, col.get(), nullptr, v64.get(),
+                                                       v32.get(), nullptr);
+        }
+        hip_check(hipGetLastError(), "k_ell_fill launch");
+    }
+    hip_check(hipStreamSynchronize(s), "ELL build sync"); // the staging buffers go out of scope
+}
 
 void System::set_amg(const cfdg_amg_view& v) {
     require(v.n_levels >= 1 && v.levels != nullptr,

@@ -199,3 +199,30 @@ def test_precision_stress_sphere_against_oracle(oracle, prec):
     else:
         assert _rel(x, xo) <= 1e-6  # both residuals are 1e-10 |b|; the operator's conditioning
         assert st.iterations <= so.iterations + 6
+
+
+def test_device_built_layouts_match_host_built(monkeypatch):
+    """Hierarchy matrices laid out by the device fill kernel give bit-identical solves to the
+    host conversion (CFDG_HOST_ELL), in FP64 and mixed precision."""
+    s, pat, a = _pressure_csr(20, 0.2, seed=9)
+    view = AmgHierarchy(a, AmgSettings(strength_decay=0.5)).view()
+    tol = 1e-9 * np.linalg.norm(s.source)
+    out = {}
+    for host in (False, True):
+        if host:
+            monkeypatch.setenv("CFDG_HOST_ELL", "1")
+        else:
+            monkeypatch.delenv("CFDG_HOST_ELL", raising=False)
+        dev = cfd_amd.DeviceSolver(pat)
+        dev.assemble(s.diag, s.upper, s.lower)
+        dev.set_amg(view)
+        dev.set_rhs(s.source)
+        for prec in (Precision.FP64, Precision.MIXED_FP32):
+            dev.set_x(None)
+            st = dev.solve_resident(Method.Cg, True, prec, SolveControl(tol, 300))
+            assert st.converged
+            out[(host, prec)] = (st.iterations, dev.get_x())
+        dev.close()
+    for prec in (Precision.FP64, Precision.MIXED_FP32):
+        assert out[(False, prec)][0] == out[(True, prec)][0]
+        assert np.array_equal(out[(False, prec)][1], out[(True, prec)][1])
