@@ -37,6 +37,7 @@ struct Request {
     std::vector<double*> x_out;
     Clock::time_point arrival;
     Clock::time_point ready;    // the current step became ready
+    Clock::time_point resident; // the system and right-hand sides became resident
     Clock::time_point finished;
     std::size_t reserved = 0;   // bytes counted against the budget
     std::unique_ptr<System> sys;
@@ -260,7 +261,7 @@ private:
                 }
                 for (Request* r : starting) {
                     r->state = RQ_ACTIVE;
-                    r->ready = r->arrival;
+                    r->ready = r->resident;
                     try {
                         begin_step(*r);
                     } catch (const InvalidArgument& e) {
@@ -342,6 +343,7 @@ private:
             return; // already failed; its memory goes with the server
         }
         r.state = RQ_READY;
+        r.resident = Clock::now();
         ready_.push_back(&r);
         cv_.notify_all();
     }

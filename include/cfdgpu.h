@@ -252,8 +252,10 @@ typedef struct cfdg_server_request {
     /* Outputs, written before cfdg_server_wait returns (any may be NULL): */
     double* const* x_out;        /* n_steps solutions (entries may be NULL) */
     cfdg_solve_stats* stats_out; /* n_steps */
-    double* step_latency_ms;     /* n_steps: from the step being ready (arrival for the first,
-                                    the previous step's result after) to its own result */
+    double* step_latency_ms;     /* n_steps: from the step being ready (the system resident in
+                                    HBM for the first, the previous step's result after) to its
+                                    own result; waiting for a free slot is included, staging is
+                                    not (cfdg_server_wait's latency runs from submission) */
 } cfdg_server_request;
 
 typedef struct cfdg_server_counters {
