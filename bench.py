@@ -278,7 +278,9 @@ def served(count: int = 24, rate: float = 200.0, seed: int = 20240607) -> dict:
                 "p99_request_ms": round(float(np.percentile(req_lat, 99)), 3),
                 "all_converged": ok, "max_true_rel_residual": worst,
                 "peak_active_solves": c.peak_active_solves,
-                "peak_resident_gb": round(c.peak_resident_bytes / 1e9, 2)}
+                "peak_resident_gb": round(c.peak_resident_bytes / 1e9, 2),
+                "staging_thread_s": round(c.staging_ms / 1e3, 3), "retiring_thread_s": round(c.retiring_ms / 1e3, 3),
+                "engine_busy_s": round(c.engine_busy_ms / 1e3, 3)}
 
     mixed = replay(Precision.MIXED_FP32, 64)
     fp64 = replay(Precision.FP64, 64)

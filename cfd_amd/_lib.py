@@ -136,6 +136,9 @@ class ServerCounters(C.Structure):
         ("active_solves", C.c_int),
         ("peak_active_solves", C.c_int),
         ("waiting", C.c_int),
+        ("staging_ms", C.c_double),
+        ("retiring_ms", C.c_double),
+        ("engine_busy_ms", C.c_double),
     ]
 
 
@@ -184,6 +187,7 @@ _SIGNATURES = {
     "cfdg_backend_name": (ST, [C.c_char_p, C.c_size_t]),
     "cfdg_device_available": (ST, [P_INT]),
     "cfdg_device_description": (ST, [C.c_char_p, C.c_size_t]),
+    "cfdg_release_cached_memory": (ST, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "cfdg_last_error": (ST, [C.c_char_p, C.c_size_t]),
     "cfdg_solve_csr": (ST, [C.POINTER(Csr), P_DBL, P_DBL, C.c_int, C.POINTER(SolveControl),
                             C.POINTER(SolveStats)]),

@@ -51,6 +51,18 @@ extern "C" cfdg_status cfdg_device_available(int* available) {
     });
 }
 
+extern "C" cfdg_status cfdg_release_cached_memory(size_t* freed_bytes, size_t* reserved_bytes) {
+    return guarded([&] {
+        const std::size_t f = pool_release_cached();
+        if (freed_bytes != nullptr) {
+            *freed_bytes = f;
+        }
+        if (reserved_bytes != nullptr) {
+            *reserved_bytes = pool_reserved_bytes();
+        }
+    });
+}
+
 extern "C" cfdg_status cfdg_device_description(char* buf, size_t len) {
     return guarded([&] {
         int count = 0;

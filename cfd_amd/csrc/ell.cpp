@@ -37,7 +37,6 @@ std::vector<int> ell_slice_offsets(int n_rows, const int* rp, int lanes) {
     const int n_slices = (n_rows + rows - 1) / rows;
     std::vector<int> sp(static_cast<std::size_t>(n_slices) + 1, 0);
     std::vector<long long> w(static_cast<std::size_t>(n_slices));
-#pragma omp parallel for schedule(static) if (n_slices > 4096)
     for (int s = 0; s < n_slices; ++s) {
         w[static_cast<std::size_t>(s)] = static_cast<long long>(slice_rounds(s, n_rows, rp, lanes)) * k_wave;
     }
@@ -57,7 +56,6 @@ bool short_offsets_fit(int n_rows, int n_cols, const int* rp, const int* ci) {
         return false;
     }
     int fits = 1;
-#pragma omp parallel for schedule(static) reduction(min : fits) if (n_rows > 20000)
     for (int r = 0; r < n_rows; ++r) {
         for (int k = rp[r]; k < rp[r + 1]; ++k) {
             const int off = ci[k] - r;
@@ -89,7 +87,6 @@ HostEll csr_to_ell(int n_rows, int n_cols, const int* rp, const int* ci, int lan
     }
     e.col.resize(static_cast<std::size_t>(total));
     e.src.resize(static_cast<std::size_t>(total));
-#pragma omp parallel for schedule(static) if (e.n_slices > 256)
     for (int s = 0; s < e.n_slices; ++s) {
         const int base = e.sp[static_cast<std::size_t>(s)];
         const int w = (e.sp[static_cast<std::size_t>(s) + 1] - base) / k_wave;
@@ -158,7 +155,6 @@ void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hi
     if (short_ok) {
         d.resize(e.col.size());
         int fits = 1;
-#pragma omp parallel for schedule(static) reduction(min : fits) if (e.n_slices > 256)
         for (int sl = 0; sl < e.n_slices; ++sl) {
             const int base = e.sp[static_cast<std::size_t>(sl)];
             const int end = e.sp[static_cast<std::size_t>(sl) + 1];
@@ -253,7 +249,6 @@ void DevEll::upload_csr(int rows, int cols, const int* rp, const int* ci, const 
     v64.upload(va, nnz, s);
     std::vector<float> f(nnz);
     const long long m = static_cast<long long>(nnz);
-#pragma omp parallel for schedule(static) if (m > (1 << 16))
     for (long long k = 0; k < m; ++k) {
         f[static_cast<std::size_t>(k)] = static_cast<float>(va[k]);
     }

@@ -37,7 +37,6 @@ template <class T>
 std::vector<T> ell_values(const HostEll& e, const double* va) {
     std::vector<T> v(e.col.size());
     const long long n = static_cast<long long>(v.size());
-#pragma omp parallel for schedule(static) if (n > (1 << 16))
     for (long long k = 0; k < n; ++k) {
         const int s = e.src[static_cast<std::size_t>(k)];
         v[static_cast<std::size_t>(k)] = s >= 0 ? static_cast<T>(va[s]) : T(0);

@@ -28,7 +28,15 @@ inline void hip_check(hipError_t e, const char* what) {
 // Throws NoDevice unless a HIP device is visible.
 void require_device();
 
-// Owning device allocation (hipMalloc / hipFree).
+// Device memory pool (pool.cpp): ranges of large hipMalloc slabs, reused after release without
+// hipFree. pool_release_cached returns the empty slabs to the device (bytes freed).
+void* pool_alloc(std::size_t bytes);
+void pool_free(void* p) noexcept;
+std::size_t pool_release_cached();
+std::size_t pool_reserved_bytes();
+
+// Owning device allocation from the pool; the owner must have no device work pending on it
+// when it is released or resized.
 template <class T>
 class DevBuf {
 public:
@@ -58,7 +66,7 @@ public:
         }
         release();
         if (n > 0) {
-            hip_check(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), "hipMalloc");
+            p_ = static_cast<T*>(pool_alloc(n * sizeof(T)));
         }
         n_ = n;
     }
@@ -81,7 +89,7 @@ public:
 private:
     void release() noexcept {
         if (p_ != nullptr) {
-            (void)hipFree(p_);
+            pool_free(p_);
             p_ = nullptr;
         }
         n_ = 0;

@@ -191,6 +191,9 @@ public:
         c.active_solves = active_;
         c.peak_active_solves = peak_active_;
         c.waiting = static_cast<int>(waiting_.size());
+        c.staging_ms = staging_ms_;
+        c.retiring_ms = retiring_ms_;
+        c.engine_busy_ms = engine_busy_ms_;
         return c;
     }
 
@@ -270,14 +273,18 @@ private:
                         retire(*r);
                     }
                 }
+                double busy = 0.0;
                 if (runner_->busy()) {
+                    const Clock::time_point t0 = Clock::now();
                     done.clear();
                     runner_->step(done);
                     for (const SlotRunner::Done& d : done) {
                         on_done(d);
                     }
+                    busy = ms_between(t0, Clock::now());
                 }
                 std::lock_guard<std::mutex> lk(mu_);
+                engine_busy_ms_ += busy;
                 active_ = runner_->slots() - runner_->free_slots();
                 peak_active_ = std::max(peak_active_, active_);
             }
@@ -307,11 +314,10 @@ private:
         for (std::size_t k = 0; k < steps; ++k) {
             hip_check(hipMemcpy(r.rhs_dev.get() + k * n, r.rhs[k], n * sizeof(double), hipMemcpyHostToDevice),
                       "H2D right-hand side");
-            // |b| over 64 fixed chunks summed in order (reproducible for any thread count)
+            // |b| from 64 fixed chunks summed in order
             constexpr int chunks = 64;
             double part[chunks];
             const double* b = r.rhs[k];
-#pragma omp parallel for schedule(static) if (n > 100000)
             for (int c = 0; c < chunks; ++c) {
                 double t = 0.0;
                 for (std::size_t i = n * c / chunks; i < n * (c + 1) / chunks; ++i) {
@@ -426,11 +432,14 @@ private:
                     cv_.wait(lk);
                 }
             }
+            const Clock::time_point t0 = Clock::now();
             try {
                 if (finishing) {
                     finish(*job);
                 } else {
                     load(*job);
+                    std::lock_guard<std::mutex> lk(mu_);
+                    staging_ms_ += ms_between(t0, Clock::now());
                     continue;
                 }
             } catch (const std::exception& e) {
@@ -439,6 +448,8 @@ private:
                 job->finished = Clock::now();
             }
             release(*job);
+            std::lock_guard<std::mutex> lk(mu_);
+            (finishing ? retiring_ms_ : staging_ms_) += ms_between(t0, Clock::now());
         }
     }
 
@@ -480,6 +491,9 @@ private:
     std::size_t peak_resident_ = 0;
     int active_ = 0;
     int peak_active_ = 0;
+    double staging_ms_ = 0.0;
+    double retiring_ms_ = 0.0;
+    double engine_busy_ms_ = 0.0;
     bool stop_ = false;
     std::string fatal_;
     // Declared after the requests so that the runner (which drains its stream) goes first.

@@ -145,7 +145,6 @@ __global__ void __launch_bounds__(k_block) k_scale_dinv(int n, const double* __r
 // diffusion operators with zero row sums, which is also their largest eigenvalue on fine grids).
 double gershgorin(const cfdg_csr& a) {
     double g = 0.0;
-#pragma omp parallel for schedule(static) reduction(max : g) if (a.n_rows > 20000)
     for (int i = 0; i < a.n_rows; ++i) {
         double d = 0.0;
         double s = 0.0;
@@ -240,7 +239,6 @@ __global__ void k_init_states(const SysDev* __restrict__ sys, int n_sys, const S
 // inverse_diagonal (krylov_openmp.cpp:147-157): 1 / a_ii, 1.0 for a zero or missing diagonal.
 std::vector<double> inverse_diagonal(const cfdg_csr& m) {
     std::vector<double> d(static_cast<std::size_t>(m.n_rows), 1.0);
-#pragma omp parallel for schedule(static) if (m.n_rows > 20000)
     for (int i = 0; i < m.n_rows; ++i) {
         for (int k = m.row_ptr[i]; k < m.row_ptr[i + 1]; ++k) {
             if (m.col[k] == i && m.val[k] != 0.0) {
@@ -258,12 +256,10 @@ void check_csr(const cfdg_csr& m, int rows, int cols, const char* what) {
     require(m.nnz == 0 || (m.col != nullptr && m.val != nullptr), what);
     require(m.row_ptr[0] == 0 && m.row_ptr[rows] == m.nnz, what);
     int ok = 1;
-#pragma omp parallel for schedule(static) reduction(min : ok) if (rows > 20000)
     for (int i = 0; i < rows; ++i) {
         ok = std::min(ok, m.row_ptr[i + 1] >= m.row_ptr[i] ? 1 : 0);
     }
     require(ok != 0, what);
-#pragma omp parallel for schedule(static) reduction(min : ok) if (m.nnz > 100000)
     for (int k = 0; k < m.nnz; ++k) {
         ok = std::min(ok, (m.col[k] >= 0 && m.col[k] < cols) ? 1 : 0);
     }
@@ -336,7 +332,6 @@ System::System(const cfdg_ldu_pattern& pt) {
             "LduPattern: null face positions");
     require(pt.row_ptr[0] == 0 && pt.row_ptr[pt.n] == pt.nnz, "LduPattern: row_ptr mismatch");
     int cols_ok = 1;
-#pragma omp parallel for schedule(static) reduction(min : cols_ok) if (pt.nnz > 100000)
     for (int k = 0; k < pt.nnz; ++k) {
         cols_ok = std::min(cols_ok, (pt.col[k] >= 0 && pt.col[k] < pt.n) ? 1 : 0);
     }
@@ -347,7 +342,6 @@ System::System(const cfdg_ldu_pattern& pt) {
     std::vector<int> csr_src(static_cast<std::size_t>(pt.nnz), -1);
     int pos_ok = 1;
     const int nnz = pt.nnz;
-#pragma omp parallel for schedule(static) reduction(min : pos_ok) if (n_ > 20000)
     for (int i = 0; i < n_; ++i) {
         const int p = pt.diag_pos[i];
         if (p >= 0 && p < nnz) {
@@ -356,7 +350,6 @@ System::System(const cfdg_ldu_pattern& pt) {
             pos_ok = 0;
         }
     }
-#pragma omp parallel for schedule(static) reduction(min : pos_ok) if (nf_ > 20000)
     for (int f = 0; f < nf_; ++f) {
         const int pu = pt.upper_pos[f];
         const int pl = pt.lower_pos[f];
@@ -730,7 +723,6 @@ void System::set_amg(const cfdg_amg_view& v) {
             d.a.upload_csr(d.n, d.n, a.row_ptr, a.col, a.val, s);
             const std::vector<double> dinv = inverse_diagonal(a);
             std::vector<double> wd(dinv.size());
-#pragma omp parallel for schedule(static) if (dinv.size() > 20000)
             for (std::size_t i = 0; i < dinv.size(); ++i) {
                 wd[i] = w * dinv[i];
             }
@@ -739,7 +731,6 @@ void System::set_amg(const cfdg_amg_view& v) {
             // box hierarchies) with a 15% margin.
             const double w32 = mixed_weight(1.15 * device_lambda(d.a, d.wd64, w, s));
             std::vector<float> wd32(dinv.size());
-#pragma omp parallel for schedule(static) if (dinv.size() > 20000)
             for (std::size_t i = 0; i < dinv.size(); ++i) {
                 wd32[i] = static_cast<float>(w32 * dinv[i]);
             }
@@ -1485,7 +1476,10 @@ private:
     hipGraphExec_t iter_exec_[2] = {nullptr, nullptr};
 };
 
-System::~System() = default;
+System::~System() {
+    // Buffers go back to the device pool, where they are reused at once.
+    (void)hipStreamSynchronize(stream_.get());
+}
 
 unsigned long long System::next_uid() {
     static std::atomic<unsigned long long> counter{0};

@@ -43,6 +43,9 @@ class ServerCounters:
     active_solves: int
     peak_active_solves: int
     waiting: int
+    staging_ms: float = 0.0
+    retiring_ms: float = 0.0
+    engine_busy_ms: float = 0.0
 
 
 @dataclass
@@ -142,7 +145,8 @@ class Server:
         L.call("cfdg_server_query", self._h, C.byref(c))
         return ServerCounters(c.submitted, c.completed, c.solves, c.resident_bytes,
                               c.peak_resident_bytes, c.budget_bytes, c.active_solves,
-                              c.peak_active_solves, c.waiting)
+                              c.peak_active_solves, c.waiting, c.staging_ms, c.retiring_ms,
+                              c.engine_busy_ms)
 
     def close(self) -> None:
         """Completes every submitted request, then stops the server."""

@@ -122,6 +122,10 @@ cfdg_status cfdg_device_description(char* buf, size_t len);
 cfdg_status cfdg_last_error(char* buf, size_t len);
 /* Library version as major * 1000 + minor. */
 int cfdg_version(void);
+/* No counterpart in the reference. Device buffers come from a pool of large slabs that keeps
+ * released memory for reuse; this returns the slabs no buffer uses to the device and reports
+ * the bytes freed (may be NULL) and the bytes the pool still holds (may be NULL). */
+cfdg_status cfdg_release_cached_memory(size_t* freed_bytes, size_t* reserved_bytes);
 
 /* ------------------------------------------------------------------------------------------
  * One-shot solves (device)
@@ -268,6 +272,9 @@ typedef struct cfdg_server_counters {
     int active_solves;
     int peak_active_solves;
     int waiting;                 /* submitted requests not yet resident */
+    double staging_ms;           /* summed over staging threads: making requests resident */
+    double retiring_ms;          /* summed over staging threads: returning results */
+    double engine_busy_ms;       /* engine thread time with at least one solve in flight */
 } cfdg_server_counters;
 
 /* ------------------------------------------------------------------------------------------
