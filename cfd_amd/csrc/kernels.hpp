@@ -797,13 +797,20 @@ __global__ void __launch_bounds__(k_block) k_residual(const SysDev* __restrict__
     }
 }
 
-// b_{l+1} = R_l v_l[in]; rows of level l + 1.
+// b_{l+1} = R_l v_l[in]; rows of level l + 1. With x1_out >= 0 the next level's first
+// Jacobi sweep from zero is stored too, v_{l+1}[x1_out] = wd b_{l+1} (what k_jacobi_zero
+// would compute from the stored b), unless level l + 1 is the coarsest.
 template <class T>
-__global__ void __launch_bounds__(k_block) k_restrict(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in) {
+__global__ void __launch_bounds__(k_block) k_restrict(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in,
+                                                      int x1_out) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, r)
     const T y = mat_row<T>(M, row, sub, static_cast<const T*>(L.v[in]));
     if (sub == 0 && row < L.n_next) {
-        gm(static_cast<T*>(S.lv[level + 1].b))[row] = y;
+        const LevelDev& N = S.lv[level + 1];
+        gm(static_cast<T*>(N.b))[row] = y;
+        if (x1_out >= 0 && !N.coarsest) {
+            gm(static_cast<T*>(N.v[x1_out]))[row] = gm(static_cast<const T*>(N.wd))[row] * y;
+        }
     }
 }
 
@@ -910,8 +917,7 @@ enum FinOp : int {
 };
 
 template <int Op>
-__global__ void __launch_bounds__(k_fin_block) k_finalize(const SysDev* __restrict__ sys, const int* __restrict__ ids) {
-    const SysDev& S = sys[ids[blockIdx.x]];
+__device__ __forceinline__ void finalize_step(const SysDev& S) {
     SolveState* st = S.st;
     const int status = st->status;
     if (Op == FIN_TRUE) {
@@ -1085,6 +1091,22 @@ __global__ void __launch_bounds__(k_fin_block) k_finalize(const SysDev* __restri
             }
         }
         return;
+    }
+}
+
+// One workgroup per system (ids: descriptor indices). With status_out, the system's status
+// after the step is also stored at status_out[id]: the last step of an iteration reports every
+// system's state to host-mapped memory this way, with no launch of its own.
+template <int Op>
+__global__ void __launch_bounds__(k_fin_block) k_finalize(const SysDev* __restrict__ sys, const int* __restrict__ ids,
+                                                         int* __restrict__ status_out) {
+    const int id = ids[blockIdx.x];
+    finalize_step<Op>(sys[id]);
+    if (status_out != nullptr) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            status_out[id] = sys[id].st->status;
+        }
     }
 }
 

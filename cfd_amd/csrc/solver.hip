@@ -202,26 +202,6 @@ __global__ void __launch_bounds__(k_block) k_level0_dots(const SysDev* __restric
     }
 }
 
-// Number of systems whose solve is still running (one workgroup).
-__global__ void __launch_bounds__(k_block) k_count_running(const SysDev* __restrict__ sys, int n_sys, int* __restrict__ out) {
-    __shared__ int red[k_slices_per_block];
-    int c = 0;
-    for (int s = threadIdx.x; s < n_sys; s += k_block) {
-        const int st = sys[s].st->status;
-        c += (st == ST_RUNNING || st == ST_CONVERGED_PENDING) ? 1 : 0;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        c += __shfl_xor(c, off, k_wave);
-    }
-    if (threadIdx.x % k_wave == 0) {
-        red[threadIdx.x / k_wave] = c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        out[0] = red[0] + red[1] + red[2] + red[3];
-    }
-}
-
 __global__ void k_gather_states(const SysDev* __restrict__ sys, int n_sys, SolveState* __restrict__ out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n_sys) {
@@ -1097,33 +1077,51 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
     const int L = tb.l_max;
     CFDG_LAUNCH(k_coarse<T>, tb.cn[0], s, d, tb.c[0].get(), 0, fin);
     std::vector<int> cur(static_cast<std::size_t>(L), 0);
+    // Where the first sweep from zero of level l is stored as v[x1_slot(l)] = wd b (-1: not
+    // stored, it is formed per gathered entry); for l >= 1 the restriction that produces b
+    // stores it.
+    auto x1_slot = [&](int l) {
+        const std::size_t li = static_cast<std::size_t>(l);
+        if (sweeps == 1) {
+            return tb.x1[li] != 0 ? -1 : 0;
+        }
+        if (sweeps >= 2) {
+            return tb.a_ell[li] != 0 ? -1 : 2;
+        }
+        return -1;
+    };
     for (int l = 0; l + 1 < L; ++l) {
         const std::size_t li = static_cast<std::size_t>(l);
         const int g = tb.tn[li];                        // row-wise blocks
         const int2* blk = tb.t[li].get();
         const int ga = tb.tmn[0][li];                   // operator blocks
         const int2* ba = tb.tm[0][li].get();
+        const int x1_next = l + 2 < L ? x1_slot(l + 1) : -1;
         int c = 0;
         if (sweeps == 0) {
             CFDG_LAUNCH(k_zero_level<T>, g, s, d, blk, l, 0);
         } else if (sweeps == 1) {
-            // x1 = wd b is formed inside the residual (or stored first, for many long rows) and
+            // x1 = wd b is formed inside the residual (or stored, for many long rows) and
             // inside the prolongation; bitwise the same as a stored sweep
             cur[li] = 0;
             if (tb.x1[li] != 0) {
                 CFDG_LAUNCH(k_residual_x1<T>, ga, s, d, ba, l, 1);
             } else {
-                CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 0);
+                if (l == 0) {
+                    CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 0);
+                }
                 CFDG_LAUNCH(k_residual<T>, ga, s, d, ba, l, 0, 1);
             }
-            CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1);
+            CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1, x1_next);
             CFDG_LAUNCH(k_coarse<T>, tb.cn[li + 1], s, d, tb.c[li + 1].get(), l + 1, fin);
             continue;
         } else {
             if (tb.a_ell[li] != 0) {
                 CFDG_LAUNCH(k_presmooth2<T>, ga, s, d, ba, l, 0);
             } else { // long rows: form x1 = wd b once instead of gathering it per entry
-                CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 2);
+                if (l == 0) {
+                    CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 2);
+                }
                 CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, 2, 0);
             }
             for (int k = 2; k < sweeps; ++k) {
@@ -1134,7 +1132,7 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
         }
         cur[li] = c;
         CFDG_LAUNCH(k_residual<T>, ga, s, d, ba, l, c, 1);
-        CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1);
+        CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1, x1_next);
         CFDG_LAUNCH(k_coarse<T>, tb.cn[li + 1], s, d, tb.c[li + 1].get(), l + 1, fin);
     }
     for (int l = L - 2; l >= 0; --l) {
@@ -1169,8 +1167,8 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
 }
 
 template <int Op>
-void fin(const SysDev* d, const Tables& tb, hipStream_t s) {
-    k_finalize<Op><<<tb.n_sys, k_fin_block, 0, s>>>(d, tb.ids.get());
+void fin(const SysDev* d, const Tables& tb, hipStream_t s, int* status_out = nullptr) {
+    k_finalize<Op><<<tb.n_sys, k_fin_block, 0, s>>>(d, tb.ids.get(), status_out);
 }
 
 enum VecId { V_X = 0, V_R, V_P, V_Q, V_Z, V_R0, V_V, V_S, V_T, V_PH, V_SH };
@@ -1211,7 +1209,10 @@ void enqueue_init(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t
 }
 
 // One iteration; parity = iteration number mod 2 selects the search-direction buffers.
-void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t s, int parity) {
+// One iteration for every listed system; its last step stores each system's status at
+// status_out[id].
+void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t s, int parity,
+                       int* status_out) {
     const int g0 = tb.tn[0];
     const int2* b0 = tb.t[0].get();
     if (pl.method == CFDG_METHOD_BICGSTAB) {
@@ -1225,7 +1226,7 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
         CFDG_LAUNCH(k_spmv_dot<true>, g0, s, d, b0, V_SH, V_T, V_T, SLOT_TT, V_S, SLOT_TS);
         fin<FIN_BICG_OMEGA>(d, tb, s);
         CFDG_LAUNCH(k_bicg_xr, g0, s, d, b0);
-        fin<FIN_BICG_RES>(d, tb, s);
+        fin<FIN_BICG_RES>(d, tb, s, status_out);
         return;
     }
     const int fz = final_index(pl.sweeps);
@@ -1241,19 +1242,19 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
     fin<FIN_ALPHA>(d, tb, s);
     if (!pl.amg) {
         CFDG_LAUNCH(k_update_xr<1>, g0, s, d, b0, parity);
-        fin<FIN_CG_RES_BETA>(d, tb, s);
+        fin<FIN_CG_RES_BETA>(d, tb, s, status_out);
         return;
     }
     if (pl.fp32) {
         CFDG_LAUNCH(k_update_xr<2>, g0, s, d, b0, parity);
         fin<FIN_RES>(d, tb, s);
         enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s);
-        fin<FIN_BETA_FLEX>(d, tb, s);
+        fin<FIN_BETA_FLEX>(d, tb, s, status_out);
     } else {
         CFDG_LAUNCH(k_update_xr<0>, g0, s, d, b0, parity);
         fin<FIN_RES>(d, tb, s);
         enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s);
-        fin<FIN_BETA>(d, tb, s);
+        fin<FIN_BETA>(d, tb, s, status_out);
     }
 }
 
@@ -1261,13 +1262,14 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
 
 // Everything a set of systems needs to be solved together, built once and reused: launch
 // tables, device descriptors, and (unless kernels are being probed) HIP graphs of the solver's
-// start-up sequence and of one iteration. Iteration k of the graph pipeline writes the number
-// of still-running systems into pinned host memory slot k mod 2 from the device; the host waits
-// for the previous slot while the next iteration is already queued.
+// start-up sequence and of one iteration. Iteration k of the graph pipeline writes every
+// system's status into pinned host memory slot k mod 2 from the device; the host waits for the
+// previous slot while the next iteration is already queued.
 class Runner {
 public:
     Runner(const std::vector<System*>& systems, const BatchOptions& opt, hipStream_t s)
-        : sys_(systems), s_(s), running_(2), init_host_(systems.size()), out_host_(systems.size()) {
+        : sys_(systems), s_(s), running_(2 * systems.size()), init_host_(systems.size()),
+          out_host_(systems.size()) {
         require(!sys_.empty(), "solve_batch: no systems");
         const bool amg = opt.use_amg && opt.method == CFDG_METHOD_CG;
         const bool fp32 = amg && opt.precision != CFDG_PREC_FP64;
@@ -1356,7 +1358,9 @@ public:
         if (graphs && init_exec_ == nullptr) {
             capture_graphs();
         }
-        running_[0] = running_[1] = 1;
+        for (std::size_t i = 0; i < running_.size(); ++i) {
+            running_[i] = ST_RUNNING;
+        }
         hip_check(hipEventRecord(t0_.get(), s_), "event record");
         hip_check(hipMemcpyAsync(d_init_.get(), init_host_.get(), sys_.size() * sizeof(SolveState),
                                  hipMemcpyHostToDevice, s_),
@@ -1375,15 +1379,14 @@ public:
             if (graphs) {
                 hip_check(hipGraphLaunch(iter_exec_[slot], s_), "graph launch");
             } else {
-                enqueue_iteration(pl_, d, tb_, s_, slot);
-                k_count_running<<<1, k_block, 0, s_>>>(d, n_sys, running_dev_ + slot);
+                enqueue_iteration(pl_, d, tb_, s_, slot, running_dev_ + slot * n_sys);
             }
             hip_check(hipEventRecord(ev_[slot].get(), s_), "event record");
             hip_check(hipGetLastError(), "iteration launch");
             if (k >= 2) {
                 const int prev = (k - 1) & 1;
                 hip_check(hipEventSynchronize(ev_[prev].get()), "event sync");
-                if (running_[static_cast<std::size_t>(prev)] == 0) {
+                if (!any_running(prev)) {
                     break;
                 }
             }
@@ -1431,6 +1434,18 @@ public:
     }
 
 private:
+    // Statuses that iteration slot `slot` reported (host-mapped): is any system still going?
+    [[nodiscard]] bool any_running(int slot) const {
+        const std::size_t n = sys_.size();
+        for (std::size_t i = 0; i < n; ++i) {
+            const int st = running_[static_cast<std::size_t>(slot) * n + i];
+            if (st == ST_RUNNING || st == ST_CONVERGED_PENDING) {
+                return true;
+            }
+        }
+        return false;
+    }
+
     void capture_graphs() {
         const SysDev* d = desc_.get();
         const int n_sys = static_cast<int>(sys_.size());
@@ -1447,8 +1462,7 @@ private:
         init_exec_ = capture([&] { enqueue_init(pl_, d, tb_, s_); });
         for (int slot = 0; slot < 2; ++slot) {
             iter_exec_[slot] = capture([&] {
-                enqueue_iteration(pl_, d, tb_, s_, slot);
-                k_count_running<<<1, k_block, 0, s_>>>(d, n_sys, running_dev_ + slot);
+                enqueue_iteration(pl_, d, tb_, s_, slot, running_dev_ + slot * n_sys);
             });
         }
     }
@@ -1462,7 +1476,7 @@ private:
     DevBuf<SysDev> desc_;
     DevBuf<SolveState> d_init_;
     DevBuf<SolveState> d_out_;
-    PinnedBuf<int> running_;
+    PinnedBuf<int> running_;     // per iteration slot (2) and system: status after the iteration
     int* running_dev_ = nullptr;
     PinnedBuf<SolveState> init_host_;
     PinnedBuf<SolveState> out_host_;
@@ -1487,16 +1501,6 @@ unsigned long long System::next_uid() {
 }
 
 namespace {
-
-// Status of each listed system, written to pinned host memory at out[slot].
-__global__ void k_slot_status(const SysDev* __restrict__ sys, const int* __restrict__ ids, int n,
-                              int* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const int id = ids[i];
-        out[id] = sys[id].st->status;
-    }
-}
 
 enum SlotPhase : int { SLOT_FREE = 0, SLOT_RUNNING = 1, SLOT_FINISHING = 2 };
 
@@ -1645,9 +1649,7 @@ void SlotRunner::step(std::vector<Done>& done) {
         }
         const Tables& tb = m.merged[m.cur];
         const int parity = static_cast<int>(m.g & 1);
-        enqueue_iteration(m.pl, d, tb, m.s, parity);
-        k_slot_status<<<ceil_div(tb.n_sys, k_block), k_block, 0, m.s>>>(
-            d, tb.ids.get(), tb.n_sys, m.status_dev + static_cast<std::size_t>(parity) * m.k);
+        enqueue_iteration(m.pl, d, tb, m.s, parity, m.status_dev + static_cast<std::size_t>(parity) * m.k);
         hip_check(hipEventRecord(m.ev[parity].get(), m.s), "event record");
         hip_check(hipGetLastError(), "slot iteration launch");
         m.covered[parity].clear();
