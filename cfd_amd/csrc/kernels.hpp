@@ -713,6 +713,98 @@ __global__ void __launch_bounds__(k_block) k_presmooth2(const SysDev* __restrict
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// V-cycle operations on one row (or one lane of a row's group), used by the level kernels below.
+// ------------------------------------------------------------------------------------------
+
+// v[out] = wd b.
+template <class T>
+__device__ __forceinline__ void op_jacobi_zero(const LevelDev& L, int row, int out) {
+    gm(static_cast<T*>(L.v[out]))[row] = gm(static_cast<const T*>(L.wd))[row] * gm(static_cast<const T*>(L.b))[row];
+}
+
+// v[out] = v[in] + wd (b - A v[in]); returns the new entry (0 for rows past the end / sub > 0).
+template <class T>
+__device__ __forceinline__ T op_jacobi(const LevelDev& L, const EllView& M, int row, int sub, int in, int out) {
+    const T* x = static_cast<const T*>(L.v[in]);
+    const auto b = gm(static_cast<const T*>(L.b));
+    const auto wd = gm(static_cast<const T*>(L.wd));
+    const T ax = mat_row<T>(M, row, sub, x);
+    T z = T(0);
+    if (sub == 0 && row < L.n) {
+        z = x[row] + wd[row] * (b[row] - ax);
+        gm(static_cast<T*>(L.v[out]))[row] = z;
+    }
+    return z;
+}
+
+// v[out] = b - A v[in].
+template <class T>
+__device__ __forceinline__ void op_residual(const LevelDev& L, const EllView& M, int row, int sub, int in, int out) {
+    const T ax = mat_row<T>(M, row, sub, static_cast<const T*>(L.v[in]));
+    if (sub == 0 && row < L.n) {
+        gm(static_cast<T*>(L.v[out]))[row] = gm(static_cast<const T*>(L.b))[row] - ax;
+    }
+}
+
+// v[out] = b - A x1 with x1 = wd b formed per gathered column.
+template <class T>
+__device__ __forceinline__ void op_residual_x1(const LevelDev& L, const EllView& M, int row, int sub, int out) {
+    const auto b = gm(static_cast<const T*>(L.b));
+    const auto wd = gm(static_cast<const T*>(L.wd));
+    const T ax = mat_row_x1<T>(M, row, sub, wd, b);
+    if (sub == 0 && row < L.n) {
+        gm(static_cast<T*>(L.v[out]))[row] = b[row] - ax;
+    }
+}
+
+// b_{l+1} = R_l v_l[in] for a row of level l + 1; with x1_out >= 0 also v_{l+1}[x1_out] = wd b
+// unless level l + 1 is the coarsest.
+template <class T>
+__device__ __forceinline__ void op_restrict(const LevelDev& L, const LevelDev& N, const EllView& M, int row, int sub,
+                                            int in, int x1_out) {
+    const T y = mat_row<T>(M, row, sub, static_cast<const T*>(L.v[in]));
+    if (sub == 0 && row < L.n_next) {
+        gm(static_cast<T*>(N.b))[row] = y;
+        if (x1_out >= 0 && !N.coarsest) {
+            gm(static_cast<T*>(N.v[x1_out]))[row] = gm(static_cast<const T*>(N.wd))[row] * y;
+        }
+    }
+}
+
+// v_l[io] += P_l v_{l+1}[coarse_buf]; X1: v_l[io] = wd b + P_l v_{l+1}[coarse_buf] (the level's
+// single pre-sweep from zero was not stored).
+template <class T, bool X1>
+__device__ __forceinline__ void op_prolong(const LevelDev& L, const LevelDev& N, const EllView& M, int row, int sub,
+                                           int io, int coarse_buf) {
+    const T t = mat_row<T>(M, row, sub, static_cast<const T*>(N.v[coarse_buf]));
+    if (sub == 0 && row < L.n) {
+        const auto x = gm(static_cast<T*>(L.v[io]));
+        if constexpr (X1) {
+            x[row] = gm(static_cast<const T*>(L.wd))[row] * gm(static_cast<const T*>(L.b))[row] + t;
+        } else {
+            x[row] = x[row] + t;
+        }
+    }
+}
+
+// Coarsest level, row i: v[out]_i = sum_j Cinv_ij b_j in FP64 by one wavefront (lane-strided
+// partial sums, then the wavefront butterfly).
+template <class T>
+__device__ __forceinline__ void op_coarse_row(const LevelDev& L, int i, int lane, int out) {
+    const int n = L.n;
+    const double* m = L.cinv + static_cast<long long>(i) * n;
+    const auto b = gm(static_cast<const T*>(L.b));
+    double s = 0.0;
+    for (int j = lane; j < n; j += k_wave) {
+        s += m[j] * static_cast<double>(b[j]);
+    }
+    s = wave_sum(s);
+    if (lane == 0) {
+        gm(static_cast<T*>(L.v[out]))[i] = static_cast<T>(s);
+    }
+}
+
 // One Jacobi sweep from x = 0: v[out] = wd b (row-wise, 256 rows per workgroup).
 template <class T>
 __global__ void __launch_bounds__(k_block) k_jacobi_zero(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
@@ -724,9 +816,7 @@ __global__ void __launch_bounds__(k_block) k_jacobi_zero(const SysDev* __restric
     if (L.coarsest || row >= L.n) {
         return;
     }
-    const auto b = gm(static_cast<const T*>(L.b));
-    const auto wd = gm(static_cast<const T*>(L.wd));
-    gm(static_cast<T*>(L.v[out]))[row] = wd[row] * b[row];
+    op_jacobi_zero<T>(L, row, out);
 }
 
 // v[out] = 0 (smoothing-free V-cycles; row-wise).
@@ -749,17 +839,12 @@ __global__ void __launch_bounds__(k_block) k_zero_level(const SysDev* __restrict
 template <class T, int Dots>
 __global__ void __launch_bounds__(k_block) k_jacobi(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in, int out) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
+    const T z = op_jacobi<T>(L, M, row, sub, in, out);
     double d1 = 0.0;
     double d2 = 0.0;
-    const T* x = static_cast<const T*>(L.v[in]);
-    const auto b = gm(static_cast<const T*>(L.b));
-    const auto wd = gm(static_cast<const T*>(L.wd));
-    const T ax = mat_row<T>(M, row, sub, x);
-    if (sub == 0 && row < L.n) {
-        const T z = x[row] + wd[row] * (b[row] - ax);
-        gm(static_cast<T*>(L.v[out]))[row] = z;
+    if (Dots != 0 && sub == 0 && row < L.n) {
         if (Dots == 1) {
-            d1 = static_cast<double>(b[row]) * static_cast<double>(z);
+            d1 = static_cast<double>(gm(static_cast<const T*>(L.b))[row]) * static_cast<double>(z);
         }
         if (Dots == 2) {
             const double zd = static_cast<double>(z);
@@ -790,11 +875,7 @@ __global__ void __launch_bounds__(k_block) k_jacobi(const SysDev* __restrict__ s
 template <class T>
 __global__ void __launch_bounds__(k_block) k_residual(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in, int out) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
-    const T* x = static_cast<const T*>(L.v[in]);
-    const T ax = mat_row<T>(M, row, sub, x);
-    if (sub == 0 && row < L.n) {
-        gm(static_cast<T*>(L.v[out]))[row] = gm(static_cast<const T*>(L.b))[row] - ax;
-    }
+    op_residual<T>(L, M, row, sub, in, out);
 }
 
 // b_{l+1} = R_l v_l[in]; rows of level l + 1. With x1_out >= 0 the next level's first
@@ -804,14 +885,7 @@ template <class T>
 __global__ void __launch_bounds__(k_block) k_restrict(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in,
                                                       int x1_out) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, r)
-    const T y = mat_row<T>(M, row, sub, static_cast<const T*>(L.v[in]));
-    if (sub == 0 && row < L.n_next) {
-        const LevelDev& N = S.lv[level + 1];
-        gm(static_cast<T*>(N.b))[row] = y;
-        if (x1_out >= 0 && !N.coarsest) {
-            gm(static_cast<T*>(N.v[x1_out]))[row] = gm(static_cast<const T*>(N.wd))[row] * y;
-        }
-    }
+    op_restrict<T>(L, S.lv[level + 1], M, row, sub, in, x1_out);
 }
 
 // v_l[io] += P_l v_{l+1}[coarse_buf]. X1: the level's single pre-sweep from zero was not
@@ -820,15 +894,7 @@ template <class T, bool X1 = false>
 __global__ void __launch_bounds__(k_block) k_prolong(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int io,
                           int coarse_buf) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, p)
-    const T t = mat_row<T>(M, row, sub, static_cast<const T*>(S.lv[level + 1].v[coarse_buf]));
-    if (sub == 0 && row < L.n) {
-        const auto x = gm(static_cast<T*>(L.v[io]));
-        if constexpr (X1) {
-            x[row] = gm(static_cast<const T*>(L.wd))[row] * gm(static_cast<const T*>(L.b))[row] + t;
-        } else {
-            x[row] = x[row] + t;
-        }
-    }
+    op_prolong<T, X1>(L, S.lv[level + 1], M, row, sub, io, coarse_buf);
 }
 
 // v[out] = b - A x1 with x1 = wd b, the single Jacobi sweep from zero, formed per gathered
@@ -836,12 +902,7 @@ __global__ void __launch_bounds__(k_block) k_prolong(const SysDev* __restrict__ 
 template <class T>
 __global__ void __launch_bounds__(k_block) k_residual_x1(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
-    const auto b = gm(static_cast<const T*>(L.b));
-    const auto wd = gm(static_cast<const T*>(L.wd));
-    const T ax = mat_row_x1<T>(M, row, sub, wd, b);
-    if (sub == 0 && row < L.n) {
-        gm(static_cast<T*>(L.v[out]))[row] = b[row] - ax;
-    }
+    op_residual_x1<T>(L, M, row, sub, out);
 }
 
 // Coarsest level: v[out] = C^-1 b with the dense FP64 inverse, one wavefront per row; blocks
@@ -860,17 +921,7 @@ __global__ void __launch_bounds__(k_block) k_coarse(const SysDev* __restrict__ s
     if (!L.coarsest || i >= L.n) {
         return;
     }
-    const int n = L.n;
-    const double* m = L.cinv + static_cast<long long>(i) * n;
-    const auto b = gm(static_cast<const T*>(L.b));
-    double s = 0.0;
-    for (int j = lane; j < n; j += k_wave) {
-        s += m[j] * static_cast<double>(b[j]);
-    }
-    s = wave_sum(s);
-    if (lane == 0) {
-        gm(static_cast<T*>(L.v[out]))[i] = static_cast<T>(s);
-    }
+    op_coarse_row<T>(L, i, lane, out);
 }
 
 // ------------------------------------------------------------------------------------------
