@@ -125,6 +125,26 @@ class ServerRequest(C.Structure):
     ]
 
 
+class FvMesh(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int),
+        ("n_internal_faces", C.c_int),
+        ("n_faces", C.c_int),
+        ("owner", C.POINTER(C.c_int)),
+        ("neighbour", C.POINTER(C.c_int)),
+        ("cell_offsets", C.POINTER(C.c_int)),
+        ("cell_faces", C.POINTER(C.c_int)),
+        ("weight", C.POINTER(C.c_double)),
+        ("sf", C.POINTER(C.c_double)),
+        ("face_centre", C.POINTER(C.c_double)),
+        ("cell_centre", C.POINTER(C.c_double)),
+        ("volume", C.POINTER(C.c_double)),
+        ("correction", C.POINTER(C.c_double)),
+        ("lsq_d", C.POINTER(C.c_double)),
+        ("lsq_inverse", C.POINTER(C.c_double)),
+    ]
+
+
 class ServerCounters(C.Structure):
     _fields_ = [
         ("submitted", C.c_ulonglong),
@@ -238,6 +258,16 @@ _SIGNATURES = {
     "cfdg_amg_coarse_inverse": (ST, [VP, P_DBL]),
     "cfdg_amg_view_of": (ST, [VP, C.POINTER(AmgLevelView), C.POINTER(AmgView)]),
     "cfdg_amg_setup_ms": (ST, [VP, P_DBL]),
+    "cfdg_fv_create": (ST, [C.POINTER(FvMesh), C.POINTER(VP)]),
+    "cfdg_fv_destroy": (ST, [VP]),
+    "cfdg_fv_gradient": (ST, [VP, P_DBL, P_DBL, C.c_int, P_DBL]),
+    "cfdg_fv_vector_gradient": (ST, [VP, P_DBL, P_DBL, C.c_int, P_DBL]),
+    "cfdg_fv_scalar_corrections": (ST, [VP, P_DBL, P_DBL, C.c_int, P_DBL, P_DBL, C.c_int, P_DBL]),
+    "cfdg_fv_momentum_corrections": (ST, [VP, P_DBL, P_DBL, C.c_int, C.c_int, C.c_int, P_DBL, C.c_double,
+                                          C.c_int, P_DBL]),
+    "cfdg_fv_on_device": (ST, [VP, P_INT]),
+    "cfdg_fv_timing": (ST, [VP, C.c_int]),
+    "cfdg_fv_last_kernel_ms": (ST, [VP, P_DBL]),
     "cfdg_box_sizes": (ST, [C.POINTER(BoxSpec), P_INT, P_INT]),
     "cfdg_box_build": (ST, [C.POINTER(BoxSpec), P_INT, P_INT, P_DBL, P_DBL, P_DBL, P_DBL]),
     "cfdg_uniform": (ST, [C.c_uint64, C.c_int64, C.c_double, C.c_double, P_DBL]),

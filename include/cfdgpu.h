@@ -278,6 +278,70 @@ typedef struct cfdg_server_counters {
 } cfdg_server_counters;
 
 /* ------------------------------------------------------------------------------------------
+ * Finite-volume cell kernels (replaces gpu::DeviceFv, fv_device.hpp:14-71): cell gradients
+ * (Green-Gauss, weighted least squares, skewness-corrected Gauss), the Barth-Jespersen /
+ * Venkatakrishnan limiter and the explicit face corrections of the scalar and momentum
+ * equations. The mesh is copied to the device once; each call moves only the fields. Every cell
+ * gathers over its faces in ascending face order, so results equal the reference's serial face
+ * loops (no atomics, deterministic).
+ * ---------------------------------------------------------------------------------------- */
+
+/* FvMeshView (fv_device.hpp:16-32). Faces [0, n_internal_faces) are internal, boundary faces
+ * follow; vectors are 3 doubles per entry. */
+typedef struct cfdg_fv_mesh {
+    int n_cells;
+    int n_internal_faces;
+    int n_faces;
+    const int* owner;          /* n_faces */
+    const int* neighbour;      /* n_internal_faces */
+    const int* cell_offsets;   /* n_cells + 1: faces of each cell (CSR) ... */
+    const int* cell_faces;     /* ... in ascending face order */
+    const double* weight;      /* n_internal_faces: phi_f = w phi_P + (1 - w) phi_N */
+    const double* sf;          /* 3 n_faces: face area vectors */
+    const double* face_centre; /* 3 n_faces */
+    const double* cell_centre; /* 3 n_cells */
+    const double* volume;      /* n_cells */
+    const double* correction;  /* 3 n_faces: non-orthogonal correction vectors */
+    const double* lsq_d;       /* 3 n_faces: d / |d|^2 (d = C_N - C_P or C_f - C_P) */
+    const double* lsq_inverse; /* 9 n_cells: (sum d d^T / |d|^2)^-1, column-major */
+} cfdg_fv_mesh;
+
+typedef struct cfdg_fv cfdg_fv;
+
+/* Replaces DeviceFv::DeviceFv (fv_device.hpp:36, fv_device.cpp:24-59): checks the sizes,
+ * copies the mesh to the device. Errors as the reference: CFDG_ERR_INVALID_ARGUMENT for invalid
+ * sizes or a missing array. */
+cfdg_status cfdg_fv_create(const cfdg_fv_mesh* mesh, cfdg_fv** out);
+cfdg_status cfdg_fv_destroy(cfdg_fv* fv);
+/* Replaces DeviceFv::gradient (fv_device.hpp:43-45): scheme 0 Green-Gauss, 1 least squares,
+ * 2 skewness-corrected Gauss; phi n_cells, bv n_boundary, grad 3 n_cells. */
+cfdg_status cfdg_fv_gradient(cfdg_fv* fv, const double* phi, const double* bv, int scheme,
+                             double* grad);
+/* Replaces DeviceFv::vector_gradient (fv_device.hpp:46-50): u 3 n_cells, ub 3 n_boundary, grad
+ * 9 n_cells, column-major 3x3 per cell with grad(j, i) = du_i / dx_j. u, ub and the gradient
+ * stay on the device for cfdg_fv_momentum_corrections. */
+cfdg_status cfdg_fv_vector_gradient(cfdg_fv* fv, const double* u, const double* ub, int scheme,
+                                    double* grad);
+/* Replaces DeviceFv::scalar_corrections (fv_device.hpp:51-56): source (n_cells) += explicit
+ * internal-face fluxes; flux and gamma per internal face. */
+cfdg_status cfdg_fv_scalar_corrections(cfdg_fv* fv, const double* phi, const double* bv, int scheme,
+                                       const double* flux, const double* gamma, int linear_upwind,
+                                       double* source);
+/* Replaces DeviceFv::momentum_corrections (fv_device.hpp:57-66) with the field and gradient of
+ * the last cfdg_fv_vector_gradient (CFDG_ERR_INVALID_ARGUMENT before one, as the reference's
+ * logic_error); eps2: 3 values, read only when venkat. source 3 n_cells. */
+cfdg_status cfdg_fv_momentum_corrections(cfdg_fv* fv, const double* flux, const double* nu_face,
+                                         int linear_upwind, int limited, int venkat,
+                                         const double* eps2, double central, int transpose_term,
+                                         double* source);
+/* Replaces DeviceFv::on_device (fv_device.hpp:68): 1. */
+cfdg_status cfdg_fv_on_device(const cfdg_fv* fv, int* on_device);
+/* No counterpart in the reference: device time (ms) of the kernels of the last call on this
+ * object when timing is on (HIP events around each launch; off by default). */
+cfdg_status cfdg_fv_timing(cfdg_fv* fv, int enable);
+cfdg_status cfdg_fv_last_kernel_ms(const cfdg_fv* fv, double* ms);
+
+/* ------------------------------------------------------------------------------------------
  * Layout tuning (no counterpart in the reference). Coarse operators and transfer matrices are
  * stored as sliced ELL or CSR with `lanes` lanes per row. The library's built-in rule picks
  * them from the mean row length; a tuned table (written by python -m cfd_amd.autotune from

@@ -27,6 +27,13 @@ class OrcStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double)]
 
 
+class OrcFvMesh(C.Structure):
+    _fields_ = [("nc", C.c_int), ("nif", C.c_int), ("nf", C.c_int),
+                ("own", P_INT), ("nei", P_INT), ("start", P_INT), ("faces", P_INT),
+                ("w", P_DBL), ("sf", P_DBL), ("cf", P_DBL), ("cc", P_DBL), ("vol", P_DBL),
+                ("kv", P_DBL), ("lsq_d", P_DBL), ("lsq", P_DBL)]
+
+
 _orc = None
 _ref = None
 
@@ -81,6 +88,14 @@ def orc() -> C.CDLL:
             "orc_amg_cg_h": (OrcStats, [C.c_void_p, P_INT, P_INT, P_DBL, C.c_int, C.c_double, P_DBL,
                                         P_DBL, C.c_double, C.c_int]),
             "orc_sgs_pcg": (OrcStats, [C.c_void_p, P_DBL, P_DBL, C.c_double, C.c_int]),
+            "orc_fv_gradient": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, C.c_int, P_DBL]),
+            "orc_fv_vector_gradient": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, C.c_int, P_DBL]),
+            "orc_fv_scalar_corrections": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, C.c_int, P_DBL, P_DBL,
+                                                 C.c_int, P_DBL]),
+            "orc_fv_limiter": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, P_DBL, C.c_int, P_DBL, P_DBL]),
+            "orc_fv_momentum_corrections": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, P_DBL, P_DBL, P_DBL,
+                                                   C.c_int, C.c_int, C.c_int, P_DBL, C.c_double, C.c_int,
+                                                   P_DBL]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(h, name)
@@ -116,6 +131,14 @@ def ref() -> C.CDLL:
             "ref_device_solver_set_amg": (None, [C.c_void_p] + amg_args),
             "ref_device_solver_solve": (None, [C.c_void_p, P_DBL, P_DBL, P_DBL, P_DBL, P_DBL, C.c_int,
                                                C.c_int, C.c_double, C.c_int] + stats),
+            "ref_fv_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, P_INT, P_INT, P_INT, P_INT] + [P_DBL] * 8),
+            "ref_fv_destroy": (None, [C.c_void_p]),
+            "ref_fv_gradient": (None, [C.c_void_p, P_DBL, P_DBL, C.c_int, P_DBL]),
+            "ref_fv_vector_gradient": (None, [C.c_void_p, P_DBL, P_DBL, C.c_int, P_DBL]),
+            "ref_fv_scalar_corrections": (None, [C.c_void_p, P_DBL, P_DBL, C.c_int, P_DBL, P_DBL, C.c_int,
+                                                 P_DBL]),
+            "ref_fv_momentum_corrections": (None, [C.c_void_p, P_DBL, P_DBL, C.c_int, C.c_int, C.c_int, P_DBL,
+                                                   C.c_double, C.c_int, P_DBL]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(h, name)
@@ -353,3 +376,112 @@ def hierarchy_lists(h: Hierarchy):
     p = [h.matrix(lvl, 1) if lvl + 1 < nl else None for lvl in range(nl)]
     r = [h.matrix(lvl, 2) if lvl + 1 < nl else None for lvl in range(nl)]
     return a, p, r
+
+
+# ---- Finite-volume cell kernels (gpu::DeviceFv) ----------------------------------------------
+
+class FvOracle:
+    """The oracle's restatement of gpu::DeviceFv for one mesh (any object with fv_box's fields);
+    momentum_corrections takes the field and its gradient explicitly."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self._keep = [_i(mesh.owner), _i(mesh.neighbour), _i(mesh.cell_offsets), _i(mesh.cell_faces)] + \
+            [_d(a) for a in (mesh.weight, mesh.sf, mesh.face_centre, mesh.cell_centre, mesh.volume,
+                             mesh.correction, mesh.lsq_d, mesh.lsq_inverse)]
+        k = self._keep
+        self.m = OrcFvMesh(mesh.n_cells, mesh.n_internal_faces, mesh.n_faces, ip(k[0]), ip(k[1]), ip(k[2]),
+                           ip(k[3]), dp(k[4]), dp(k[5]), dp(k[6]), dp(k[7]), dp(k[8]), dp(k[9]), dp(k[10]),
+                           dp(k[11]))
+
+    @property
+    def nc(self):
+        return self.mesh.n_cells
+
+    def gradient(self, phi, bv, scheme):
+        out = np.zeros(3 * self.nc)
+        phi, bv = _d(phi), _d(bv)
+        orc().orc_fv_gradient(C.byref(self.m), dp(phi), dp(bv), int(scheme), dp(out))
+        return out.reshape(self.nc, 3)
+
+    def vector_gradient(self, u, ub, scheme):
+        out = np.zeros(9 * self.nc)
+        u, ub = _d(u).reshape(-1), _d(ub).reshape(-1)
+        orc().orc_fv_vector_gradient(C.byref(self.m), dp(u), dp(ub), int(scheme), dp(out))
+        return out.reshape(self.nc, 9)
+
+    def scalar_corrections(self, phi, bv, scheme, flux, gamma, linear_upwind, source):
+        out = _d(source).reshape(-1).copy()
+        phi, bv, flux, gamma = _d(phi), _d(bv), _d(flux), _d(gamma)
+        orc().orc_fv_scalar_corrections(C.byref(self.m), dp(phi), dp(bv), int(scheme), dp(flux), dp(gamma),
+                                        int(bool(linear_upwind)), dp(out))
+        return out
+
+    def limiter(self, u, ub, grad, venkat, eps2):
+        out = np.zeros(3 * self.nc)
+        u, ub, grad, e = _d(u).reshape(-1), _d(ub).reshape(-1), _d(grad).reshape(-1), _d(eps2)
+        orc().orc_fv_limiter(C.byref(self.m), dp(u), dp(ub), dp(grad), int(bool(venkat)), dp(e), dp(out))
+        return out.reshape(self.nc, 3)
+
+    def momentum_corrections(self, u, ub, grad, flux, nu_face, linear_upwind, limited, venkat, eps2, central,
+                             transpose_term, source):
+        out = _d(source).reshape(-1).copy()
+        u, ub, grad = _d(u).reshape(-1), _d(ub).reshape(-1), _d(grad).reshape(-1)
+        flux, nu, e = _d(flux), _d(nu_face), _d(eps2)
+        orc().orc_fv_momentum_corrections(C.byref(self.m), dp(u), dp(ub), dp(grad), dp(flux), dp(nu),
+                                          int(bool(linear_upwind)), int(bool(limited)), int(bool(venkat)), dp(e),
+                                          float(central), int(bool(transpose_term)), dp(out))
+        return out.reshape(self.nc, 3)
+
+
+class FvReference:
+    """The reference's own gpu::DeviceFv (host threads), from _ref/libcfdref.so."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self._keep = [_i(mesh.owner), _i(mesh.neighbour), _i(mesh.cell_offsets), _i(mesh.cell_faces)] + \
+            [_d(a) for a in (mesh.weight, mesh.sf, mesh.face_centre, mesh.cell_centre, mesh.volume,
+                             mesh.correction, mesh.lsq_d, mesh.lsq_inverse)]
+        k = self._keep
+        self.h = ref().ref_fv_create(mesh.n_cells, mesh.n_internal_faces, mesh.n_faces, ip(k[0]), ip(k[1]),
+                                     ip(k[2]), ip(k[3]), *[dp(a) for a in k[4:]])
+
+    def close(self):
+        if self.h:
+            ref().ref_fv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def nc(self):
+        return self.mesh.n_cells
+
+    def gradient(self, phi, bv, scheme):
+        out = np.zeros(3 * self.nc)
+        phi, bv = _d(phi), _d(bv)
+        ref().ref_fv_gradient(self.h, dp(phi), dp(bv), int(scheme), dp(out))
+        return out.reshape(self.nc, 3)
+
+    def vector_gradient(self, u, ub, scheme):
+        out = np.zeros(9 * self.nc)
+        u, ub = _d(u).reshape(-1), _d(ub).reshape(-1)
+        ref().ref_fv_vector_gradient(self.h, dp(u), dp(ub), int(scheme), dp(out))
+        return out.reshape(self.nc, 9)
+
+    def scalar_corrections(self, phi, bv, scheme, flux, gamma, linear_upwind, source):
+        out = _d(source).reshape(-1).copy()
+        phi, bv, flux, gamma = _d(phi), _d(bv), _d(flux), _d(gamma)
+        ref().ref_fv_scalar_corrections(self.h, dp(phi), dp(bv), int(scheme), dp(flux), dp(gamma),
+                                        int(bool(linear_upwind)), dp(out))
+        return out
+
+    def momentum_corrections(self, flux, nu_face, linear_upwind, limited, venkat, eps2, central, transpose_term,
+                             source):
+        out = _d(source).reshape(-1).copy()
+        flux, nu, e = _d(flux), _d(nu_face), _d(eps2)
+        ref().ref_fv_momentum_corrections(self.h, dp(flux), dp(nu), int(bool(linear_upwind)), int(bool(limited)),
+                                          int(bool(venkat)), dp(e), float(central), int(bool(transpose_term)),
+                                          dp(out))
+        return out.reshape(self.nc, 3)

@@ -1255,3 +1255,295 @@ orc_stats orc_sgs_pcg(const orc_amg* h, const double* b, double* x, double tol, 
     free(ap);
     return st;
 }
+
+
+/* ================================================================================================
+ * Finite-volume cell kernels (gpu::DeviceFv). Every cell gathers over its faces in ascending
+ * face order; arithmetic in the order of src/gpu/src/fv_cell.inc.
+ * ============================================================================================== */
+
+/* Gradient of component comp of a field with `stride` values per cell / boundary face into
+ * out[c * ostride + ooff + j] (fv_cell.inc:25-73). */
+static void orc_fv_cell_gradient(const orc_fv_mesh* m, int c, const double* phi, const double* bv,
+                                 int stride, int comp, int least_squares, double* out,
+                                 int ostride, int ooff) {
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    const double pc = phi[c * stride + comp];
+    for (int k = m->start[c]; k < m->start[c + 1]; ++k) {
+        const int f = m->faces[k];
+        if (least_squares) {
+            double dv;
+            if (f < m->nif) {
+                dv = phi[m->nei[f] * stride + comp] - phi[m->own[f] * stride + comp];
+            } else {
+                dv = bv[(f - m->nif) * stride + comp] - pc;
+            }
+            g0 += m->lsq_d[3 * f] * dv;
+            g1 += m->lsq_d[3 * f + 1] * dv;
+            g2 += m->lsq_d[3 * f + 2] * dv;
+        } else if (f < m->nif) {
+            const double wf = m->w[f];
+            const double pf = wf * phi[m->own[f] * stride + comp] + (1.0 - wf) * phi[m->nei[f] * stride + comp];
+            if (m->own[f] == c) {
+                g0 += m->sf[3 * f] * pf;
+                g1 += m->sf[3 * f + 1] * pf;
+                g2 += m->sf[3 * f + 2] * pf;
+            } else {
+                g0 -= m->sf[3 * f] * pf;
+                g1 -= m->sf[3 * f + 1] * pf;
+                g2 -= m->sf[3 * f + 2] * pf;
+            }
+        } else {
+            const double b = bv[(f - m->nif) * stride + comp];
+            g0 += m->sf[3 * f] * b;
+            g1 += m->sf[3 * f + 1] * b;
+            g2 += m->sf[3 * f + 2] * b;
+        }
+    }
+    double* o = out + (size_t)c * ostride + ooff;
+    if (least_squares) {
+        const double* a = m->lsq + 9 * (size_t)c; /* column-major */
+        o[0] = a[0] * g0 + a[3] * g1 + a[6] * g2;
+        o[1] = a[1] * g0 + a[4] * g1 + a[7] * g2;
+        o[2] = a[2] * g0 + a[5] * g1 + a[8] * g2;
+    } else {
+        const double v = m->vol[c];
+        o[0] = g0 / v;
+        o[1] = g1 / v;
+        o[2] = g2 / v;
+    }
+}
+
+/* Skewness-corrected Gauss gradient with the least-squares gradient g (3 per cell)
+ * (fv_cell.inc:77-118). */
+static void orc_fv_cell_gradient_corrected(const orc_fv_mesh* m, int c, const double* phi,
+                                           const double* bv, int stride, int comp, const double* g,
+                                           double* out, int ostride, int ooff) {
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (int k = m->start[c]; k < m->start[c + 1]; ++k) {
+        const int f = m->faces[k];
+        if (f < m->nif) {
+            const int p = m->own[f];
+            const int n = m->nei[f];
+            const double wf = m->w[f];
+            double pf = wf * phi[p * stride + comp] + (1.0 - wf) * phi[n * stride + comp];
+            double dot = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                const double xip = wf * m->cc[3 * p + j] + (1.0 - wf) * m->cc[3 * n + j];
+                const double gf = wf * g[3 * p + j] + (1.0 - wf) * g[3 * n + j];
+                if (j == 0) {
+                    dot = gf * (m->cf[3 * f] - xip);
+                } else {
+                    dot = dot + gf * (m->cf[3 * f + j] - xip);
+                }
+            }
+            pf += dot;
+            if (p == c) {
+                g0 += m->sf[3 * f] * pf;
+                g1 += m->sf[3 * f + 1] * pf;
+                g2 += m->sf[3 * f + 2] * pf;
+            } else {
+                g0 -= m->sf[3 * f] * pf;
+                g1 -= m->sf[3 * f + 1] * pf;
+                g2 -= m->sf[3 * f + 2] * pf;
+            }
+        } else {
+            const double b = bv[(f - m->nif) * stride + comp];
+            g0 += m->sf[3 * f] * b;
+            g1 += m->sf[3 * f + 1] * b;
+            g2 += m->sf[3 * f + 2] * b;
+        }
+    }
+    double* o = out + (size_t)c * ostride + ooff;
+    const double v = m->vol[c];
+    o[0] = g0 / v;
+    o[1] = g1 / v;
+    o[2] = g2 / v;
+}
+
+/* One component's gradient over all cells (fv_openmp.cpp:257-311): scheme 2 first forms the
+ * least-squares gradient, then the corrected one. */
+static void orc_fv_gradient_loop(const orc_fv_mesh* m, const double* phi, const double* bv, int stride,
+                                 int comp, int scheme, double* out, int ostride, int ooff) {
+    if (scheme == 2) {
+        double* tmp = (double*)malloc(sizeof(double) * 3 * (size_t)m->nc);
+        for (int c = 0; c < m->nc; ++c) {
+            orc_fv_cell_gradient(m, c, phi, bv, stride, comp, 1, tmp, 3, 0);
+        }
+        for (int c = 0; c < m->nc; ++c) {
+            orc_fv_cell_gradient_corrected(m, c, phi, bv, stride, comp, tmp, out, ostride, ooff);
+        }
+        free(tmp);
+        return;
+    }
+    for (int c = 0; c < m->nc; ++c) {
+        orc_fv_cell_gradient(m, c, phi, bv, stride, comp, scheme == 1, out, ostride, ooff);
+    }
+}
+
+/* fv_openmp.cpp:130-135 */
+void orc_fv_gradient(const orc_fv_mesh* m, const double* phi, const double* bv, int scheme, double* grad) {
+    orc_fv_gradient_loop(m, phi, bv, 1, 0, scheme, grad, 3, 0);
+}
+
+/* fv_openmp.cpp:137-145: component i of u into columns 3 i .. 3 i + 2. */
+void orc_fv_vector_gradient(const orc_fv_mesh* m, const double* u, const double* ub, int scheme,
+                            double* grad) {
+    for (int i = 0; i < 3; ++i) {
+        orc_fv_gradient_loop(m, u, ub, 3, i, scheme, grad, 9, 3 * i);
+    }
+}
+
+/* fv_cell.inc:167-195 with the gradient of fv_openmp.cpp:147-152. */
+void orc_fv_scalar_corrections(const orc_fv_mesh* m, const double* phi, const double* bv, int scheme,
+                               const double* flux, const double* gamma, int linear_upwind, double* source) {
+    double* g = (double*)malloc(sizeof(double) * 3 * (size_t)m->nc);
+    orc_fv_gradient(m, phi, bv, scheme, g);
+    for (int c = 0; c < m->nc; ++c) {
+        double s = 0.0;
+        for (int k = m->start[c]; k < m->start[c + 1]; ++k) {
+            const int f = m->faces[k];
+            if (f >= m->nif) {
+                break;
+            }
+            const int p = m->own[f];
+            const int n = m->nei[f];
+            const double fl = flux[f];
+            double e = 0.0;
+            if (linear_upwind) {
+                const int up = fl >= 0.0 ? p : n;
+                const double r0 = m->cf[3 * f] - m->cc[3 * up];
+                const double r1 = m->cf[3 * f + 1] - m->cc[3 * up + 1];
+                const double r2 = m->cf[3 * f + 2] - m->cc[3 * up + 2];
+                e -= fl * (g[3 * up] * r0 + g[3 * up + 1] * r1 + g[3 * up + 2] * r2);
+            }
+            const double wf = m->w[f];
+            const double gf0 = wf * g[3 * p] + (1.0 - wf) * g[3 * n];
+            const double gf1 = wf * g[3 * p + 1] + (1.0 - wf) * g[3 * n + 1];
+            const double gf2 = wf * g[3 * p + 2] + (1.0 - wf) * g[3 * n + 2];
+            e += gamma[f] * (m->kv[3 * f] * gf0 + m->kv[3 * f + 1] * gf1 + m->kv[3 * f + 2] * gf2);
+            s = p == c ? s + e : s - e;
+        }
+        source[c] += s;
+    }
+    free(g);
+}
+
+/* Barth-Jespersen / Venkatakrishnan limiter (fv_cell.inc:123-164). */
+void orc_fv_limiter(const orc_fv_mesh* m, const double* u, const double* ub, const double* grad, int venkat,
+                    const double* eps2, double* lim) {
+    const double e[3] = {venkat ? eps2[0] : 0.0, venkat ? eps2[1] : 0.0, venkat ? eps2[2] : 0.0};
+    for (int c = 0; c < m->nc; ++c) {
+        double lo[3], hi[3];
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = u[3 * c + i];
+            hi[i] = u[3 * c + i];
+        }
+        for (int k = m->start[c]; k < m->start[c + 1]; ++k) {
+            const int f = m->faces[k];
+            const double* v = f < m->nif ? u + 3 * (m->own[f] == c ? m->nei[f] : m->own[f]) : ub + 3 * (f - m->nif);
+            for (int i = 0; i < 3; ++i) {
+                lo[i] = v[i] < lo[i] ? v[i] : lo[i];
+                hi[i] = v[i] > hi[i] ? v[i] : hi[i];
+            }
+        }
+        double l[3] = {1.0, 1.0, 1.0};
+        const double* gc = grad + 9 * (size_t)c;
+        for (int k = m->start[c]; k < m->start[c + 1]; ++k) {
+            const int f = m->faces[k];
+            const double r0 = m->cf[3 * f] - m->cc[3 * c];
+            const double r1 = m->cf[3 * f + 1] - m->cc[3 * c + 1];
+            const double r2 = m->cf[3 * f + 2] - m->cc[3 * c + 2];
+            for (int i = 0; i < 3; ++i) {
+                const double d = gc[3 * i] * r0 + gc[3 * i + 1] * r1 + gc[3 * i + 2] * r2;
+                if (d < 1e-300 && d > -1e-300) {
+                    continue;
+                }
+                const double bound = d > 0.0 ? hi[i] - u[3 * c + i] : lo[i] - u[3 * c + i];
+                double q;
+                if (venkat) {
+                    q = (bound * bound + e[i] + 2.0 * d * bound) / (bound * bound + 2.0 * d * d + d * bound + e[i]);
+                } else {
+                    q = bound / d > 0.0 ? bound / d : 0.0;
+                }
+                l[i] = q < l[i] ? q : l[i];
+            }
+        }
+        for (int i = 0; i < 3; ++i) {
+            lim[3 * c + i] = l[i];
+        }
+    }
+}
+
+/* fv_cell.inc:199-260 with the limiter pass of fv_openmp.cpp:202-246 (only when limited and
+ * linear upwind). */
+void orc_fv_momentum_corrections(const orc_fv_mesh* m, const double* u, const double* ub, const double* grad,
+                                 const double* flux, const double* nu_face, int linear_upwind, int limited,
+                                 int venkat, const double* eps2, double central, int transpose_term,
+                                 double* source) {
+    const int use_lim = limited && linear_upwind;
+    double* lim = NULL;
+    if (use_lim) {
+        lim = (double*)malloc(sizeof(double) * 3 * (size_t)m->nc);
+        orc_fv_limiter(m, u, ub, grad, venkat, eps2, lim);
+    }
+    for (int c = 0; c < m->nc; ++c) {
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int k = m->start[c]; k < m->start[c + 1]; ++k) {
+            const int f = m->faces[k];
+            if (f >= m->nif) {
+                break;
+            }
+            const int p = m->own[f];
+            const int n = m->nei[f];
+            const double fl = flux[f];
+            double e[3] = {0.0, 0.0, 0.0};
+            if (linear_upwind) {
+                const int up = fl >= 0.0 ? p : n;
+                const double r0 = m->cf[3 * f] - m->cc[3 * up];
+                const double r1 = m->cf[3 * f + 1] - m->cc[3 * up + 1];
+                const double r2 = m->cf[3 * f + 2] - m->cc[3 * up + 2];
+                const double* gu = grad + 9 * (size_t)up;
+                for (int i = 0; i < 3; ++i) {
+                    double d = gu[3 * i] * r0 + gu[3 * i + 1] * r1 + gu[3 * i + 2] * r2;
+                    if (central > 0.0) {
+                        const double wc = m->w[f];
+                        const double uf = wc * u[3 * p + i] + (1.0 - wc) * u[3 * n + i];
+                        d = (1.0 - central) * d + central * (uf - u[3 * up + i]);
+                    }
+                    if (use_lim) {
+                        d = lim[3 * up + i] * d;
+                    }
+                    e[i] -= fl * d;
+                }
+            }
+            const double wf = m->w[f];
+            double gf[9];
+            for (int j = 0; j < 9; ++j) {
+                gf[j] = wf * grad[9 * (size_t)p + j] + (1.0 - wf) * grad[9 * (size_t)n + j];
+            }
+            const double nu = nu_face[f];
+            const double k0 = m->kv[3 * f];
+            const double k1 = m->kv[3 * f + 1];
+            const double k2 = m->kv[3 * f + 2];
+            for (int i = 0; i < 3; ++i) {
+                e[i] += nu * (gf[3 * i] * k0 + gf[3 * i + 1] * k1 + gf[3 * i + 2] * k2);
+            }
+            if (transpose_term) {
+                const double a0 = m->sf[3 * f];
+                const double a1 = m->sf[3 * f + 1];
+                const double a2 = m->sf[3 * f + 2];
+                for (int i = 0; i < 3; ++i) {
+                    e[i] += nu * (gf[i] * a0 + gf[i + 3] * a1 + gf[i + 6] * a2);
+                }
+            }
+            for (int i = 0; i < 3; ++i) {
+                s[i] = p == c ? s[i] + e[i] : s[i] - e[i];
+            }
+        }
+        for (int i = 0; i < 3; ++i) {
+            source[3 * c + i] += s[i];
+        }
+    }
+    free(lim);
+}

@@ -89,6 +89,43 @@ orc_stats orc_amg_cg_h(const orc_amg* h, const int* rp, const int* ci, const dou
  * (fv::pcg amg.cpp:253-285, cycle amg.cpp:192-216). Breakdown when p.Ap <= 0. */
 orc_stats orc_sgs_pcg(const orc_amg* h, const double* b, double* x, double tol, int max_it);
 
+/* ---- Finite-volume cell kernels (gpu::DeviceFv, src/gpu/src/fv_cell.inc, fv_openmp.cpp) ----
+ * The mesh struct has FvMeshView's fields in its order (src/gpu/include/cfd/gpu/fv_device.hpp:16-32). */
+typedef struct orc_fv_mesh {
+    int nc, nif, nf;
+    const int* own;
+    const int* nei;
+    const int* start;
+    const int* faces;
+    const double* w;
+    const double* sf;
+    const double* cf;
+    const double* cc;
+    const double* vol;
+    const double* kv;
+    const double* lsq_d;
+    const double* lsq;
+} orc_fv_mesh;
+
+/* DeviceFv::gradient of a scalar field: scheme 0 Green-Gauss, 1 least squares, 2 corrected. */
+void orc_fv_gradient(const orc_fv_mesh* m, const double* phi, const double* bv, int scheme,
+                     double* grad);
+/* DeviceFv::vector_gradient: u 3 per cell, grad 9 per cell (column i = d u_i / d x). */
+void orc_fv_vector_gradient(const orc_fv_mesh* m, const double* u, const double* ub, int scheme,
+                            double* grad);
+/* DeviceFv::scalar_corrections: source += explicit internal-face fluxes. */
+void orc_fv_scalar_corrections(const orc_fv_mesh* m, const double* phi, const double* bv,
+                               int scheme, const double* flux, const double* gamma,
+                               int linear_upwind, double* source);
+/* Limiter of DeviceFv::momentum_corrections (lim 3 per cell). */
+void orc_fv_limiter(const orc_fv_mesh* m, const double* u, const double* ub, const double* grad,
+                    int venkat, const double* eps2, double* lim);
+/* DeviceFv::momentum_corrections with the field u / ub and gradient grad given explicitly. */
+void orc_fv_momentum_corrections(const orc_fv_mesh* m, const double* u, const double* ub,
+                                 const double* grad, const double* flux, const double* nu_face,
+                                 int linear_upwind, int limited, int venkat, const double* eps2,
+                                 double central, int transpose_term, double* source);
+
 #ifdef __cplusplus
 }
 #endif

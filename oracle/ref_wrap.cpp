@@ -2,10 +2,12 @@
 //
 // TEST INFRASTRUCTURE ONLY. Built by oracle/Makefile, in this container only, together with
 // the reference's Eigen-free accelerator sources (src/gpu/src/krylov_openmp.cpp,
-// dispatch.cpp, device_solver.cpp, compiled where they lie under the reference checkout with no
+// dispatch.cpp, device_solver.cpp, fv_device.cpp, fv_openmp.cpp, compiled where they lie under
+// the reference checkout with no
 // backend defined, i.e. the "host" build of cfd::gpu) into oracle/_ref/libcfdref.so. Tests use
 // it to check the oracle's restatement against the reference's actual host algorithms and
 // bench.py times it as the CPU baseline. Nothing here is part of the product.
+#include <cfd/gpu/fv_device.hpp>
 #include <cfd/gpu/krylov.hpp>
 
 #include <cstring>
@@ -133,6 +135,57 @@ void ref_device_solver_solve(void* h, const double* diag, const double* upper,
         diag, upper, lower, b, x, method == 0 ? Method::Cg : Method::BiCgStab, use_amg != 0,
         SolveControl{tol, max_it});
     put(st, iterations, converged, residual);
+}
+
+// gpu::DeviceFv on host threads (fv_device.cpp:51-59 -> fv_openmp.cpp). The mesh arrays follow
+// FvMeshView's field order (fv_device.hpp:16-32).
+void* ref_fv_create(int n_cells, int n_internal_faces, int n_faces, const int* owner,
+                    const int* neighbour, const int* cell_offsets, const int* cell_faces,
+                    const double* weight, const double* sf, const double* face_centre,
+                    const double* cell_centre, const double* volume, const double* correction,
+                    const double* lsq_d, const double* lsq_inverse) {
+    cfd::gpu::FvMeshView v;
+    v.n_cells = n_cells;
+    v.n_internal_faces = n_internal_faces;
+    v.n_faces = n_faces;
+    v.owner = owner;
+    v.neighbour = neighbour;
+    v.cell_offsets = cell_offsets;
+    v.cell_faces = cell_faces;
+    v.weight = weight;
+    v.sf = sf;
+    v.face_centre = face_centre;
+    v.cell_centre = cell_centre;
+    v.volume = volume;
+    v.correction = correction;
+    v.lsq_d = lsq_d;
+    v.lsq_inverse = lsq_inverse;
+    return new cfd::gpu::DeviceFv(v);
+}
+
+void ref_fv_destroy(void* h) { delete static_cast<cfd::gpu::DeviceFv*>(h); }
+
+void ref_fv_gradient(void* h, const double* phi, const double* bv, int scheme, double* grad) {
+    static_cast<cfd::gpu::DeviceFv*>(h)->gradient(phi, bv, scheme, grad);
+}
+
+void ref_fv_vector_gradient(void* h, const double* u, const double* ub, int scheme, double* grad) {
+    static_cast<cfd::gpu::DeviceFv*>(h)->vector_gradient(u, ub, scheme, grad);
+}
+
+void ref_fv_scalar_corrections(void* h, const double* phi, const double* bv, int scheme,
+                               const double* flux, const double* gamma, int linear_upwind,
+                               double* source) {
+    static_cast<cfd::gpu::DeviceFv*>(h)->scalar_corrections(phi, bv, scheme, flux, gamma,
+                                                            linear_upwind != 0, source);
+}
+
+void ref_fv_momentum_corrections(void* h, const double* flux, const double* nu_face,
+                                 int linear_upwind, int limited, int venkat, const double* eps2,
+                                 double central, int transpose_term, double* source) {
+    static_cast<cfd::gpu::DeviceFv*>(h)->momentum_corrections(
+        flux, nu_face, linear_upwind != 0, limited != 0, venkat != 0, eps2, central,
+        transpose_term != 0, source);
 }
 
 } // extern "C"

@@ -43,6 +43,8 @@ def test_no_device_is_an_error_not_a_fallback():
         cfd_amd.DeviceSolver(pat)
     with pytest.raises(L.NoDevice):
         cfd_amd.Server()
+    with pytest.raises(L.NoDevice):
+        cfd_amd.DeviceFv(cfd_amd.fv_box(3, 2, 2))
 
 
 def test_footprint_estimate_bounds_the_stored_layout():
