@@ -293,6 +293,118 @@ def served(count: int = 24, rate: float = 200.0, seed: int = 20240607) -> dict:
                 round(mixed["solves_per_s"] / plain["solves_per_s"], 3)}
 
 
+def fv_bytes(m, op: str) -> float:
+    """Algorithmic bytes of one finite-volume operation: every array element it needs, read or
+    written once (the neighbour values a cell gathers are counted once, with their array)."""
+    nc, nif, nf = m.n_cells, m.n_internal_faces, m.n_faces
+    nb = nf - nif
+    lists = 4 * (nc + 1) + 4 * (2 * nif + nb)          # cell_offsets + cell_faces
+    inner = 4 * (nc + 1) + 4 * (2 * nif + nc)          # lists read up to the first boundary face
+    own_nei = 8 * nif
+
+    def gauss(k):                                        # k components
+        return lists + own_nei + 8 * nif + 24 * nf + 8 * k * (nc + nb) + 8 * nc + 24 * k * nc
+
+    def lsq(k):
+        return lists + own_nei + 24 * nf + 8 * k * (nc + nb) + 72 * nc + 24 * k * nc
+
+    def corrected(k):
+        second = lists + own_nei + 8 * nif + 24 * nf + 24 * nif + 24 * nc + 8 * k * (nc + nb) + 24 * k * nc \
+            + 8 * nc + 24 * k * nc
+        return lsq(k) + second
+
+    grad = {"gradient_gauss": gauss(1), "gradient_least_squares": lsq(1), "gradient_corrected": corrected(1),
+            "vector_gradient_least_squares": lsq(3), "vector_gradient_corrected": corrected(3)}
+    if op in grad:
+        return float(grad[op])
+    if op == "scalar_corrections":
+        corr = inner + own_nei + 8 * nif + 24 * nif + 24 * nc + 24 * nc + 8 * nif + 8 * nif + 24 * nif + 16 * nc
+        return float(lsq(1) + corr)
+    if op == "momentum_corrections":
+        limiter = lists + own_nei + 24 * nc + 24 * nb + 72 * nc + 24 * nf + 24 * nc + 24 * nc
+        mom = inner + own_nei + 8 * nif + 24 * nif + 24 * nc + 72 * nc + 24 * nc + 24 * nc + 8 * nif + 8 * nif \
+            + 24 * nif + 24 * nif + 48 * nc
+        return float(limiter + mom)
+    raise KeyError(op)
+
+
+def fv_section(n_side: int = 128, reps: int = 10, cpu: bool = True) -> dict:
+    """Finite-volume cell kernels (cfdg_fv_*) on an n_side^3 jittered hex box: device time of each
+    operation's kernels (HIP events in the library), algorithmic GB/s against the 8 TB/s HBM
+    roofline, the PCIe-inclusive time of the call, and the reference's gpu::DeviceFv host path on
+    the same mesh (oracle/_ref, all host cores) as the CPU baseline."""
+    from cfd_amd.fv import DeviceFv, fv_box
+
+    t0 = time.perf_counter()
+    mesh = fv_box(n_side, n_side, n_side, jitter=0.2, seed=11)
+    rng = np.random.default_rng(5)
+    nc, nb, nif = mesh.n_cells, mesh.n_boundary, mesh.n_internal_faces
+    phi, bv = rng.standard_normal(nc), rng.standard_normal(nb)
+    u, ub = rng.standard_normal((nc, 3)), rng.standard_normal((nb, 3))
+    flux, nu = rng.standard_normal(nif), rng.uniform(0.01, 0.2, nif)
+    src, src3 = np.zeros(nc), np.zeros((nc, 3))
+    eps2 = np.array([1e-3, 1e-3, 1e-3])
+    log(f"fv: {nc} cells, {mesh.n_faces} faces, mesh built in {time.perf_counter() - t0:.1f} s")
+    ops = {
+        "gradient_gauss": lambda d: d.gradient(phi, bv, 0),
+        "gradient_least_squares": lambda d: d.gradient(phi, bv, 1),
+        "gradient_corrected": lambda d: d.gradient(phi, bv, 2),
+        "vector_gradient_least_squares": lambda d: d.vector_gradient(u, ub, 1),
+        "vector_gradient_corrected": lambda d: d.vector_gradient(u, ub, 2),
+        "scalar_corrections": lambda d: d.scalar_corrections(phi, bv, 1, flux, nu, True, src),
+        "momentum_corrections": lambda d: d.momentum_corrections(flux, nu, True, True, False, eps2, 0.0, True, src3),
+    }
+    out = {"workload": f"{n_side}^3 jittered hex box ({nc} cells, {mesh.n_faces} faces), FP64",
+           "roofline_peak_gbs": 8000.0, "ops": {}}
+    with DeviceFv(mesh) as dev:
+        dev.vector_gradient(u, ub, 1)
+        for name, f in ops.items():
+            f(dev)  # warm-up
+            dev.timing(True)
+            dms = []
+            for _ in range(reps):
+                f(dev)
+                dms.append(dev.last_kernel_ms())
+            dev.timing(False)
+            t = time.perf_counter()
+            for _ in range(reps):
+                f(dev)
+            wall = (time.perf_counter() - t) / reps
+            ms = float(np.median(dms))
+            nbytes = fv_bytes(mesh, name)
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            out["ops"][name] = {"device_ms": round(ms, 4), "bytes": int(nbytes), "achieved_gbs": round(gbs, 1),
+                                "frac": round(gbs / 8000.0, 4), "call_ms_with_pcie": round(1e3 * wall, 2)}
+            log(f"fv {name}: {ms:.3f} ms device, {gbs:.0f} GB/s ({gbs / 80.0:.1f}% of HBM), "
+                f"{1e3 * wall:.1f} ms with PCIe")
+    dom = max(out["ops"], key=lambda k: out["ops"][k]["device_ms"])
+    out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": out["ops"][dom]["achieved_gbs"], "peak": 8000.0,
+                       "unit": "GB/s", "frac": out["ops"][dom]["frac"]}
+    if cpu:
+        out["cpu_baseline"] = fv_cpu_baseline(mesh, ops)
+    return out
+
+
+def fv_cpu_baseline(mesh, ops) -> dict:
+    """The reference's gpu::DeviceFv on host threads (oracle/_ref), one call of each operation on
+    the same mesh and fields, in the same order (the momentum corrections use the vector gradient
+    of the call before them)."""
+    from oracle import bind as ob
+
+    kind = "reference" if ob.ref_available() else "port"
+    host = ob.FvReference(mesh) if kind == "reference" else None
+    res = {}
+    for name, f in ops.items():
+        if host is None:
+            break
+        t = time.perf_counter()
+        f(host)
+        res[name] = round(1e3 * (time.perf_counter() - t), 2)
+    cores = os.cpu_count() or 1
+    return {"unit": "ms per call", "ms": res, "cores": int(os.environ.get("OMP_NUM_THREADS", cores)), "kind": kind,
+            "sample": "one call of each operation on the same mesh (host arrays, no transfers)"}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +416,8 @@ def main(argv=None) -> int:
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-stress", action="store_true")
+    ap.add_argument("--skip-fv", action="store_true")
+    ap.add_argument("--fv-only", action="store_true", help="only the finite-volume kernel section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 10 device copies of 256 MiB (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -317,6 +431,9 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
                           "copies": 10}))
+        return 0
+    if args.fv_only:
+        print(json.dumps({"fv": fv_section(cpu=not args.skip_cpu)}), flush=True)
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -486,6 +603,8 @@ def main(argv=None) -> int:
         out["served"] = served()
     if not args.skip_stress:
         out["precision_stress"] = stress()
+    if not args.skip_fv:
+        out["fv"] = fv_section(cpu=not args.skip_cpu)
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(view, b, tol, args.cpu_seconds)
     print(json.dumps(out), flush=True)
