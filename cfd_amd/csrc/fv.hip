@@ -4,12 +4,15 @@
 // gpu::DeviceFv, src/gpu/include/cfd/gpu/fv_device.hpp:34-71; per-cell math of
 // src/gpu/src/fv_cell.inc:25-260, sequencing of src/gpu/src/fv_openmp.cpp:130-322).
 //
-// One thread per cell gathers over the cell's faces in ascending face order (the mesh's CSR
-// face list), with the reference's operation order and no contraction of multiply-adds, so every
-// result equals the serial CPU face loops bit for bit; nothing is scattered, nothing is atomic.
-// The three components of a vector gradient are formed in one pass over the faces (the face
-// geometry is read once instead of three times). Workgroups map to XCDs in contiguous row
-// ranges, so the neighbours a cell gathers from sit in the same L2.
+// One thread per cell runs over the cell's faces in ascending face order, with the reference's
+// operation order and no contraction of multiply-adds, so every result equals the serial CPU face
+// loops bit for bit; nothing is scattered, nothing is atomic. The face data a cell needs is laid
+// out once, at creation, in per-cell slots, slot-major over the 64 cells of a wavefront and one
+// array per component (the sliced layout of the solver's matrices): every kernel streams it
+// coalesced instead of gathering 3-vectors of faces shared with other cells through the face list
+// (which re-read 2.6-5x the useful bytes). The three components of a vector gradient are formed in
+// one pass over the slots. Workgroups map to XCDs in contiguous cell ranges, so the neighbour
+// values a cell gathers sit in the same L2.
 #include "common.hpp"
 #include "hip_util.hpp"
 
@@ -17,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <memory>
 #include <string>
 #include <vector>
@@ -47,34 +51,52 @@ __device__ __forceinline__ int fv_cell() {
     return blk * k_fv_block + static_cast<int>(threadIdx.x);
 }
 
-// Device copies of the mesh arrays.
-struct FvMesh {
+// The mesh as per-cell face slots. Slot k of cell 64 s + lane (slice s = one wavefront) sits at
+// base[s] + 64 k + lane, so a wavefront's loads of one slot are 64 consecutive entries; a slice
+// has as many slots as its cell with the most faces, and each cell's slots keep its ascending
+// face order (padding, face -1, comes last). A face's data is stored in the slots of both its
+// cells, component by component: every kernel streams it once, coalesced, instead of gathering
+// 3-vectors of faces that other cells share through the face list.
+struct FvSlots {
     int nc;
     int nif;
-    int nf;
-    const int* own;
-    const int* nei;
-    const int* start;
-    const int* faces;
-    const double* w;
-    const double* sf;
-    const double* cf;
-    const double* cc;
+    const int* base;           // n_slices + 1
+    const int* face;           // face id (per-call face arrays: flux, gamma, nu), -1 padding
+    const int* other;          // internal face: the other cell; boundary face: -1 - boundary index
+    const unsigned char* owns; // 1 when the slot's cell owns the face
+    const double* w;           // interpolation weight (internal faces)
+    const double* sf0;
+    const double* sf1;
+    const double* sf2;
+    const double* ld0;         // least-squares vector d / |d|^2
+    const double* ld1;
+    const double* ld2;
+    const double* cf0;         // face centre
+    const double* cf1;
+    const double* cf2;
+    const double* kv0;         // non-orthogonal correction vector
+    const double* kv1;
+    const double* kv2;
+    const double* cc;          // 3 per cell
     const double* vol;
-    const double* kv;
-    const double* lsq_d;
-    const double* lsq;
+    const double* lsq;         // 9 per cell, column-major
 };
 
-// Gradient of NC components (comp0 .. comp0 + NC - 1) of a field with `stride` values per cell
-// and boundary face; component i goes to out[c * ostride + ooff + 3 i + j]. Least squares:
-// A (sum_f d_f dphi_f) with A the cell's column-major inverse; Green-Gauss: sum_f S_f phi_f / V
-// with phi_f = w phi_P + (1 - w) phi_N (fv_cell.inc:25-73).
+// Slot range of the calling cell's slice: first slot index of this lane and the slice's slot count.
+__device__ __forceinline__ void slot_range(const FvSlots& m, int c, int& first, int& count) {
+    const int s = c >> 6;
+    const int b0 = g(m.base)[s];
+    first = b0 + (c & 63);
+    count = (g(m.base)[s + 1] - b0) >> 6;
+}
+
+// Gradient of NC components of a field with `stride` values per cell and boundary face; component
+// i goes to out[c * ostride + 3 i + j]. Least squares: A (sum_f d_f dphi_f), A the cell's
+// column-major inverse; Green-Gauss: sum_f S_f phi_f / V with phi_f = w phi_P + (1 - w) phi_N
+// (fv_cell.inc:25-73).
 template <bool Lsq, int NC>
-__device__ __forceinline__ void cell_gradient(const FvMesh& m, int c, const double* phi, const double* bv,
-                                              int stride, int comp0, double* out, int ostride, int ooff) {
-    const auto own = g(m.own);
-    const auto nei = g(m.nei);
+__device__ __forceinline__ void cell_gradient(const FvSlots& m, int c, const double* phi, const double* bv, int stride,
+                                              double* out, int ostride) {
     const auto fp = g(phi);
     double g0[NC];
     double g1[NC];
@@ -85,46 +107,41 @@ __device__ __forceinline__ void cell_gradient(const FvMesh& m, int c, const doub
         g0[i] = 0.0;
         g1[i] = 0.0;
         g2[i] = 0.0;
-        pc[i] = fp[c * stride + comp0 + i];
+        pc[i] = fp[c * stride + i];
     }
-    const int k1 = g(m.start)[c + 1];
-    for (int k = g(m.start)[c]; k < k1; ++k) {
-        const int f = g(m.faces)[k];
+    int first = 0;
+    int count = 0;
+    slot_range(m, c, first, count);
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        if (g(m.face)[t] < 0) {
+            break; // padding follows the cell's faces
+        }
+        const int o = g(m.other)[t];
+        const bool mine = g(m.owns)[t] != 0;
+        const int p = mine ? c : o;
+        const int n = mine ? o : c;
         if (Lsq) {
-            const double d0 = g(m.lsq_d)[3 * f];
-            const double d1 = g(m.lsq_d)[3 * f + 1];
-            const double d2 = g(m.lsq_d)[3 * f + 2];
-            if (f < m.nif) {
-                const int p = own[f];
-                const int n = nei[f];
+            const double d0 = g(m.ld0)[t];
+            const double d1 = g(m.ld1)[t];
+            const double d2 = g(m.ld2)[t];
 #pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    const double dv = fp[n * stride + comp0 + i] - fp[p * stride + comp0 + i];
-                    g0[i] += d0 * dv;
-                    g1[i] += d1 * dv;
-                    g2[i] += d2 * dv;
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    const double dv = g(bv)[(f - m.nif) * stride + comp0 + i] - pc[i];
-                    g0[i] += d0 * dv;
-                    g1[i] += d1 * dv;
-                    g2[i] += d2 * dv;
-                }
+            for (int i = 0; i < NC; ++i) {
+                const double dv = o >= 0 ? fp[n * stride + i] - fp[p * stride + i] : g(bv)[(-1 - o) * stride + i] - pc[i];
+                g0[i] += d0 * dv;
+                g1[i] += d1 * dv;
+                g2[i] += d2 * dv;
             }
         } else {
-            const double s0 = g(m.sf)[3 * f];
-            const double s1 = g(m.sf)[3 * f + 1];
-            const double s2 = g(m.sf)[3 * f + 2];
-            if (f < m.nif) {
-                const int p = own[f];
-                const int n = nei[f];
-                const double wf = g(m.w)[f];
+            const double s0 = g(m.sf0)[t];
+            const double s1 = g(m.sf1)[t];
+            const double s2 = g(m.sf2)[t];
+            if (o >= 0) {
+                const double wf = g(m.w)[t];
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
-                    const double pf = wf * fp[p * stride + comp0 + i] + (1.0 - wf) * fp[n * stride + comp0 + i];
-                    if (p == c) {
+                    const double pf = wf * fp[p * stride + i] + (1.0 - wf) * fp[n * stride + i];
+                    if (mine) {
                         g0[i] += s0 * pf;
                         g1[i] += s1 * pf;
                         g2[i] += s2 * pf;
@@ -137,7 +154,7 @@ __device__ __forceinline__ void cell_gradient(const FvMesh& m, int c, const doub
             } else {
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
-                    const double b = g(bv)[(f - m.nif) * stride + comp0 + i];
+                    const double b = g(bv)[(-1 - o) * stride + i];
                     g0[i] += s0 * b;
                     g1[i] += s1 * b;
                     g2[i] += s2 * b;
@@ -145,37 +162,34 @@ __device__ __forceinline__ void cell_gradient(const FvMesh& m, int c, const doub
             }
         }
     }
-    double* o = out + static_cast<long long>(c) * ostride + ooff;
+    double* out_c = out + static_cast<long long>(c) * ostride;
     if (Lsq) {
         const auto a = g(m.lsq) + 9LL * c;
         const double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6], a7 = a[7],
                      a8 = a[8];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            o[3 * i] = a0 * g0[i] + a3 * g1[i] + a6 * g2[i];
-            o[3 * i + 1] = a1 * g0[i] + a4 * g1[i] + a7 * g2[i];
-            o[3 * i + 2] = a2 * g0[i] + a5 * g1[i] + a8 * g2[i];
+            out_c[3 * i] = a0 * g0[i] + a3 * g1[i] + a6 * g2[i];
+            out_c[3 * i + 1] = a1 * g0[i] + a4 * g1[i] + a7 * g2[i];
+            out_c[3 * i + 2] = a2 * g0[i] + a5 * g1[i] + a8 * g2[i];
         }
     } else {
         const double v = g(m.vol)[c];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            o[3 * i] = g0[i] / v;
-            o[3 * i + 1] = g1[i] / v;
-            o[3 * i + 2] = g2[i] / v;
+            out_c[3 * i] = g0[i] / v;
+            out_c[3 * i + 1] = g1[i] / v;
+            out_c[3 * i + 2] = g2[i] / v;
         }
     }
 }
 
-// Skewness-corrected Green-Gauss gradient with the least-squares gradients lg (3 per component
-// and cell, component i of cell c at lg[c * 3 NC + 3 i]): face values w phi_P + (1 - w) phi_N +
-// (w g_P + (1 - w) g_N) . (x_f - x_ip), x_ip = w C_P + (1 - w) C_N (fv_cell.inc:77-118).
+// Skewness-corrected Green-Gauss gradient with the least-squares gradients lg (component i of cell
+// c at lg[c * 3 NC + 3 i]): face values w phi_P + (1 - w) phi_N + (w g_P + (1 - w) g_N) . (x_f - x_ip),
+// x_ip = w C_P + (1 - w) C_N (fv_cell.inc:77-118).
 template <int NC>
-__device__ __forceinline__ void cell_gradient_corrected(const FvMesh& m, int c, const double* phi, const double* bv,
-                                                        int stride, int comp0, const double* lg, double* out,
-                                                        int ostride, int ooff) {
-    const auto own = g(m.own);
-    const auto nei = g(m.nei);
+__device__ __forceinline__ void cell_gradient_corrected(const FvSlots& m, int c, const double* phi, const double* bv,
+                                                        int stride, const double* lg, double* out, int ostride) {
     const auto fp = g(phi);
     const auto gl = g(lg);
     const auto cc = g(m.cc);
@@ -188,25 +202,33 @@ __device__ __forceinline__ void cell_gradient_corrected(const FvMesh& m, int c, 
         g1[i] = 0.0;
         g2[i] = 0.0;
     }
-    const int k1 = g(m.start)[c + 1];
-    for (int k = g(m.start)[c]; k < k1; ++k) {
-        const int f = g(m.faces)[k];
-        const double s0 = g(m.sf)[3 * f];
-        const double s1 = g(m.sf)[3 * f + 1];
-        const double s2 = g(m.sf)[3 * f + 2];
-        if (f < m.nif) {
-            const int p = own[f];
-            const int n = nei[f];
-            const double wf = g(m.w)[f];
+    int first = 0;
+    int count = 0;
+    slot_range(m, c, first, count);
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        if (g(m.face)[t] < 0) {
+            break;
+        }
+        const int o = g(m.other)[t];
+        const double s0 = g(m.sf0)[t];
+        const double s1 = g(m.sf1)[t];
+        const double s2 = g(m.sf2)[t];
+        if (o >= 0) {
+            const bool mine = g(m.owns)[t] != 0;
+            const int p = mine ? c : o;
+            const int n = mine ? o : c;
+            const double wf = g(m.w)[t];
+            const double cfj[3] = {g(m.cf0)[t], g(m.cf1)[t], g(m.cf2)[t]};
             double r[3];
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const double xip = wf * cc[3 * p + j] + (1.0 - wf) * cc[3 * n + j];
-                r[j] = g(m.cf)[3 * f + j] - xip;
+                r[j] = cfj[j] - xip;
             }
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
-                double pf = wf * fp[p * stride + comp0 + i] + (1.0 - wf) * fp[n * stride + comp0 + i];
+                double pf = wf * fp[p * stride + i] + (1.0 - wf) * fp[n * stride + i];
                 double dot = 0.0;
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
@@ -214,7 +236,7 @@ __device__ __forceinline__ void cell_gradient_corrected(const FvMesh& m, int c, 
                     dot = j == 0 ? gf * r[0] : dot + gf * r[j];
                 }
                 pf += dot;
-                if (p == c) {
+                if (mine) {
                     g0[i] += s0 * pf;
                     g1[i] += s1 * pf;
                     g2[i] += s2 * pf;
@@ -227,47 +249,47 @@ __device__ __forceinline__ void cell_gradient_corrected(const FvMesh& m, int c, 
         } else {
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
-                const double b = g(bv)[(f - m.nif) * stride + comp0 + i];
+                const double b = g(bv)[(-1 - o) * stride + i];
                 g0[i] += s0 * b;
                 g1[i] += s1 * b;
                 g2[i] += s2 * b;
             }
         }
     }
-    double* o = out + static_cast<long long>(c) * ostride + ooff;
+    double* out_c = out + static_cast<long long>(c) * ostride;
     const double v = g(m.vol)[c];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
-        o[3 * i] = g0[i] / v;
-        o[3 * i + 1] = g1[i] / v;
-        o[3 * i + 2] = g2[i] / v;
+        out_c[3 * i] = g0[i] / v;
+        out_c[3 * i + 1] = g1[i] / v;
+        out_c[3 * i + 2] = g2[i] / v;
     }
 }
 
 template <bool Lsq, int NC>
-__global__ void __launch_bounds__(k_fv_block) k_fv_gradient(FvMesh m, const double* phi, const double* bv, int stride,
+__global__ void __launch_bounds__(k_fv_block) k_fv_gradient(FvSlots m, const double* phi, const double* bv, int stride,
                                                             double* out, int ostride) {
     const int c = fv_cell();
     if (c < m.nc) {
-        cell_gradient<Lsq, NC>(m, c, phi, bv, stride, 0, out, ostride, 0);
+        cell_gradient<Lsq, NC>(m, c, phi, bv, stride, out, ostride);
     }
 }
 
 template <int NC>
-__global__ void __launch_bounds__(k_fv_block) k_fv_gradient_corrected(FvMesh m, const double* phi, const double* bv,
+__global__ void __launch_bounds__(k_fv_block) k_fv_gradient_corrected(FvSlots m, const double* phi, const double* bv,
                                                                       int stride, const double* lg, double* out,
                                                                       int ostride) {
     const int c = fv_cell();
     if (c < m.nc) {
-        cell_gradient_corrected<NC>(m, c, phi, bv, stride, 0, lg, out, ostride, 0);
+        cell_gradient_corrected<NC>(m, c, phi, bv, stride, lg, out, ostride);
     }
 }
 
-// Limiter of the three components of u (fv_cell.inc:123-164): bounds from the cell and its
-// face neighbours (boundary values for boundary faces), then per face and component the ratio
-// of the admissible change to the extrapolated one, Barth-Jespersen min(1, bound / d) or the
+// Limiter of the three components of u (fv_cell.inc:123-164): bounds from the cell and its face
+// neighbours (boundary values for boundary faces), then per face and component the ratio of the
+// admissible change to the extrapolated one, Barth-Jespersen min(1, bound / d) or the
 // Venkatakrishnan function with eps^2 per component; |d| < 1e-300 is skipped.
-__global__ void __launch_bounds__(k_fv_block) k_fv_limiter(FvMesh m, const double* u, const double* ub,
+__global__ void __launch_bounds__(k_fv_block) k_fv_limiter(FvSlots m, const double* u, const double* ub,
                                                            const double* grad, int venkat, double e0, double e1,
                                                            double e2, double* lim) {
     const int c = fv_cell();
@@ -285,17 +307,16 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_limiter(FvMesh m, const doubl
         lo[i] = uc[i];
         hi[i] = uc[i];
     }
-    const int k0 = g(m.start)[c];
-    const int k1 = g(m.start)[c + 1];
-    for (int k = k0; k < k1; ++k) {
-        const int f = g(m.faces)[k];
-        gptr<double> v;
-        if (f < m.nif) {
-            const int p = g(m.own)[f];
-            v = fu + 3 * (p == c ? g(m.nei)[f] : p);
-        } else {
-            v = g(ub) + 3 * (f - m.nif);
+    int first = 0;
+    int count = 0;
+    slot_range(m, c, first, count);
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        if (g(m.face)[t] < 0) {
+            break;
         }
+        const int o = g(m.other)[t];
+        const auto v = o >= 0 ? fu + 3 * o : g(ub) + 3 * (-1 - o);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const double x = v[i];
@@ -313,11 +334,14 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_limiter(FvMesh m, const doubl
     const double c0 = g(m.cc)[3 * c];
     const double c1 = g(m.cc)[3 * c + 1];
     const double c2 = g(m.cc)[3 * c + 2];
-    for (int k = k0; k < k1; ++k) {
-        const int f = g(m.faces)[k];
-        const double r0 = g(m.cf)[3 * f] - c0;
-        const double r1 = g(m.cf)[3 * f + 1] - c1;
-        const double r2 = g(m.cf)[3 * f + 2] - c2;
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        if (g(m.face)[t] < 0) {
+            break;
+        }
+        const double r0 = g(m.cf0)[t] - c0;
+        const double r1 = g(m.cf1)[t] - c1;
+        const double r2 = g(m.cf2)[t] - c2;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const double d = gm[3 * i] * r0 + gm[3 * i + 1] * r1 + gm[3 * i + 2] * r2;
@@ -341,9 +365,9 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_limiter(FvMesh m, const doubl
 }
 
 // Explicit internal-face fluxes of a scalar equation into source (fv_cell.inc:167-195):
-// e_f = -F_f g_up . (x_f - C_up) [linear upwind] + gamma_f k_f . (w g_P + (1 - w) g_N),
-// added for the owner and subtracted for the neighbour.
-__global__ void __launch_bounds__(k_fv_block) k_fv_scalar_corrections(FvMesh m, const double* grad, const double* flux,
+// e_f = -F_f g_up . (x_f - C_up) [linear upwind] + gamma_f k_f . (w g_P + (1 - w) g_N), added for
+// the owner and subtracted for the neighbour.
+__global__ void __launch_bounds__(k_fv_block) k_fv_scalar_corrections(FvSlots m, const double* grad, const double* flux,
                                                                       const double* gamma, int linear_upwind,
                                                                       double* source) {
     const int c = fv_cell();
@@ -353,29 +377,34 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_scalar_corrections(FvMesh m, 
     const auto gg = g(grad);
     const auto cc = g(m.cc);
     double s = 0.0;
-    const int k1 = g(m.start)[c + 1];
-    for (int k = g(m.start)[c]; k < k1; ++k) {
-        const int f = g(m.faces)[k];
-        if (f >= m.nif) {
+    int first = 0;
+    int count = 0;
+    slot_range(m, c, first, count);
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        const int o = g(m.other)[t];
+        const int f = g(m.face)[t];
+        if (f < 0 || o < 0) {
             break; // boundary faces follow the internal ones
         }
-        const int p = g(m.own)[f];
-        const int n = g(m.nei)[f];
+        const bool mine = g(m.owns)[t] != 0;
+        const int p = mine ? c : o;
+        const int n = mine ? o : c;
         const double fl = g(flux)[f];
         double e = 0.0;
         if (linear_upwind != 0) {
             const int up = fl >= 0.0 ? p : n;
-            const double r0 = g(m.cf)[3 * f] - cc[3 * up];
-            const double r1 = g(m.cf)[3 * f + 1] - cc[3 * up + 1];
-            const double r2 = g(m.cf)[3 * f + 2] - cc[3 * up + 2];
+            const double r0 = g(m.cf0)[t] - cc[3 * up];
+            const double r1 = g(m.cf1)[t] - cc[3 * up + 1];
+            const double r2 = g(m.cf2)[t] - cc[3 * up + 2];
             e -= fl * (gg[3 * up] * r0 + gg[3 * up + 1] * r1 + gg[3 * up + 2] * r2);
         }
-        const double wf = g(m.w)[f];
+        const double wf = g(m.w)[t];
         const double gf0 = wf * gg[3 * p] + (1.0 - wf) * gg[3 * n];
         const double gf1 = wf * gg[3 * p + 1] + (1.0 - wf) * gg[3 * n + 1];
         const double gf2 = wf * gg[3 * p + 2] + (1.0 - wf) * gg[3 * n + 2];
-        e += g(gamma)[f] * (g(m.kv)[3 * f] * gf0 + g(m.kv)[3 * f + 1] * gf1 + g(m.kv)[3 * f + 2] * gf2);
-        s = p == c ? s + e : s - e;
+        e += g(gamma)[f] * (g(m.kv0)[t] * gf0 + g(m.kv1)[t] * gf1 + g(m.kv2)[t] * gf2);
+        s = mine ? s + e : s - e;
     }
     source[c] += s;
 }
@@ -384,7 +413,7 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_scalar_corrections(FvMesh m, 
 // -F_f lim_up (g_up^T (x_f - C_up)) [linear upwind, blended with the linear face value when
 // central > 0, limited when lim is given] + nu_f (g_f^T k_f) [+ nu_f g_f S_f], g_f the
 // interpolated 3x3 gradient.
-__global__ void __launch_bounds__(k_fv_block) k_fv_momentum_corrections(FvMesh m, const double* grad, const double* lim,
+__global__ void __launch_bounds__(k_fv_block) k_fv_momentum_corrections(FvSlots m, const double* grad, const double* lim,
                                                                         int limited, const double* flux,
                                                                         const double* nu_face, int linear_upwind,
                                                                         double central, const double* u,
@@ -397,22 +426,27 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_momentum_corrections(FvMesh m
     const auto fu = g(u);
     const auto cc = g(m.cc);
     double s[3] = {0.0, 0.0, 0.0};
-    const int k1 = g(m.start)[c + 1];
-    for (int k = g(m.start)[c]; k < k1; ++k) {
-        const int f = g(m.faces)[k];
-        if (f >= m.nif) {
+    int first = 0;
+    int count = 0;
+    slot_range(m, c, first, count);
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        const int o = g(m.other)[t];
+        const int f = g(m.face)[t];
+        if (f < 0 || o < 0) {
             break;
         }
-        const int p = g(m.own)[f];
-        const int n = g(m.nei)[f];
+        const bool mine = g(m.owns)[t] != 0;
+        const int p = mine ? c : o;
+        const int n = mine ? o : c;
         const double fl = g(flux)[f];
-        const double wf = g(m.w)[f];
+        const double wf = g(m.w)[t];
         double e[3] = {0.0, 0.0, 0.0};
         if (linear_upwind != 0) {
             const int up = fl >= 0.0 ? p : n;
-            const double r0 = g(m.cf)[3 * f] - cc[3 * up];
-            const double r1 = g(m.cf)[3 * f + 1] - cc[3 * up + 1];
-            const double r2 = g(m.cf)[3 * f + 2] - cc[3 * up + 2];
+            const double r0 = g(m.cf0)[t] - cc[3 * up];
+            const double r1 = g(m.cf1)[t] - cc[3 * up + 1];
+            const double r2 = g(m.cf2)[t] - cc[3 * up + 2];
             const auto gu = gg + 9LL * up;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
@@ -433,17 +467,17 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_momentum_corrections(FvMesh m
             gf[j] = wf * gg[9LL * p + j] + (1.0 - wf) * gg[9LL * n + j];
         }
         const double nu = g(nu_face)[f];
-        const double k0 = g(m.kv)[3 * f];
-        const double kk1 = g(m.kv)[3 * f + 1];
-        const double k2 = g(m.kv)[3 * f + 2];
+        const double k0 = g(m.kv0)[t];
+        const double kk1 = g(m.kv1)[t];
+        const double k2 = g(m.kv2)[t];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             e[i] += nu * (gf[3 * i] * k0 + gf[3 * i + 1] * kk1 + gf[3 * i + 2] * k2);
         }
         if (transpose_term != 0) {
-            const double a0 = g(m.sf)[3 * f];
-            const double a1 = g(m.sf)[3 * f + 1];
-            const double a2 = g(m.sf)[3 * f + 2];
+            const double a0 = g(m.sf0)[t];
+            const double a1 = g(m.sf1)[t];
+            const double a2 = g(m.sf2)[t];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 e[i] += nu * (gf[i] * a0 + gf[i + 3] * a1 + gf[i + 6] * a2);
@@ -451,12 +485,80 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_momentum_corrections(FvMesh m
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            s[i] = p == c ? s[i] + e[i] : s[i] - e[i];
+            s[i] = mine ? s[i] + e[i] : s[i] - e[i];
         }
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         source[3 * c + i] += s[i];
+    }
+}
+
+// Fills the slots of every cell of every slice (cells past the last one get padding) from the
+// face-indexed mesh arrays; values are copied, never recomputed.
+struct FvFaceArrays {
+    int nc;
+    int nif;
+    const int* start;
+    const int* faces;
+    const int* own;
+    const int* nei;
+    const double* w;
+    const double* sf;
+    const double* lsq_d;
+    const double* cf;
+    const double* kv;
+};
+
+struct FvSlotOut {
+    int* face;
+    int* other;
+    unsigned char* owns;
+    double* w;
+    double* v[12]; // sf0..2, ld0..2, cf0..2, kv0..2
+};
+
+__global__ void __launch_bounds__(k_fv_block) k_fv_fill_slots(FvFaceArrays a, const int* base, int n_cells_padded,
+                                                              FvSlotOut o) {
+    const int c = static_cast<int>(blockIdx.x) * k_fv_block + static_cast<int>(threadIdx.x);
+    if (c >= n_cells_padded) {
+        return;
+    }
+    const int s = c >> 6;
+    const int first = base[s] + (c & 63);
+    const int count = (base[s + 1] - base[s]) >> 6;
+    const int k0 = c < a.nc ? a.start[c] : 0;
+    const int deg = c < a.nc ? a.start[c + 1] - k0 : 0;
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        if (k >= deg) {
+            o.face[t] = -1;
+            o.other[t] = 0;
+            o.owns[t] = 0;
+            o.w[t] = 0.0;
+            for (int j = 0; j < 12; ++j) {
+                o.v[j][t] = 0.0;
+            }
+            continue;
+        }
+        const int f = a.faces[k0 + k];
+        o.face[t] = f;
+        if (f < a.nif) {
+            const int p = a.own[f];
+            o.other[t] = p == c ? a.nei[f] : p;
+            o.owns[t] = p == c ? 1 : 0;
+            o.w[t] = a.w[f];
+        } else {
+            o.other[t] = -1 - (f - a.nif);
+            o.owns[t] = 1;
+            o.w[t] = 0.0;
+        }
+        for (int j = 0; j < 3; ++j) {
+            o.v[j][t] = a.sf[3 * f + j];
+            o.v[3 + j][t] = a.lsq_d[3 * f + j];
+            o.v[6 + j][t] = a.cf[3 * f + j];
+            o.v[9 + j][t] = a.kv[3 * f + j];
+        }
     }
 }
 
@@ -514,19 +616,63 @@ public:
             require(v.neighbour[k] >= 0 && v.neighbour[k] < nc_, "DeviceFv: neighbour out of range");
         }
         hipStream_t s = stream_.get();
-        own_.upload(v.owner, f, s);
-        const int zero_i = 0;
-        const double zero_d = 0.0;
-        nei_.upload(nif_ > 0 ? v.neighbour : &zero_i, at_least_one(nif_), s);
-        w_.upload(nif_ > 0 ? v.weight : &zero_d, at_least_one(nif_), s);
-        start_.upload(v.cell_offsets, c + 1, s);
-        faces_.upload(n_list > 0 ? v.cell_faces : &zero_i, at_least_one(n_list), s);
-        sf_.upload(v.sf, 3 * f, s);
-        cf_.upload(v.face_centre, 3 * f, s);
+        // Slice s (cells 64 s .. 64 s + 63) gets 64 x (most faces of its cells) slots.
+        n_slices_ = (nc_ + 63) / 64;
+        std::vector<int> base(static_cast<std::size_t>(n_slices_) + 1, 0);
+        long long total = 0;
+        for (int sl = 0; sl < n_slices_; ++sl) {
+            int width = 0;
+            for (int cell = 64 * sl; cell < std::min(nc_, 64 * sl + 64); ++cell) {
+                width = std::max(width, v.cell_offsets[cell + 1] - v.cell_offsets[cell]);
+            }
+            total += 64LL * width;
+            require(total <= 0x7fffffffLL, "DeviceFv: mesh too large for 32-bit face slots");
+            base[static_cast<std::size_t>(sl) + 1] = static_cast<int>(total);
+        }
+        const std::size_t slots = at_least_one(total);
+        base_.upload(base.data(), base.size(), s);
+        sface_.resize(slots);
+        sother_.resize(slots);
+        sowns_.resize(slots);
+        sw_.resize(slots);
+        for (DevBuf<double>& a : sv_) {
+            a.resize(slots);
+        }
+        {
+            // face-indexed copies, needed only to fill the slots
+            DevBuf<int> start;
+            DevBuf<int> faces;
+            DevBuf<int> own;
+            DevBuf<int> nei;
+            DevBuf<double> w;
+            DevBuf<double> sf;
+            DevBuf<double> lsq_d;
+            DevBuf<double> cf;
+            DevBuf<double> kv;
+            const int zero_i = 0;
+            const double zero_d = 0.0;
+            start.upload(v.cell_offsets, c + 1, s);
+            faces.upload(n_list > 0 ? v.cell_faces : &zero_i, at_least_one(n_list), s);
+            own.upload(v.owner, f, s);
+            nei.upload(nif_ > 0 ? v.neighbour : &zero_i, at_least_one(nif_), s);
+            w.upload(nif_ > 0 ? v.weight : &zero_d, at_least_one(nif_), s);
+            sf.upload(v.sf, 3 * f, s);
+            lsq_d.upload(v.lsq_d, 3 * f, s);
+            cf.upload(v.face_centre, 3 * f, s);
+            kv.upload(v.correction, 3 * f, s);
+            const FvFaceArrays a{nc_, nif_, start.get(), faces.get(), own.get(), nei.get(), w.get(),
+                                 sf.get(), lsq_d.get(), cf.get(), kv.get()};
+            FvSlotOut o{sface_.get(), sother_.get(), sowns_.get(), sw_.get(), {}};
+            for (int j = 0; j < 12; ++j) {
+                o.v[j] = sv_[static_cast<std::size_t>(j)].get();
+            }
+            const int padded = 64 * n_slices_;
+            k_fv_fill_slots<<<(padded + k_fv_block - 1) / k_fv_block, k_fv_block, 0, s>>>(a, base_.get(), padded, o);
+            hip_check(hipGetLastError(), "k_fv_fill_slots launch");
+            hip_check(hipStreamSynchronize(s), "DeviceFv slots sync"); // the copies go out of scope
+        }
         cc_.upload(v.cell_centre, 3 * c, s);
         vol_.upload(v.volume, c, s);
-        kv_.upload(v.correction, 3 * f, s);
-        lsq_d_.upload(v.lsq_d, 3 * f, s);
         lsq_.upload(v.lsq_inverse, 9 * c, s);
         const std::size_t nb = at_least_one(nf_ - nif_);
         phi_.resize(c);
@@ -628,17 +774,19 @@ private:
         require(scheme >= 0 && scheme <= 2, "DeviceFv: gradient scheme must be 0, 1 or 2");
     }
 
-    [[nodiscard]] FvMesh mesh() const {
-        return FvMesh{nc_,         nif_,         nf_,       own_.get(), nei_.get(), start_.get(),
-                      faces_.get(), w_.get(),    sf_.get(), cf_.get(),  cc_.get(),  vol_.get(),
-                      kv_.get(),    lsq_d_.get(), lsq_.get()};
+    [[nodiscard]] FvSlots mesh() const {
+        const auto v = [&](int j) { return static_cast<const double*>(sv_[static_cast<std::size_t>(j)].get()); };
+        return FvSlots{nc_,   nif_, base_.get(), sface_.get(), sother_.get(), sowns_.get(), sw_.get(),
+                       v(0),  v(1), v(2),        v(3),         v(4),          v(5),         v(6),
+                       v(7),  v(8), v(9),        v(10),        v(11),         cc_.get(),    vol_.get(),
+                       lsq_.get()};
     }
 
     // Gradient of NC interleaved components (fv_openmp.cpp:257-311): least squares, Green-Gauss,
     // or least squares into tmp followed by the corrected Gauss gradient.
     template <int NC>
     void gradient_loop(const double* phi, const double* bv, int scheme, double* out, int ostride) {
-        const FvMesh m = mesh();
+        const FvSlots m = mesh();
         if (scheme == 2) {
             double* tmp = tmp_.get();
             launch([&](hipStream_t s) {
@@ -701,9 +849,13 @@ private:
     int nc_ = 0;
     int nif_ = 0;
     int nf_ = 0;
+    int n_slices_ = 0;
     Stream stream_;
-    DevBuf<int> own_, nei_, start_, faces_;
-    DevBuf<double> w_, sf_, cf_, cc_, vol_, kv_, lsq_d_, lsq_;
+    DevBuf<int> base_, sface_, sother_;
+    DevBuf<unsigned char> sowns_;
+    DevBuf<double> sw_;
+    DevBuf<double> sv_[12]; // sf, lsq_d, face centre, correction: three components each
+    DevBuf<double> cc_, vol_, lsq_;
     DevBuf<double> phi_, bv_, sgrad_, vphi_, vbv_, vgrad_, lim_, flux_, coef_, src_, tmp_;
     Event e0_;
     Event e1_;
