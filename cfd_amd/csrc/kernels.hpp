@@ -85,9 +85,10 @@ __device__ __forceinline__ gmem_t<T>* gm(T* p) {
     return (gmem_t<T>*)p;
 }
 
-// Column index of slot k: 32-bit columns (ColAbs) or row + 16-bit offset (ColOff). The format
-// is chosen once per matrix (a uniform branch), never per slot: a per-lane pointer test would
-// turn every column load into a divergent branch with its own wait.
+// Column index of slot k: 32-bit columns (ColAbs), row + 16-bit offset (ColOff, square
+// matrices) or the slice's base column + unsigned 16-bit offset (ColBase, transfer operators).
+// The format is chosen once per matrix (a uniform branch), never per slot: a per-lane pointer
+// test would turn every column load into a divergent branch with its own wait.
 struct ColAbs {
     const gmem_t<const int>* __restrict__ col;
     __device__ __forceinline__ int operator()(int k) const { return col[k]; }
@@ -97,6 +98,12 @@ struct ColOff {
     const gmem_t<const short>* __restrict__ dcol;
     int row;
     __device__ __forceinline__ int operator()(int k) const { return row + static_cast<int>(dcol[k]); }
+};
+
+struct ColBase {
+    const gmem_t<const unsigned short>* __restrict__ dcol;
+    int base;
+    __device__ __forceinline__ int operator()(int k) const { return base + static_cast<int>(dcol[k]); }
 };
 
 template <class T, class V>
@@ -226,6 +233,9 @@ __device__ __forceinline__ T ell_row_f(const EllView& m, int slice, int lane, F 
         }
         return ell_sum<T, T>(ci, gm(static_cast<const T*>(m.val)) + base + lane, w, load);
     };
+    if (m.cbase != nullptr) {
+        return sum(ColBase{gm(reinterpret_cast<const unsigned short*>(m.dcol)) + base + lane, gm(m.cbase)[slice]});
+    }
     if (m.dcol != nullptr) {
         return sum(ColOff{gm(m.dcol) + base + lane, row});
     }

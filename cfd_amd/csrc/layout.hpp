@@ -71,7 +71,8 @@ struct EllView {
     int csr;         // 1: CSR rows, 0: sliced ELL
     const int* sp;   // ELL slice offsets or CSR row offsets
     const int* col;
-    const short* dcol;
+    const short* dcol;  // 16-bit columns: col = row + dcol, or col = cbase[slice] + (unsigned) dcol
+    const int* cbase;   // per slice: first column of the 16-bit window (non-square matrices)
     const void* val; // double, float or BF16 bits
     const float* rscale;
     const float* diag;

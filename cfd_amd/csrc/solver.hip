@@ -28,17 +28,22 @@ __global__ void __launch_bounds__(k_block) k_fill(int n, double* __restrict__ x,
     }
 }
 
+// Column formats of a sliced-ELL fill: 32-bit columns, 16-bit offsets column - row (square
+// matrices), or unsigned 16-bit offsets column - cbase[slice].
+enum ColFormat : int { COL_ABS = 0, COL_ROW = 1, COL_BASE = 2 };
+
 // Sliced-ELL fill from CSR, one thread per (slice, lane): slot sp[s] + 64 k + lane holds entry
 // k lanes + lane mod lanes of row s 64 / lanes + lane / lanes, or padding (value 0, a column the
-// row touches anyway), exactly as csr_to_ell. Short: 16-bit offsets column - row. src != nullptr:
-// also the CSR position map (LDU patterns); va may be null (values assembled later).
-template <bool Short>
+// row touches anyway, or the slice's base column), exactly as csr_to_ell. src != nullptr: also
+// the CSR position map (LDU patterns); va may be null (values assembled later).
+template <int Fmt>
 __global__ void __launch_bounds__(k_block) k_ell_fill(int n_rows, int n_cols, int n_slices, int lanes,
                                                       const int* __restrict__ sp, const int* __restrict__ rp,
                                                       const int* __restrict__ ci, const double* __restrict__ va,
                                                       const int* __restrict__ csr_src, int* __restrict__ col,
                                                       short* __restrict__ dcol, double* __restrict__ v64,
-                                                      float* __restrict__ v32, int* __restrict__ src) {
+                                                      float* __restrict__ v32, int* __restrict__ src,
+                                                      const int* __restrict__ cbase = nullptr) {
     const long long t = static_cast<long long>(blockIdx.x) * k_block + threadIdx.x;
     const int slice = static_cast<int>(t / k_wave);
     if (slice >= n_slices) {
@@ -55,13 +60,18 @@ __global__ void __launch_bounds__(k_block) k_ell_fill(int n_rows, int n_cols, in
     }
     const int base = sp[slice];
     const int w = (sp[slice + 1] - base) / k_wave;
+    if constexpr (Fmt == COL_BASE) {
+        pad = cbase[slice];
+    }
     for (int k = 0; k < w; ++k) {
         const int j = k * lanes + sub;
         const long long slot = static_cast<long long>(base) + static_cast<long long>(k) * k_wave + lane;
         const bool real = j < len;
         const int c = real ? ci[start + j] : pad;
-        if constexpr (Short) {
+        if constexpr (Fmt == COL_ROW) {
             dcol[slot] = static_cast<short>(c - r);
+        } else if constexpr (Fmt == COL_BASE) {
+            dcol[slot] = static_cast<short>(static_cast<unsigned short>(c - cbase[slice]));
         } else {
             col[slot] = c;
         }
@@ -73,6 +83,36 @@ __global__ void __launch_bounds__(k_block) k_ell_fill(int n_rows, int n_cols, in
         if (src != nullptr) {
             src[slot] = real ? csr_src[start + j] : -1;
         }
+    }
+}
+
+// Smallest and largest column of the real entries of each slice (0 for a slice with none); the
+// grid covers whole slices, so a wavefront exits as a whole.
+__global__ void __launch_bounds__(k_block) k_slice_cols(int n_rows, int n_slices, int lanes, const int* __restrict__ rp,
+                                                        const int* __restrict__ ci, int* __restrict__ lo,
+                                                        int* __restrict__ hi) {
+    const long long t = static_cast<long long>(blockIdx.x) * k_block + threadIdx.x;
+    const int slice = static_cast<int>(t / k_wave);
+    if (slice >= n_slices) {
+        return;
+    }
+    const int lane = static_cast<int>(t % k_wave);
+    const int r = slice * (k_wave / lanes) + lane / lanes;
+    int mn = 0x7fffffff;
+    int mx = -1;
+    if (r < n_rows) {
+        for (int j = rp[r] + lane % lanes; j < rp[r + 1]; j += lanes) {
+            mn = min(mn, ci[j]);
+            mx = max(mx, ci[j]);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, __shfl_xor(mn, off, k_wave));
+        mx = max(mx, __shfl_xor(mx, off, k_wave));
+    }
+    if (lane == 0) {
+        lo[slice] = mx < 0 ? 0 : mn;
+        hi[slice] = mx < 0 ? 0 : mx;
     }
 }
 
@@ -383,11 +423,11 @@ System::System(const cfdg_ldu_pattern& pt) {
         a0_src_.resize(slots);
         const int grid = ceil_div(static_cast<long long>(a0_.n_slices) * k_wave, k_block);
         if (short_cols) {
-            k_ell_fill<true><<<grid, k_block, 0, s>>>(n_, n_, a0_.n_slices, 1, a0_.sp.get(), rp_d.get(), ci_d.get(),
+            k_ell_fill<COL_ROW><<<grid, k_block, 0, s>>>(n_, n_, a0_.n_slices, 1, a0_.sp.get(), rp_d.get(), ci_d.get(),
                                                       nullptr, src_d.get(), nullptr, a0_.dcol.get(), nullptr,
                                                       nullptr, a0_src_.get());
         } else {
-            k_ell_fill<false><<<grid, k_block, 0, s>>>(n_, n_, a0_.n_slices, 1, a0_.sp.get(), rp_d.get(), ci_d.get(),
+            k_ell_fill<COL_ABS><<<grid, k_block, 0, s>>>(n_, n_, a0_.n_slices, 1, a0_.sp.get(), rp_d.get(), ci_d.get(),
                                                        nullptr, src_d.get(), a0_.col.get(), nullptr, nullptr,
                                                        nullptr, a0_src_.get());
         }
@@ -641,8 +681,37 @@ void DevEll::build_on_device(int rows, int cols, const int* rp, const int* ci, c
     rp_d.upload(rp, static_cast<std::size_t>(rows) + 1, s);
     ci_d.upload(ci, nnz, s);
     va_d.upload(va, nnz, s);
-    const bool short_cols = short_offsets_fit(rows, cols, rp, ci);
-    if (short_cols) {
+    const int grid = ceil_div(static_cast<long long>(n_slices) * k_wave, k_block);
+    // Square matrices: 16-bit offsets from the row when every column lies within 32767 of it.
+    // Others (transfer operators): unsigned 16-bit offsets from the slice's smallest column when
+    // every slice's columns span at most 65535.
+    int fmt = COL_ABS;
+    cbase.resize(0);
+    if (rows == cols) {
+        fmt = short_offsets_fit(rows, cols, rp, ci) ? COL_ROW : COL_ABS;
+    } else if (grid > 0 && std::getenv("CFDG_NO_SHORT_COLS") == nullptr) {
+        DevBuf<int> hi;
+        cbase.resize(static_cast<std::size_t>(n_slices));
+        hi.resize(static_cast<std::size_t>(n_slices));
+        k_slice_cols<<<grid, k_block, 0, s>>>(rows, n_slices, lanes, rp_d.get(), ci_d.get(), cbase.get(), hi.get());
+        hip_check(hipGetLastError(), "k_slice_cols launch");
+        std::vector<int> lo_h(static_cast<std::size_t>(n_slices));
+        std::vector<int> hi_h(static_cast<std::size_t>(n_slices));
+        hip_check(hipMemcpyAsync(lo_h.data(), cbase.get(), lo_h.size() * sizeof(int), hipMemcpyDeviceToHost, s),
+                  "D2H slice columns");
+        hip_check(hipMemcpyAsync(hi_h.data(), hi.get(), hi_h.size() * sizeof(int), hipMemcpyDeviceToHost, s),
+                  "D2H slice columns");
+        hip_check(hipStreamSynchronize(s), "slice columns sync");
+        bool fits = true;
+        for (std::size_t k = 0; k < lo_h.size() && fits; ++k) {
+            fits = hi_h[k] - lo_h[k] <= 65535;
+        }
+        fmt = fits ? COL_BASE : COL_ABS;
+        if (!fits) {
+            cbase.resize(0);
+        }
+    }
+    if (fmt != COL_ABS) {
         dcol.resize(slots);
         col.resize(0);
     } else {
@@ -651,16 +720,19 @@ void DevEll::build_on_device(int rows, int cols, const int* rp, const int* ci, c
     }
     v64.resize(slots);
     v32.resize(slots);
-    const int grid = ceil_div(static_cast<long long>(n_slices) * k_wave, k_block);
     if (grid > 0) {
-        if (short_cols) {
-            k_ell_fill<true><<<grid, k_block, 0, s>>>(rows, cols, n_slices, lanes, sp.get(), rp_d.get(), ci_d.get(),
-                                                      va_d.get(), nullptr, nullptr, dcol.get(), v64.get(),
-                                                      v32.get(), nullptr);
+        if (fmt == COL_ROW) {
+            k_ell_fill<COL_ROW><<<grid, k_block, 0, s>>>(rows, cols, n_slices, lanes, sp.get(), rp_d.get(),
+                                                         ci_d.get(), va_d.get(), nullptr, nullptr, dcol.get(),
+                                                         v64.get(), v32.get(), nullptr);
+        } else if (fmt == COL_BASE) {
+            k_ell_fill<COL_BASE><<<grid, k_block, 0, s>>>(rows, cols, n_slices, lanes, sp.get(), rp_d.get(),
+                                                          ci_d.get(), va_d.get(), nullptr, nullptr, dcol.get(),
+                                                          v64.get(), v32.get(), nullptr, cbase.get());
         } else {
-            k_ell_fill<false><<<grid, k_block, 0, s>>>(rows, cols, n_slices, lanes, sp.get(), rp_d.get(), ci_d.get(),
-                                                       va_d.get(), nullptr, col.get(), nullptr, v64.get(),
-                                                       v32.get(), nullptr);
+            k_ell_fill<COL_ABS><<<grid, k_block, 0, s>>>(rows, cols, n_slices, lanes, sp.get(), rp_d.get(),
+                                                         ci_d.get(), va_d.get(), nullptr, col.get(), nullptr,
+                                                         v64.get(), v32.get(), nullptr);
         }
         hip_check(hipGetLastError(), "k_ell_fill launch");
     }
