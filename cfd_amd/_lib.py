@@ -250,6 +250,7 @@ _SIGNATURES = {
     "cfdg_system_probe_read": (ST, [VP, C.c_int, P_DBL, C.POINTER(C.c_longlong), P_DBL]),
     "cfdg_amg_default_settings": (ST, [C.POINTER(AmgSettings)]),
     "cfdg_amg_build": (ST, [C.POINTER(Csr), C.POINTER(AmgSettings), C.POINTER(VP)]),
+    "cfdg_amg_build_device": (ST, [C.POINTER(Csr), C.POINTER(AmgSettings), C.POINTER(VP)]),
     "cfdg_amg_destroy": (ST, [VP]),
     "cfdg_amg_n_levels": (ST, [VP, P_INT]),
     "cfdg_amg_level_size": (ST, [VP, C.c_int, C.c_int, P_INT, P_INT, P_INT]),

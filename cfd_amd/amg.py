@@ -35,12 +35,14 @@ class AmgSettings:
 class AmgHierarchy:
     """Owns a cfdg_amg handle; exports levels as CsrView copies."""
 
-    def __init__(self, a: CsrView, settings: AmgSettings | None = None):
+    def __init__(self, a: CsrView, settings: AmgSettings | None = None, device: bool = False):
+        """device: the sparse products of the setup run on the GPU (cfdg_amg_build_device; the
+        hierarchy is bit-identical to the host setup's), the aggregation on the host."""
         self.settings = settings or AmgSettings()
         h = C.c_void_p()
         ca = a.c()
         cs = self.settings.c()
-        L.call("cfdg_amg_build", C.byref(ca), C.byref(cs), C.byref(h))
+        L.call("cfdg_amg_build_device" if device else "cfdg_amg_build", C.byref(ca), C.byref(cs), C.byref(h))
         self._h = h
 
     def __del__(self):

@@ -6,6 +6,7 @@
 // the first product, the order of Eigen's conservative sparse product, so values, exact zeros
 // and therefore the pruned patterns do not depend on the thread count. This file is compiled
 // with -ffp-contract=off: the x86-64 reference has no fused multiply-add either.
+#include "amg_device.hpp"
 #include "common.hpp"
 
 #include <algorithm>
@@ -468,9 +469,15 @@ std::vector<double> coarse_inverse(const Csr& a) {
     return inv;
 }
 
-void build(AmgHierarchy& h, Csr a0) {
+// The setup loop (amg.cpp:128-190). With `device`, the sparse products of every level run on
+// the GPU (DeviceSetup, bit-identical results) and only the aggregation stays here.
+void build(AmgHierarchy& h, Csr a0, bool device = false) {
     const cfdg_amg_settings& s = h.settings;
     h.levels.clear();
+    std::unique_ptr<DeviceSetup> dev;
+    if (device) {
+        dev = std::make_unique<DeviceSetup>(a0);
+    }
     Csr current = std::move(a0);
     for (int l = 0; l < s.max_levels; ++l) {
         Level level;
@@ -501,12 +508,17 @@ void build(AmgHierarchy& h, Csr a0) {
             rho = std::max(rho, row * std::abs(inv_diag[static_cast<std::size_t>(i)]));
         }
         const double omega = rho > 0.0 ? 4.0 / (3.0 * rho) : 0.0;
-        const Csr smoothed = smooth_tentative(current, inv_diag, agg, n_agg);
-        level.p = prolongation(smoothed, agg, omega);
-        level.r = transpose(level.p);
-        const Csr ap = spgemm(current, level.p);
-        Csr coarse = spgemm(level.r, ap);
-        prune_zeros(coarse);
+        Csr coarse;
+        if (dev) {
+            dev->level(inv_diag, agg, n_agg, omega, level.p, level.r, coarse);
+        } else {
+            const Csr smoothed = smooth_tentative(current, inv_diag, agg, n_agg);
+            level.p = prolongation(smoothed, agg, omega);
+            level.r = transpose(level.p);
+            const Csr ap = spgemm(current, level.p);
+            coarse = spgemm(level.r, ap);
+            prune_zeros(coarse);
+        }
         level.agg = std::move(agg);
         h.levels.push_back(std::move(level));
         current = std::move(coarse);
@@ -589,8 +601,8 @@ extern "C" cfdg_status cfdg_amg_default_settings(cfdg_amg_settings* s) {
     });
 }
 
-extern "C" cfdg_status cfdg_amg_build(const cfdg_csr* a, const cfdg_amg_settings* settings,
-                                      cfdg_amg** out) {
+namespace {
+cfdg_status amg_build(const cfdg_csr* a, const cfdg_amg_settings* settings, cfdg_amg** out, bool device) {
     return guarded([&] {
         require(a != nullptr && out != nullptr, "null argument");
         *out = nullptr;
@@ -608,11 +620,22 @@ extern "C" cfdg_status cfdg_amg_build(const cfdg_csr* a, const cfdg_amg_settings
         const auto t0 = std::chrono::steady_clock::now();
         Csr c = copy_csr(*a, "amg setup: invalid CSR matrix");
         sort_rows(c);
-        build(amg->h, std::move(c));
+        build(amg->h, std::move(c), device);
         const auto t1 = std::chrono::steady_clock::now();
         amg->h.setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         *out = amg.release();
     });
+}
+} // namespace
+
+extern "C" cfdg_status cfdg_amg_build(const cfdg_csr* a, const cfdg_amg_settings* settings,
+                                      cfdg_amg** out) {
+    return amg_build(a, settings, out, false);
+}
+
+extern "C" cfdg_status cfdg_amg_build_device(const cfdg_csr* a, const cfdg_amg_settings* settings,
+                                             cfdg_amg** out) {
+    return amg_build(a, settings, out, true);
 }
 
 extern "C" cfdg_status cfdg_amg_destroy(cfdg_amg* amg) {

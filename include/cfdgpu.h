@@ -409,6 +409,12 @@ typedef struct cfdg_amg_settings {
 cfdg_status cfdg_amg_default_settings(cfdg_amg_settings* s);
 cfdg_status cfdg_amg_build(const cfdg_csr* a, const cfdg_amg_settings* settings,
                            cfdg_amg** out);
+/* Replaces the same setup (amg.cpp:122-190) with the sparse products of every level (smoothed
+ * prolongation, transpose, Galerkin product, pruning) on the GPU and the greedy aggregation
+ * (amg.cpp:12-78, sequential by definition) on the host. The hierarchy is bit-identical to
+ * cfdg_amg_build's; CFDG_ERR_NO_DEVICE without a GPU. */
+cfdg_status cfdg_amg_build_device(const cfdg_csr* a, const cfdg_amg_settings* settings,
+                                  cfdg_amg** out);
 cfdg_status cfdg_amg_destroy(cfdg_amg* amg);
 cfdg_status cfdg_amg_n_levels(const cfdg_amg* amg, int* n_levels);
 /* which: 0 = a, 1 = p, 2 = r. Sizes of a level's matrix (nnz = 0 on the coarsest p / r). */

@@ -1,0 +1,52 @@
+"""Smoothed-aggregation setup with the sparse products on the GPU (cfdg_amg_build_device)
+against the host setup (cfdg_amg_build, pinned to the reference by tests/test_oracle_pins.py and
+the golden fixtures): every level's operator, prolongation, restriction and aggregates and the
+coarsest inverse are bit-identical, so the hierarchy, and every solve with it, is the same.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cfd_amd import AmgHierarchy, AmgSettings, cases
+
+pytestmark = pytest.mark.gpu
+
+
+def same_hierarchy(h, d):
+    assert h.n_levels == d.n_levels
+    for l in range(h.n_levels):
+        for which in (0, 1, 2):
+            if which and l == h.n_levels - 1:
+                continue
+            a, b = h.matrix(l, which), d.matrix(l, which)
+            assert a.n == b.n and a.nnz == b.nnz, (l, which)
+            assert np.array_equal(a.row_ptr, b.row_ptr), (l, which)
+            assert np.array_equal(a.col, b.col), (l, which)
+            assert np.array_equal(a.val, b.val), (l, which)
+    assert np.array_equal(h.view().coarse_inverse, d.view().coarse_inverse)
+
+
+@pytest.mark.parametrize("shape,jitter,decay", [((24, 20, 18), 0.2, 0.5), ((40, 40, 40), 0.0, 1.0),
+                                                 ((64, 48, 30), 0.25, 0.5), ((7, 5, 3), 0.0, 0.5)])
+def test_device_setup_matches_host_setup(shape, jitter, decay):
+    mesh = cases.box_mesh(*shape, hi=(1.0, 1.0, shape[2] / shape[0]), jitter=jitter, seed=sum(shape))
+    s = cases.pressure_system(mesh, rhs_seed=3)
+    pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
+    a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
+    settings = AmgSettings(strength_decay=decay)
+    h = AmgHierarchy(a, settings)
+    d = AmgHierarchy(a, settings, device=True)
+    print(f"{shape}: levels {h.level_sizes()}, host {h.setup_ms:.1f} ms, device {d.setup_ms:.1f} ms", flush=True)
+    same_hierarchy(h, d)
+
+
+def test_device_setup_of_a_variable_coefficient_operator():
+    mesh = cases.box_mesh(30, 30, 30, jitter=0.2, seed=9)
+    rng = np.random.default_rng(2)
+    kappa = np.exp(rng.uniform(-3.0, 3.0, mesh.n_cells))  # strongly varying cell coefficients
+    s = cases.pressure_system(mesh, kappa=kappa, rhs_seed=4)
+    pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
+    a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
+    settings = AmgSettings(strength_decay=0.5)
+    same_hierarchy(AmgHierarchy(a, settings), AmgHierarchy(a, settings, device=True))
