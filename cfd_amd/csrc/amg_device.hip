@@ -7,8 +7,10 @@
 // product before it): entry (i, j) is the sum of a_ik b_kj over k in the storage order of row i
 // of A, starting from the first product, explicit zeros kept. On the device the products of a
 // row are written in that order with the key (i, j), a stable radix sort by key brings equal keys
-// together without reordering them, and one thread per key adds its run in order. The result is
-// bit-identical to the host products for any matrix, so the pruned patterns are identical too.
+// together without reordering them, and one thread per key adds its run in order; small levels
+// whose rows have long product lists (stalled coarsening) accumulate each row densely instead,
+// exactly as the host does. The result is bit-identical to the host products for any matrix, so
+// the pruned patterns are identical too.
 // The operator of the current level stays on the device from one level to the next.
 #include "amg_device.hpp"
 #include "hip_util.hpp"
@@ -239,6 +241,62 @@ __global__ void k_fill_one(int n, double* __restrict__ a) {
     }
 }
 
+// Dense-row products for small, dense levels: thread i accumulates row i of A B in its own
+// n_cols-wide slice of scratch (mark: first touch), in the order of the host setup, then counts
+// (rp == nullptr) or writes its entries with ascending columns.
+__global__ void k_dense_rows(int n, int n_cols, const int* __restrict__ arp, const int* __restrict__ aci,
+                             const double* __restrict__ ava, const int* __restrict__ brp, const int* __restrict__ bci,
+                             const double* __restrict__ bva, const double* __restrict__ lscale,
+                             double* __restrict__ acc_all, int* __restrict__ mark_all, long long* __restrict__ cnt) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    double* acc = acc_all + static_cast<long long>(i) * n_cols;
+    int* mark = mark_all + static_cast<long long>(i) * n_cols;
+    for (int j = 0; j < n_cols; ++j) {
+        mark[j] = 0;
+    }
+    const double s = lscale != nullptr ? lscale[i] : 1.0;
+    for (int ka = arp[i]; ka < arp[i + 1]; ++ka) {
+        const int k = aci[ka];
+        const double y = lscale != nullptr ? s * ava[ka] : ava[ka];
+        for (int kb = brp[k]; kb < brp[k + 1]; ++kb) {
+            const int j = bci[kb];
+            const double prod = bva[kb] * y;
+            if (mark[j] == 0) {
+                mark[j] = 1;
+                acc[j] = prod;
+            } else {
+                acc[j] += prod;
+            }
+        }
+    }
+    long long c = 0;
+    for (int j = 0; j < n_cols; ++j) {
+        c += mark[j];
+    }
+    cnt[i] = c;
+}
+
+__global__ void k_dense_write(int n, int n_cols, const double* __restrict__ acc_all, const int* __restrict__ mark_all,
+                              const long long* __restrict__ off, Key* __restrict__ key, double* __restrict__ val) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const double* acc = acc_all + static_cast<long long>(i) * n_cols;
+    const int* mark = mark_all + static_cast<long long>(i) * n_cols;
+    long long q = off[i];
+    for (int j = 0; j < n_cols; ++j) {
+        if (mark[j] != 0) {
+            key[q] = make_key(i, j);
+            val[q] = acc[j];
+            ++q;
+        }
+    }
+}
+
 template <class T>
 void exclusive_sum(const T* in, T* out, long long n, hipStream_t s) {
     std::size_t bytes = 0;
@@ -370,6 +428,25 @@ void spgemm(const DCsr& a, const DCsr& b, const double* lscale, bool prune, DCsr
     hip_check(hipGetLastError(), "count launch");
     exclusive_sum(cnt.get(), off.get(), n, s);
     const long long total = last_of(off.get(), cnt.get(), n, s);
+    // Small level with long product lists: dense accumulation per row instead of sorting them.
+    const long long cells = static_cast<long long>(n) * b.n_cols;
+    if (cells <= (1LL << 25) && total >= 32LL * n) {
+        DevBuf<double> acc(static_cast<std::size_t>(cells > 0 ? cells : 1));
+        DevBuf<int> mark(static_cast<std::size_t>(cells > 0 ? cells : 1));
+        k_dense_rows<<<blocks_for(n), k_blk, 0, s>>>(n, b.n_cols, a.rp.get(), a.ci.get(), a.va.get(), b.rp.get(),
+                                                     b.ci.get(), b.va.get(), lscale, acc.get(), mark.get(), cnt.get());
+        hip_check(hipGetLastError(), "dense rows launch");
+        exclusive_sum(cnt.get(), off.get(), n, s);
+        const long long kept = last_of(off.get(), cnt.get(), n, s);
+        DevBuf<Key> dkey(static_cast<std::size_t>(kept > 0 ? kept : 1));
+        DevBuf<double> dval(static_cast<std::size_t>(kept > 0 ? kept : 1));
+        k_dense_write<<<blocks_for(n), k_blk, 0, s>>>(n, b.n_cols, acc.get(), mark.get(), off.get(), dkey.get(),
+                                                      dval.get());
+        hip_check(hipGetLastError(), "dense write launch");
+        hip_check(hipStreamSynchronize(s), "dense rows sync"); // the scratch goes back to the pool
+        entries_to_csr(dkey, dval, static_cast<int>(kept), n, b.n_cols, prune, c, s);
+        return;
+    }
     require(total < static_cast<long long>(INT_MAX), "device AMG setup: too many products for one level");
     DevBuf<Key> key(static_cast<std::size_t>(total > 0 ? total : 1));
     DevBuf<double> val(static_cast<std::size_t>(total > 0 ? total : 1));
