@@ -170,6 +170,7 @@ void DevEll::upload(const HostEll& e, const double* va, bool fp64, bool fp32, hi
         short_ok = fits != 0;
     }
     cbase.resize(0); // host conversion: row-relative offsets for square matrices only
+    soff.resize(0);
     if (short_ok && !std::getenv("CFDG_NO_SHORT_COLS")) {
         dcol.upload(d.data(), d.size(), s);
         col.resize(0);
@@ -247,6 +248,7 @@ void DevEll::upload_csr(int rows, int cols, const int* rp, const int* ci, const 
     sp.upload(rp, static_cast<std::size_t>(rows) + 1, s);
     dcol.resize(0); // CSR rows always carry 32-bit columns
     cbase.resize(0);
+    soff.resize(0);
     col.upload(ci, nnz, s);
     v64.upload(va, nnz, s);
     std::vector<float> f(nnz);

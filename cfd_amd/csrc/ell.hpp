@@ -72,6 +72,7 @@ struct DevEll {
     DevBuf<int> col;
     DevBuf<short> dcol;          // 16-bit column offsets (replaces col when present)
     DevBuf<int> cbase;           // per slice, non-square matrices: dcol holds col - cbase (unsigned)
+    DevBuf<int> soff;            // per-slice stencil offsets (replaces col / dcol when present)
     DevBuf<double> v64;
     DevBuf<float> v32;
     DevBuf<unsigned short> vbf;  // BF16 values scaled per row (diagonal slot 0)
@@ -97,6 +98,7 @@ struct DevEll {
         e.col = col.size() > 0 ? col.get() : nullptr;
         e.dcol = dcol.size() > 0 ? dcol.get() : nullptr;
         e.cbase = cbase.size() > 0 ? cbase.get() : nullptr;
+        e.soff = soff.size() > 0 ? soff.get() : nullptr;
         e.val = v;
         e.rscale = rscale.get();
         e.diag = diag32.get();
@@ -106,7 +108,7 @@ struct DevEll {
     [[nodiscard]] EllView view32() const { return view(v32.get()); }
     [[nodiscard]] EllView viewbf() const { return view(vbf.get(), 1); }
     [[nodiscard]] std::size_t bytes() const {
-        return sp.bytes() + col.bytes() + dcol.bytes() + cbase.bytes() + v64.bytes() + v32.bytes() +
+        return sp.bytes() + col.bytes() + dcol.bytes() + cbase.bytes() + soff.bytes() + v64.bytes() + v32.bytes() +
                vbf.bytes() + rscale.bytes() + diag32.bytes();
     }
 };

@@ -86,7 +86,8 @@ __device__ __forceinline__ gmem_t<T>* gm(T* p) {
 }
 
 // Column index of slot k: 32-bit columns (ColAbs), row + 16-bit offset (ColOff, square
-// matrices) or the slice's base column + unsigned 16-bit offset (ColBase, transfer operators).
+// matrices), the slice's base column + unsigned 16-bit offset (ColBase, transfer operators) or
+// row + the slice's stencil offset of round k (ColStencil, square matrices, one lane per row).
 // The format is chosen once per matrix (a uniform branch), never per slot: a per-lane pointer
 // test would turn every column load into a divergent branch with its own wait.
 struct ColAbs {
@@ -106,6 +107,15 @@ struct ColBase {
     __device__ __forceinline__ int operator()(int k) const { return base + static_cast<int>(dcol[k]); }
 };
 
+// Stencil column: the slice's offset table (uniform over the wavefront) added to the row,
+// clamped to [0, last] for the value-0 slots of rows that lack an offset.
+struct ColStencil {
+    const gmem_t<const int>* __restrict__ off;
+    int row;
+    int last;
+    __device__ __forceinline__ int operator()(int k) const { return min(max(row + off[k / k_wave], 0), last); }
+};
+
 template <class T, class V>
 __device__ __forceinline__ T value_at(const gmem_t<const V>* __restrict__ val, int i) {
     if constexpr (sizeof(V) == 2) {
@@ -115,50 +125,64 @@ __device__ __forceinline__ T value_at(const gmem_t<const V>* __restrict__ val, i
     }
 }
 
+#ifndef CFDG_ELL_GROUP
+#define CFDG_ELL_GROUP 4
+#endif
+constexpr int k_ell_group = CFDG_ELL_GROUP;
+
 // Storage-order row sum of sliced-ELL slots with values val (type V) converted to T. Loads are
-// issued four entries at a time, the tail (w mod 4, uniform over the slice's wavefront) as one
-// more group with its unused loads repeating a valid slot; only real entries are summed.
+// issued k_ell_group entries at a time, the tail (w mod k_ell_group, uniform over the slice's
+// wavefront) as one more group with its unused loads repeating a valid slot; only real entries
+// are summed.
 template <class T, class V, class C, class F>
 __device__ __forceinline__ T ell_sum(const C& ci, const gmem_t<const V>* __restrict__ val, int w, F load) {
+    constexpr int G = k_ell_group;
     T s = T(0);
     int k = 0;
-    for (; k + 4 <= w; k += 4) {
-        const int c0 = ci((k + 0) * k_wave);
-        const int c1 = ci((k + 1) * k_wave);
-        const int c2 = ci((k + 2) * k_wave);
-        const int c3 = ci((k + 3) * k_wave);
-        const T a0 = value_at<T, V>(val, (k + 0) * k_wave);
-        const T a1 = value_at<T, V>(val, (k + 1) * k_wave);
-        const T a2 = value_at<T, V>(val, (k + 2) * k_wave);
-        const T a3 = value_at<T, V>(val, (k + 3) * k_wave);
-        const T x0 = load(c0);
-        const T x1 = load(c1);
-        const T x2 = load(c2);
-        const T x3 = load(c3);
-        s = madd(a0, x0, s);
-        s = madd(a1, x1, s);
-        s = madd(a2, x2, s);
-        s = madd(a3, x3, s);
+    for (; k + G <= w; k += G) {
+        int c[G];
+        T a[G];
+        T x[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            c[g] = ci((k + g) * k_wave);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            a[g] = value_at<T, V>(val, (k + g) * k_wave);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            x[g] = load(c[g]);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            s = madd(a[g], x[g], s);
+        }
     }
     const int rem = w - k;
     if (rem > 0) {
-        const int k1 = rem > 1 ? k + 1 : k;
-        const int k2 = rem > 2 ? k + 2 : k;
-        const int c0 = ci(k * k_wave);
-        const int c1 = ci(k1 * k_wave);
-        const int c2 = ci(k2 * k_wave);
-        const T a0 = value_at<T, V>(val, k * k_wave);
-        const T a1 = value_at<T, V>(val, k1 * k_wave);
-        const T a2 = value_at<T, V>(val, k2 * k_wave);
-        const T x0 = load(c0);
-        const T x1 = load(c1);
-        const T x2 = load(c2);
-        s = madd(a0, x0, s);
-        if (rem > 1) {
-            s = madd(a1, x1, s);
+        int c[G - 1];
+        T a[G - 1];
+        T x[G - 1];
+#pragma unroll
+        for (int g = 0; g < G - 1; ++g) {
+            c[g] = ci((g < rem ? k + g : k) * k_wave);
         }
-        if (rem > 2) {
-            s = madd(a2, x2, s);
+#pragma unroll
+        for (int g = 0; g < G - 1; ++g) {
+            a[g] = value_at<T, V>(val, (g < rem ? k + g : k) * k_wave);
+        }
+#pragma unroll
+        for (int g = 0; g < G - 1; ++g) {
+            x[g] = load(c[g]);
+        }
+        s = madd(a[0], x[0], s);
+#pragma unroll
+        for (int g = 1; g < G - 1; ++g) {
+            if (g < rem) {
+                s = madd(a[g], x[g], s);
+            }
         }
     }
     return s;
@@ -233,6 +257,10 @@ __device__ __forceinline__ T ell_row_f(const EllView& m, int slice, int lane, F 
         }
         return ell_sum<T, T>(ci, gm(static_cast<const T*>(m.val)) + base + lane, w, load);
     };
+    if (m.soff != nullptr) {
+        const int tab = __builtin_amdgcn_readfirstlane(base / k_wave); // uniform: one slice per wavefront
+        return sum(ColStencil{gm(m.soff) + tab, row, m.n_rows - 1});
+    }
     if (m.cbase != nullptr) {
         return sum(ColBase{gm(reinterpret_cast<const unsigned short*>(m.dcol)) + base + lane, gm(m.cbase)[slice]});
     }

@@ -60,7 +60,10 @@ struct SolveState {
 // a column the row touches anyway). With tpr == 1 a lane owns a row and sums it in CSR order.
 // CSR (csr == 1, ptr = row offsets): lane j of a row's group sums its entries j, j + tpr, ...
 // Both give the same sums for the same tpr. dcol (if not null) replaces col by 16-bit offsets
-// column - row; fmt 1 stores the values as BF16 scaled per row (a_ij = rscale_i * bf16_ij) with
+// column - row. soff (if not null; square, tpr == 1) replaces both by a per-slice stencil: round
+// k of every row of slice s reads column row + soff[sp[s] / 64 + k] (clamped to the matrix), the
+// slice's distinct offsets in ascending order, so a row's entries keep their CSR order and the
+// offsets a row lacks hold value 0. fmt 1 stores the values as BF16 scaled per row (a_ij = rscale_i * bf16_ij) with
 // the diagonal slot zero and the diagonal in FP32 in diag (rscale and diag padded to whole
 // slices; tpr == 1 only).
 struct EllView {
@@ -73,6 +76,7 @@ struct EllView {
     const int* col;
     const short* dcol;  // 16-bit columns: col = row + dcol, or col = cbase[slice] + (unsigned) dcol
     const int* cbase;   // per slice: first column of the 16-bit window (non-square matrices)
+    const int* soff;    // per-slice stencil offsets (square matrices, one lane per row)
     const void* val; // double, float or BF16 bits
     const float* rscale;
     const float* diag;

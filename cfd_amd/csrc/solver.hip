@@ -9,6 +9,8 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <climits>
+#include <limits>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -116,6 +118,118 @@ __global__ void __launch_bounds__(k_block) k_slice_cols(int n_rows, int n_slices
     }
 }
 
+// Widest per-slice stencil tried; wider unions keep the other column formats.
+constexpr int k_max_stencil = 32;
+
+// Per-slice stencil of a square matrix with one lane per row: U_s, the ascending union over the
+// slice's rows of the offsets column - row. Count pass (Write false): width[s] = |U_s|, or -1 when
+// a row's columns do not ascend strictly or |U_s| > k_max_stencil. Write pass (sp from those
+// widths): soff[sp[s] / 64 + k] = U_s[k], and slot sp[s] + 64 k + lane holds the row's entry of
+// offset U_s[k] (value, or CSR position map through csr_src), else 0 (map -1). A wavefront owns a
+// slice and takes the offsets in ascending order by a wavefront minimum, so a row's entries keep
+// their CSR order. The grid covers whole slices.
+template <bool Write>
+__global__ void __launch_bounds__(k_block) k_slice_stencil(int n_rows, int n_slices, const int* __restrict__ rp,
+                                                           const int* __restrict__ ci, int* __restrict__ width,
+                                                           const int* __restrict__ sp, int* __restrict__ soff,
+                                                           const double* __restrict__ va,
+                                                           const int* __restrict__ csr_src, double* __restrict__ v64,
+                                                           float* __restrict__ v32, int* __restrict__ src) {
+    const long long t = static_cast<long long>(blockIdx.x) * k_block + threadIdx.x;
+    const int slice = static_cast<int>(t / k_wave);
+    if (slice >= n_slices) {
+        return;
+    }
+    const int lane = static_cast<int>(t % k_wave);
+    const int r = slice * k_wave + lane;
+    int j = r < n_rows ? rp[r] : 0;
+    const int end = r < n_rows ? rp[r + 1] : 0;
+    if constexpr (!Write) {
+        bool ascending = true;
+        for (int q = j + 1; q < end; ++q) {
+            ascending = ascending && ci[q] > ci[q - 1];
+        }
+        if (!__all(ascending)) {
+            if (lane == 0) {
+                width[slice] = -1;
+            }
+            return;
+        }
+    }
+    const int w = Write ? (sp[slice + 1] - sp[slice]) / k_wave : k_max_stencil;
+    const long long base = Write ? sp[slice] : 0;
+    int k = 0;
+    for (;;) {
+        const int mine = j < end ? ci[j] - r : INT_MAX;
+        int m = mine;
+        for (int off = 32; off > 0; off >>= 1) {
+            m = min(m, __shfl_xor(m, off, k_wave));
+        }
+        if (m == INT_MAX || k >= w) { // uniform over the wavefront
+            break;
+        }
+        const bool real = mine == m;
+        if constexpr (Write) {
+            if (lane == 0) {
+                soff[base / k_wave + k] = m;
+            }
+            const long long slot = base + static_cast<long long>(k) * k_wave + lane;
+            if (va != nullptr) {
+                const double v = real ? va[j] : 0.0;
+                v64[slot] = v;
+                v32[slot] = static_cast<float>(v);
+            }
+            if (src != nullptr) {
+                src[slot] = real ? csr_src[j] : -1;
+            }
+        }
+        j += real ? 1 : 0;
+        ++k;
+    }
+    if constexpr (!Write) {
+        const bool more = __any(j < end);
+        if (lane == 0) {
+            width[slice] = more ? -1 : k;
+        }
+    }
+}
+
+// Slice offsets of the per-slice stencil layout of a square matrix with one lane per row (rp_d,
+// ci_d: its CSR arrays on the device), or empty when a slice has none, the stencil slots exceed
+// the sliced-ELL slots (ell_slots) by more than 10%, or CFDG_NO_STENCIL is set.
+std::vector<int> stencil_offsets(int rows, const int* rp_d, const int* ci_d, long long ell_slots, hipStream_t s) {
+    const int n_slices = (rows + k_wave - 1) / k_wave;
+    if (n_slices == 0 || std::getenv("CFDG_NO_STENCIL") != nullptr) {
+        return {};
+    }
+    DevBuf<int> width;
+    width.resize(static_cast<std::size_t>(n_slices));
+    const int grid = ceil_div(static_cast<long long>(n_slices) * k_wave, k_block);
+    k_slice_stencil<false><<<grid, k_block, 0, s>>>(rows, n_slices, rp_d, ci_d, width.get(), nullptr, nullptr, nullptr,
+                                                    nullptr, nullptr, nullptr, nullptr);
+    hip_check(hipGetLastError(), "k_slice_stencil launch");
+    std::vector<int> w(static_cast<std::size_t>(n_slices));
+    hip_check(hipMemcpyAsync(w.data(), width.get(), w.size() * sizeof(int), hipMemcpyDeviceToHost, s),
+              "D2H stencil widths");
+    hip_check(hipStreamSynchronize(s), "stencil widths sync");
+    std::vector<int> sp(static_cast<std::size_t>(n_slices) + 1, 0);
+    long long total = 0;
+    for (int k = 0; k < n_slices; ++k) {
+        if (w[static_cast<std::size_t>(k)] < 0) {
+            return {};
+        }
+        total += static_cast<long long>(w[static_cast<std::size_t>(k)]) * k_wave;
+        if (total > std::numeric_limits<int>::max()) {
+            return {};
+        }
+        sp[static_cast<std::size_t>(k) + 1] = static_cast<int>(total);
+    }
+    if (total == 0 || static_cast<double>(total) > 1.1 * static_cast<double>(ell_slots)) {
+        return {};
+    }
+    return sp;
+}
+
 __device__ __forceinline__ unsigned short float_to_bf16(float f) { // round to nearest even
     unsigned u = __float_as_uint(f);
     u += 0x7FFFu + ((u >> 16) & 1u);
@@ -141,6 +255,9 @@ __global__ void __launch_bounds__(k_block) k_make_bf16(EllView m, unsigned short
     const int w = (m.sp[slice + 1] - base) / k_wave;
     // Column of slot k (setup kernel, run once per assembly: a per-slot test is fine here).
     auto column = [&](int k) {
+        if (m.soff != nullptr) {
+            return row + m.soff[base / k_wave + k / k_wave];
+        }
         return m.dcol != nullptr ? row + static_cast<int>(m.dcol[base + lane + k]) : m.col[base + lane + k];
     };
     const double* val = static_cast<const double*>(m.val) + base + lane;
@@ -412,6 +529,28 @@ System::System(const cfdg_ldu_pattern& pt) {
         rp_d.upload(pt.row_ptr, static_cast<std::size_t>(n_) + 1, s);
         ci_d.upload(pt.col, static_cast<std::size_t>(pt.nnz), s);
         src_d.upload(csr_src.data(), csr_src.size(), s);
+        const int grid = ceil_div(static_cast<long long>(a0_.n_slices) * k_wave, k_block);
+        const std::vector<int> st = stencil_offsets(n_, rp_d.get(), ci_d.get(), offsets.back(), s);
+        a0_.soff.resize(0);
+        a0_.cbase.resize(0);
+        if (!st.empty()) {
+            const std::size_t st_slots = static_cast<std::size_t>(st.back());
+            a0_.sp.upload(st.data(), st.size(), s);
+            a0_.soff.resize(st_slots / k_wave);
+            a0_.col.resize(0);
+            a0_.dcol.resize(0);
+            a0_src_.resize(st_slots);
+            k_slice_stencil<true><<<grid, k_block, 0, s>>>(n_, a0_.n_slices, rp_d.get(), ci_d.get(), nullptr,
+                                                           a0_.sp.get(), a0_.soff.get(), nullptr, src_d.get(),
+                                                           nullptr, nullptr, a0_src_.get());
+            hip_check(hipGetLastError(), "k_slice_stencil launch");
+            hip_check(hipStreamSynchronize(s), "pattern build sync");
+            a0_.v64.resize(st_slots);
+            a0_.v32.resize(st_slots);
+            ldu_.resize(static_cast<std::size_t>(n_) + 2 * static_cast<std::size_t>(nf_));
+            init_vectors();
+            return;
+        }
         const bool short_cols = short_offsets_fit(n_, n_, pt.row_ptr, pt.col);
         if (short_cols) {
             a0_.dcol.resize(slots);
@@ -421,7 +560,6 @@ System::System(const cfdg_ldu_pattern& pt) {
             a0_.dcol.resize(0);
         }
         a0_src_.resize(slots);
-        const int grid = ceil_div(static_cast<long long>(a0_.n_slices) * k_wave, k_block);
         if (short_cols) {
             k_ell_fill<COL_ROW><<<grid, k_block, 0, s>>>(n_, n_, a0_.n_slices, 1, a0_.sp.get(), rp_d.get(), ci_d.get(),
                                                       nullptr, src_d.get(), nullptr, a0_.dcol.get(), nullptr,
@@ -447,9 +585,12 @@ System::System(const cfdg_csr& a) {
     n_ = a.n_rows;
     nnz0_ = a.nnz;
     nf_ = 0;
-    const HostEll e = csr_to_ell(n_, n_, a.row_ptr, a.col);
     hipStream_t s = stream_.get();
-    a0_.upload(e, a.val, true, true, s);
+    if (std::getenv("CFDG_HOST_ELL") != nullptr) { // host conversion (same layout), for checks
+        a0_.upload(csr_to_ell(n_, n_, a.row_ptr, a.col), a.val, true, true, s);
+    } else {
+        a0_.build_on_device(n_, n_, a.row_ptr, a.col, a.val, 1, s);
+    }
     init_vectors();
     const std::vector<double> d = inverse_diagonal(a);
     dinv0_.upload(d.data(), d.size(), s);
@@ -687,6 +828,25 @@ void DevEll::build_on_device(int rows, int cols, const int* rp, const int* ci, c
     // every slice's columns span at most 65535.
     int fmt = COL_ABS;
     cbase.resize(0);
+    soff.resize(0);
+    if (rows == cols && lanes == 1 && grid > 0) {
+        const std::vector<int> st = stencil_offsets(rows, rp_d.get(), ci_d.get(), offsets.back(), s);
+        if (!st.empty()) {
+            const std::size_t st_slots = static_cast<std::size_t>(st.back());
+            sp.upload(st.data(), st.size(), s);
+            soff.resize(st_slots / k_wave);
+            col.resize(0);
+            dcol.resize(0);
+            v64.resize(st_slots);
+            v32.resize(st_slots);
+            k_slice_stencil<true><<<grid, k_block, 0, s>>>(rows, n_slices, rp_d.get(), ci_d.get(), nullptr, sp.get(),
+                                                           soff.get(), va_d.get(), nullptr, v64.get(), v32.get(),
+                                                           nullptr);
+            hip_check(hipGetLastError(), "k_slice_stencil launch");
+            hip_check(hipStreamSynchronize(s), "ELL build sync");
+            return;
+        }
+    }
     if (rows == cols) {
         fmt = short_offsets_fit(rows, cols, rp, ci) ? COL_ROW : COL_ABS;
     } else if (grid > 0 && std::getenv("CFDG_NO_SHORT_COLS") == nullptr) {
