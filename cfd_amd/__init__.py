@@ -9,7 +9,7 @@ from . import _lib
 _lib.lib()  # fail at import time when the native library is missing
 
 from .amg import AmgHierarchy, AmgSettings  # noqa: E402
-from .fv import DeviceFv, FvMesh, fv_box  # noqa: E402
+from .fv import DeviceFv, FvMesh, PressureEquation, PressureInputs, fv_box, synthetic_pressure_inputs  # noqa: E402
 from .server import ServedRequest, Server, ServerCounters, estimate_footprint  # noqa: E402
 from .krylov import (AmgLevelView, AmgView, Batch, CsrView, DeviceSolver, LduPattern, Method,  # noqa: E402
                      Precision, SolveControl, SolveStats, assemble_convection_diffusion,
@@ -39,5 +39,5 @@ __all__ = [
     "LduPattern", "Method", "Precision", "SolveControl", "SolveStats",
     "assemble_convection_diffusion", "backend_name", "device_available", "device_description",
     "solve", "solve_amg_cg", "Server", "ServedRequest", "ServerCounters", "estimate_footprint",
-    "DeviceFv", "FvMesh", "fv_box",
+    "DeviceFv", "FvMesh", "fv_box", "PressureInputs", "PressureEquation", "synthetic_pressure_inputs",
 ]

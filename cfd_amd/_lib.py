@@ -145,6 +145,29 @@ class FvMesh(C.Structure):
     ]
 
 
+class PressureInputs(C.Structure):
+    _fields_ = [
+        ("upper", C.POINTER(C.c_double)),
+        ("lower", C.POINTER(C.c_double)),
+        ("diag", C.POINTER(C.c_double)),
+        ("source", C.POINTER(C.c_double)),
+        ("u", C.POINTER(C.c_double)),
+        ("delta_coeffs", C.POINTER(C.c_double)),
+        ("boundary_kind", C.POINTER(C.c_int)),
+        ("fixed_velocity", C.POINTER(C.c_double)),
+        ("outlet_pressure", C.POINTER(C.c_double)),
+        ("u_prev", C.POINTER(C.c_double)),
+        ("phi_prev", C.POINTER(C.c_double)),
+        ("alpha_u", C.c_double),
+        ("pin_reference", C.c_int),
+    ]
+
+
+class PressureOutputs(C.Structure):
+    _fields_ = [(name, C.POINTER(C.c_double))
+                for name in ("diag", "upper", "lower", "source", "phi_hbya", "rau_f", "hbya", "rau")]
+
+
 class ServerCounters(C.Structure):
     _fields_ = [
         ("submitted", C.c_ulonglong),
@@ -266,6 +289,8 @@ _SIGNATURES = {
     "cfdg_fv_scalar_corrections": (ST, [VP, P_DBL, P_DBL, C.c_int, P_DBL, P_DBL, C.c_int, P_DBL]),
     "cfdg_fv_momentum_corrections": (ST, [VP, P_DBL, P_DBL, C.c_int, C.c_int, C.c_int, P_DBL, C.c_double,
                                           C.c_int, P_DBL]),
+    "cfdg_fv_pressure_equation": (ST, [VP, C.POINTER(PressureInputs), C.POINTER(PressureOutputs)]),
+    "cfdg_system_assemble_pressure": (ST, [VP, VP]),
     "cfdg_fv_on_device": (ST, [VP, P_INT]),
     "cfdg_fv_timing": (ST, [VP, C.c_int]),
     "cfdg_fv_last_kernel_ms": (ST, [VP, P_DBL]),

@@ -14,6 +14,7 @@
 // one pass over the slots. Workgroups map to XCDs in contiguous cell ranges, so the neighbour
 // values a cell gathers sit in the same L2.
 #include "common.hpp"
+#include "fv.hpp"
 #include "hip_util.hpp"
 
 #include <cfdgpu.h>
@@ -494,6 +495,179 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_momentum_corrections(FvSlots 
     }
 }
 
+// Eigen's Vec3 dot (fixed size 3: the first two terms as one SSE2 packet, then the third).
+__device__ __forceinline__ double dot3(double a0, double a1, double a2, double b0, double b1, double b2) {
+    return (a0 * b0 + a1 * b1) + a2 * b2;
+}
+
+// H(U)/A and rAU of the pressure correction (flow_solver.cpp:688-706). Per component i the
+// momentum matrix's off-diagonal product accumulates over the cell's internal faces in face
+// order, upper_f u_i(N) where the cell owns f and lower_f u_i(P) where it is the neighbour
+// (LduMatrix::off_diagonal_product, ldu_matrix.cpp:39-49); hbya_i = (source_i - off_i) / diag_i
+// and rAU = V / mean(diag) with Eigen's (x + y) + z.
+__global__ void __launch_bounds__(k_fv_block) k_fv_hbya(FvSlots m, const double* __restrict__ upper,
+                                                        const double* __restrict__ lower,
+                                                        const double* __restrict__ diag3,
+                                                        const double* __restrict__ source3,
+                                                        const double* __restrict__ u, double* __restrict__ hbya,
+                                                        double* __restrict__ rau) {
+    const int c = fv_cell();
+    if (c >= m.nc) {
+        return;
+    }
+    const auto fu = g(u);
+    double off0 = 0.0;
+    double off1 = 0.0;
+    double off2 = 0.0;
+    int first = 0;
+    int count = 0;
+    slot_range(m, c, first, count);
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        const int f = g(m.face)[t];
+        const int o = g(m.other)[t];
+        if (f < 0 || o < 0) {
+            break; // boundary faces and padding follow the internal faces
+        }
+        const double a = g(m.owns)[t] != 0 ? g(upper)[f] : g(lower)[f];
+        off0 += a * fu[3 * o];
+        off1 += a * fu[3 * o + 1];
+        off2 += a * fu[3 * o + 2];
+    }
+    const double d0 = g(diag3)[3 * c];
+    const double d1 = g(diag3)[3 * c + 1];
+    const double d2 = g(diag3)[3 * c + 2];
+    hbya[3 * c] = (g(source3)[3 * c] - off0) / d0;
+    hbya[3 * c + 1] = (g(source3)[3 * c + 1] - off1) / d1;
+    hbya[3 * c + 2] = (g(source3)[3 * c + 2] - off2) / d2;
+    rau[c] = g(m.vol)[c] / (((d0 + d1) + d2) / 3.0);
+}
+
+struct PressureIn {
+    const double* hbya;
+    const double* rau;
+    const double* dc;
+    const int* kind;          // per boundary face
+    const double* fixed_u;    // 3 per boundary face
+    const double* outlet_p;   // per boundary face
+    const double* u_prev;     // 3 per cell (majumdar only)
+    const double* phi_prev;   // per face (majumdar only)
+    double beta;              // 1 - alpha_u
+    int majumdar;
+    int pin;
+};
+
+struct PressureOut {
+    double* diag;
+    double* upper;
+    double* lower;
+    double* source;
+    double* phi;
+    double* rau_f;
+};
+
+// Pressure equation of one cell (flow_solver.cpp:708-796, accelerator branch): over its faces in
+// face order, the predicted flux phiHbyA = (w HbyA_P + (1 - w) HbyA_N) . S and rAU_f (internal),
+// the boundary kinds' fluxes, the Majumdar term beta (phi_prev - u_prev,f . S), and the
+// convection-0 assembly of assemble_convection_diffusion (krylov_openmp.cpp:286-300): diagonal
+// sum of rAU_f dc, divergence of phiHbyA, then source = -divergence and the boundary terms, and
+// set_reference(0, 0) for cell 0. A face's two cells compute its flux identically; the owner
+// writes the face's outputs. Nothing is shared, nothing is atomic.
+__global__ void __launch_bounds__(k_fv_block) k_fv_pressure(FvSlots m, PressureIn in, PressureOut out) {
+    const int c = fv_cell();
+    if (c >= m.nc) {
+        return;
+    }
+    const auto hb = g(in.hbya);
+    const auto ra = g(in.rau);
+    double diag = 0.0;
+    double div = 0.0;
+    double src = 0.0;
+    bool boundary = false;
+    int first = 0;
+    int count = 0;
+    slot_range(m, c, first, count);
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        const int f = g(m.face)[t];
+        if (f < 0) {
+            break;
+        }
+        const int o = g(m.other)[t];
+        const double s0 = g(m.sf0)[t];
+        const double s1 = g(m.sf1)[t];
+        const double s2 = g(m.sf2)[t];
+        if (o >= 0) {
+            const bool mine = g(m.owns)[t] != 0;
+            const int p = mine ? c : o;
+            const int n = mine ? o : c;
+            const double w = g(m.w)[t];
+            const double v0 = w * hb[3 * p] + (1.0 - w) * hb[3 * n];
+            const double v1 = w * hb[3 * p + 1] + (1.0 - w) * hb[3 * n + 1];
+            const double v2 = w * hb[3 * p + 2] + (1.0 - w) * hb[3 * n + 2];
+            double phi = dot3(v0, v1, v2, s0, s1, s2);
+            const double rf = w * ra[p] + (1.0 - w) * ra[n];
+            if (in.majumdar != 0) {
+                const auto up = g(in.u_prev);
+                const double uf0 = w * up[3 * p] + (1.0 - w) * up[3 * n];
+                const double uf1 = w * up[3 * p + 1] + (1.0 - w) * up[3 * n + 1];
+                const double uf2 = w * up[3 * p + 2] + (1.0 - w) * up[3 * n + 2];
+                phi += in.beta * (g(in.phi_prev)[f] - dot3(uf0, uf1, uf2, s0, s1, s2));
+            }
+            const double gc = rf * g(in.dc)[f];
+            const double pos = phi > 0.0 ? 0.0 * phi : 0.0;
+            const double neg = phi < 0.0 ? 0.0 * phi : 0.0;
+            if (mine) {
+                diag += pos + gc;
+                div += phi;
+                out.upper[f] = neg - gc;
+                out.lower[f] = -pos - gc;
+                out.phi[f] = phi;
+                out.rau_f[f] = rf;
+            } else {
+                diag += -neg + gc;
+                div -= phi;
+            }
+            continue;
+        }
+        if (!boundary) { // source -= divergence, before the boundary terms
+            src = 0.0 - div;
+            boundary = true;
+        }
+        const int b = -1 - o;
+        const int kind = g(in.kind)[b];
+        double phi = 0.0;
+        if (kind == CFDG_BC_VELOCITY_INLET || kind == CFDG_BC_WALL) {
+            const auto fu = g(in.fixed_u);
+            phi = dot3(fu[3 * b], fu[3 * b + 1], fu[3 * b + 2], s0, s1, s2);
+        } else if (kind == CFDG_BC_PRESSURE_OUTLET) {
+            phi = dot3(hb[3 * c], hb[3 * c + 1], hb[3 * c + 2], s0, s1, s2);
+            if (in.majumdar != 0) {
+                const auto up = g(in.u_prev);
+                phi += in.beta * (g(in.phi_prev)[f] - dot3(up[3 * c], up[3 * c + 1], up[3 * c + 2], s0, s1, s2));
+            }
+        }
+        const double rf = ra[c];
+        out.phi[f] = phi;
+        out.rau_f[f] = rf;
+        src -= phi;
+        if (kind == CFDG_BC_PRESSURE_OUTLET) {
+            const double gc = rf * g(in.dc)[f];
+            diag += gc;
+            src += gc * g(in.outlet_p)[b];
+        }
+    }
+    if (!boundary) {
+        src = 0.0 - div;
+    }
+    if (in.pin != 0 && c == 0) {
+        src += diag * 0.0;
+        diag += diag;
+    }
+    out.diag[c] = diag;
+    out.source[c] = src;
+}
+
 // Fills the slots of every cell of every slice (cells past the last one get padding) from the
 // face-indexed mesh arrays; values are copied, never recomputed.
 struct FvFaceArrays {
@@ -763,6 +937,111 @@ public:
         get(src_, source, 3LL * nc_);
     }
 
+    // Pressure equation (flow_solver.cpp:688-796) from host inputs; the results stay on the
+    // device (pressure()) and are copied to the non-null outputs.
+    void pressure_equation(const cfdg_pressure_inputs& in, cfdg_pressure_outputs* out) {
+        const long long c = nc_;
+        const long long fi = nif_;
+        const long long f = nf_;
+        const long long b = nb();
+        require((fi == 0 || (in.upper != nullptr && in.lower != nullptr)) && in.diag != nullptr &&
+                    in.source != nullptr && in.u != nullptr && in.delta_coeffs != nullptr &&
+                    (b == 0 || (in.boundary_kind != nullptr && in.fixed_velocity != nullptr &&
+                                in.outlet_pressure != nullptr)),
+                "pressure_equation: null array");
+        for (long long k = 0; k < b; ++k) {
+            require(in.boundary_kind[k] >= CFDG_BC_VELOCITY_INLET && in.boundary_kind[k] <= CFDG_BC_EMPTY,
+                    "pressure_equation: boundary kind out of range");
+        }
+        const bool majumdar = in.u_prev != nullptr && in.phi_prev != nullptr && in.alpha_u < 1.0;
+        if (pupper_.size() == 0) {
+            const std::size_t c1 = at_least_one(c);
+            const std::size_t fi1 = at_least_one(fi);
+            const std::size_t f1 = at_least_one(f);
+            const std::size_t b1 = at_least_one(b);
+            mupper_.resize(fi1);
+            mlower_.resize(fi1);
+            mdiag_.resize(3 * c1);
+            msrc_.resize(3 * c1);
+            mu_.resize(3 * c1);
+            dc_.resize(f1);
+            kind_.resize(b1);
+            fixed_u_.resize(3 * b1);
+            outlet_p_.resize(b1);
+            hbya_.resize(3 * c1);
+            rau_.resize(c1);
+            phih_.resize(f1);
+            rauf_.resize(f1);
+            pdiag_.resize(c1);
+            pupper_.resize(fi1);
+            plower_.resize(fi1);
+            psrc_.resize(c1);
+        }
+        if (majumdar && uprev_.size() == 0) {
+            uprev_.resize(at_least_one(3 * c));
+            phiprev_.resize(at_least_one(f));
+        }
+        begin();
+        put(mupper_, in.upper, fi);
+        put(mlower_, in.lower, fi);
+        put(mdiag_, in.diag, 3 * c);
+        put(msrc_, in.source, 3 * c);
+        put(mu_, in.u, 3 * c);
+        put(dc_, in.delta_coeffs, f);
+        if (b > 0) {
+            hip_check(hipMemcpyAsync(kind_.get(), in.boundary_kind, static_cast<std::size_t>(b) * sizeof(int),
+                                     hipMemcpyHostToDevice, stream_.get()),
+                      "DeviceFv H2D");
+        }
+        put(fixed_u_, in.fixed_velocity, 3 * b);
+        put(outlet_p_, in.outlet_pressure, b);
+        if (majumdar) {
+            put(uprev_, in.u_prev, 3 * c);
+            put(phiprev_, in.phi_prev, f);
+        }
+        const FvSlots m = mesh();
+        launch([&](hipStream_t s) {
+            k_fv_hbya<<<blocks(), k_fv_block, 0, s>>>(m, mupper_.get(), mlower_.get(), mdiag_.get(), msrc_.get(),
+                                                      mu_.get(), hbya_.get(), rau_.get());
+        });
+        const PressureIn pin{hbya_.get(),     rau_.get(),
+                             dc_.get(),       kind_.get(),
+                             fixed_u_.get(),  outlet_p_.get(),
+                             majumdar ? uprev_.get() : nullptr,
+                             majumdar ? phiprev_.get() : nullptr,
+                             1.0 - in.alpha_u, majumdar ? 1 : 0,
+                             in.pin_reference != 0 ? 1 : 0};
+        const PressureOut pout{pdiag_.get(), pupper_.get(), plower_.get(), psrc_.get(), phih_.get(), rauf_.get()};
+        launch([&](hipStream_t s) { k_fv_pressure<<<blocks(), k_fv_block, 0, s>>>(m, pin, pout); });
+        have_pressure_ = true;
+        const auto back = [&](const DevBuf<double>& buf, double* dst, long long n) {
+            if (dst != nullptr && n > 0) {
+                hip_check(hipMemcpyAsync(dst, buf.get(), static_cast<std::size_t>(n) * sizeof(double),
+                                         hipMemcpyDeviceToHost, stream_.get()),
+                          "DeviceFv D2H");
+            }
+        };
+        if (out != nullptr) {
+            back(pdiag_, out->diag, c);
+            back(pupper_, out->upper, fi);
+            back(plower_, out->lower, fi);
+            back(psrc_, out->source, c);
+            back(phih_, out->phi_hbya, f);
+            back(rauf_, out->rau_f, f);
+            back(hbya_, out->hbya, 3 * c);
+            back(rau_, out->rau, c);
+        }
+        hip_check(hipStreamSynchronize(stream_.get()), "DeviceFv sync");
+    }
+
+    // The last pressure equation on the device (all null before the first one).
+    [[nodiscard]] FvPressureArrays pressure() const {
+        if (!have_pressure_) {
+            return FvPressureArrays{nc_, nif_, nullptr, nullptr, nullptr, nullptr};
+        }
+        return FvPressureArrays{nc_, nif_, pdiag_.get(), pupper_.get(), plower_.get(), psrc_.get()};
+    }
+
     void set_timing(bool on) { timing_ = on; }
     [[nodiscard]] double last_kernel_ms() const { return last_ms_; }
 
@@ -857,6 +1136,11 @@ private:
     DevBuf<double> sv_[12]; // sf, lsq_d, face centre, correction: three components each
     DevBuf<double> cc_, vol_, lsq_;
     DevBuf<double> phi_, bv_, sgrad_, vphi_, vbv_, vgrad_, lim_, flux_, coef_, src_, tmp_;
+    // pressure equation: momentum inputs, boundary data, intermediate and output arrays
+    DevBuf<double> mupper_, mlower_, mdiag_, msrc_, mu_, dc_, fixed_u_, outlet_p_, uprev_, phiprev_;
+    DevBuf<int> kind_;
+    DevBuf<double> hbya_, rau_, phih_, rauf_, pdiag_, pupper_, plower_, psrc_;
+    bool have_pressure_ = false;
     Event e0_;
     Event e1_;
     bool have_vector_ = false;
@@ -921,6 +1205,18 @@ extern "C" cfdg_status cfdg_fv_momentum_corrections(cfdg_fv* fv, const double* f
                                     transpose_term != 0, source);
     });
 }
+
+extern "C" cfdg_status cfdg_fv_pressure_equation(cfdg_fv* fv, const cfdg_pressure_inputs* in,
+                                                 cfdg_pressure_outputs* out) {
+    return cfdg::guarded([&] {
+        cfdg::require(fv != nullptr && in != nullptr, "DeviceFv: null argument");
+        fv->d->pressure_equation(*in, out);
+    });
+}
+
+namespace cfdg {
+FvPressureArrays fv_pressure_arrays(const cfdg_fv* fv) { return fv->d->pressure(); }
+} // namespace cfdg
 
 extern "C" cfdg_status cfdg_fv_on_device(const cfdg_fv* fv, int* on_device) {
     return cfdg::guarded([&] {

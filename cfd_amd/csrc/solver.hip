@@ -6,6 +6,7 @@
 // Krylov recurrences lives on the device, and the host reads back one integer per iteration
 // (the number of systems still running) one iteration behind the launches.
 #include "solver.hpp"
+#include "fv.hpp"
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -629,6 +630,28 @@ void System::assemble(const double* diag, const double* upper, const double* low
         hip_check(hipMemcpyAsync(ldu + n + nf, lower, nf * sizeof(double), hipMemcpyHostToDevice, s),
                   "H2D lower");
     }
+    finish_assembly();
+}
+
+void System::assemble_device(const double* diag, const double* upper, const double* lower, const double* b) {
+    require(nf_ >= 0 && a0_src_.size() > 0, "assemble: system was not created from an LDU pattern");
+    hipStream_t s = stream_.get();
+    const std::size_t n = static_cast<std::size_t>(n_);
+    const std::size_t nf = static_cast<std::size_t>(nf_);
+    double* ldu = ldu_.get();
+    hip_check(hipMemcpyAsync(ldu, diag, n * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D diag");
+    if (nf > 0) {
+        hip_check(hipMemcpyAsync(ldu + n, upper, nf * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D upper");
+        hip_check(hipMemcpyAsync(ldu + n + nf, lower, nf * sizeof(double), hipMemcpyDeviceToDevice, s),
+                  "D2D lower");
+    }
+    hip_check(hipMemcpyAsync(b_.get(), b, n * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D rhs");
+    finish_assembly();
+}
+
+void System::finish_assembly() {
+    hipStream_t s = stream_.get();
+    double* ldu = ldu_.get();
     const long long slots = static_cast<long long>(a0_src_.size());
     k_ldu_gather<<<ceil_div(slots, k_block), k_block, 0, s>>>(slots, a0_src_.get(), ldu,
                                                                a0_.v64.get(), a0_.v32.get());
@@ -2064,6 +2087,17 @@ extern "C" cfdg_status cfdg_system_assemble(cfdg_system* sys, const double* diag
     return guarded([&] {
         require(sys != nullptr, "null system");
         sys->sys->assemble(diag, upper, lower);
+    });
+}
+
+extern "C" cfdg_status cfdg_system_assemble_pressure(cfdg_system* sys, const cfdg_fv* fv) {
+    return guarded([&] {
+        require(sys != nullptr && fv != nullptr, "assemble_pressure: null handle");
+        const FvPressureArrays a = fv_pressure_arrays(fv);
+        require(a.diag != nullptr, "assemble_pressure: no pressure equation assembled on this DeviceFv");
+        require(a.n_cells == sys->sys->n() && a.n_internal_faces == sys->sys->n_faces(),
+                "assemble_pressure: the system's pattern is not the DeviceFv mesh's");
+        sys->sys->assemble_device(a.diag, a.upper, a.lower, a.source);
     });
 }
 

@@ -71,6 +71,9 @@ public:
 
     void set_amg(const cfdg_amg_view& v);
     void assemble(const double* diag, const double* upper, const double* lower);
+    // The same from device arrays (diag, upper, lower of the system's pattern) plus the
+    // right-hand side b, copied within HBM.
+    void assemble_device(const double* diag, const double* upper, const double* lower, const double* b);
     void set_rhs(const double* b);
     void set_x(const double* x);
     void get_x(double* x);
@@ -116,6 +119,7 @@ public:
 
 private:
     void init_vectors();
+    void finish_assembly(); // slot values and inverse diagonal from ldu_, then synchronise
     void ensure_level_vectors(bool fp32);
 
     int n_ = 0;

@@ -43,6 +43,7 @@ class FvMesh:
     lsq_d: np.ndarray         # (n_faces, 3)
     lsq_inverse: np.ndarray   # (n_cells, 9), column-major 3x3
     points: Optional[np.ndarray] = None
+    delta_coeffs: Optional[np.ndarray] = None  # n_faces: |S|^2 / (S . d)
 
     @property
     def n_boundary(self) -> int:
@@ -214,7 +215,97 @@ def fv_box(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), ji
     offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
     c64 = lambda a: np.ascontiguousarray(a, np.float64)
     return FvMesh(nc, nif, nf, owner, neighbour, offsets, cell_faces, c64(w), c64(sf), c64(cf), c64(cc),
-                  c64(vol), c64(kv), c64(dd), lsq, points=P)
+                  c64(vol), c64(kv), c64(dd), lsq, points=P, delta_coeffs=c64(delta))
+
+
+# Boundary kinds of the pressure equation (FlowBoundary, flow_solver.hpp:16).
+VELOCITY_INLET, WALL, PRESSURE_OUTLET, SYMMETRY, EMPTY = 0, 1, 2, 3, 4
+
+
+@dataclass
+class PressureInputs:
+    """Inputs of FlowSolver::pressure_correction's assembly (flow_solver.cpp:688-796): the
+    momentum system (shared off-diagonals, per-component diagonal and source), the velocity, the
+    mesh's delta coefficients and the boundary conditions per boundary face."""
+    upper: np.ndarray            # n_internal_faces
+    lower: np.ndarray            # n_internal_faces
+    diag: np.ndarray             # (n_cells, 3)
+    source: np.ndarray           # (n_cells, 3)
+    u: np.ndarray                # (n_cells, 3)
+    delta_coeffs: np.ndarray     # n_faces
+    boundary_kind: np.ndarray    # n_boundary, VELOCITY_INLET .. EMPTY
+    fixed_velocity: np.ndarray   # (n_boundary, 3)
+    outlet_pressure: np.ndarray  # n_boundary
+    u_prev: Optional[np.ndarray] = None    # (n_cells, 3): with phi_prev and alpha_u < 1, the
+    phi_prev: Optional[np.ndarray] = None  # n_faces:      relaxation-independent face flux
+    alpha_u: float = 1.0
+    pin_reference: bool = True   # set_reference(0, 0.0): no fixed-pressure boundary
+
+
+@dataclass
+class PressureEquation:
+    diag: np.ndarray      # n_cells
+    upper: np.ndarray     # n_internal_faces
+    lower: np.ndarray     # n_internal_faces
+    source: np.ndarray    # n_cells
+    phi_hbya: np.ndarray  # n_faces
+    rau_f: np.ndarray     # n_faces
+    hbya: np.ndarray      # (n_cells, 3)
+    rau: np.ndarray       # n_cells
+
+
+def synthetic_pressure_inputs(mesh: FvMesh, seed: int, random_kinds: bool = False, relax: bool = False,
+                              nu: float = 1e-3, dt: float = 0.1) -> PressureInputs:
+    """A momentum system of the box's shape for the pressure assembly: a swirling velocity field
+    with seeded noise, upwind convection plus diffusion off-diagonals (nu |S|^2 / (S.d) + the
+    upwinded face flux), per-component diagonals of the row sums plus V / dt (component
+    variations of a few percent), sources A u + noise. Boundaries: inlet on x-min, outlet on
+    x-max, symmetry on y-min, walls elsewhere, or (random_kinds) seeded kinds of all five;
+    relax adds u_prev / phi_prev with alpha_u = 0.7."""
+    rng = np.random.default_rng(seed)
+    nc, nif, nf, nb = mesh.n_cells, mesh.n_internal_faces, mesh.n_faces, mesh.n_boundary
+    x = mesh.cell_centre
+    u = np.stack([np.sin(np.pi * x[:, 1]) + 0.3, np.cos(np.pi * x[:, 0]) * 0.5, 0.2 * x[:, 2]], axis=1)
+    u = u + 0.05 * rng.standard_normal((nc, 3))
+    own, nei = mesh.owner, mesh.neighbour
+    w = mesh.weight
+    uf = w[:, None] * u[own[:nif]] + (1.0 - w[:, None]) * u[nei]
+    flux = np.einsum("ij,ij->i", uf, mesh.sf[:nif])
+    dcoef = mesh.delta_coeffs if mesh.delta_coeffs is not None else np.linalg.norm(mesh.sf, axis=1)
+    diff = nu * dcoef[:nif]
+    upper = -(diff + np.maximum(-flux, 0.0))
+    lower = -(diff + np.maximum(flux, 0.0))
+    rows = np.zeros(nc)
+    np.add.at(rows, own[:nif], -upper)
+    np.add.at(rows, nei, -lower)
+    base = rows + mesh.volume / dt
+    diag = base[:, None] * (1.0 + 0.03 * rng.random((nc, 3)))
+    source = diag * u + 0.01 * rng.standard_normal((nc, 3)) * base[:, None]
+    cf = mesh.face_centre[nif:]
+    if random_kinds:
+        kind = rng.integers(0, 5, nb).astype(np.int32)
+    else:
+        lo, hi = cf.min(axis=0), cf.max(axis=0)
+        tol = 1e-9 * max(1.0, float(np.abs(hi - lo).max()))
+        kind = np.full(nb, WALL, np.int32)
+        kind[cf[:, 1] <= lo[1] + tol] = SYMMETRY
+        kind[cf[:, 0] <= lo[0] + tol] = VELOCITY_INLET
+        kind[cf[:, 0] >= hi[0] - tol] = PRESSURE_OUTLET
+    fixed = np.zeros((nb, 3))
+    fixed[kind == VELOCITY_INLET] = (1.0, 0.0, 0.0)
+    fixed[kind == WALL] = 0.01 * rng.standard_normal((int((kind == WALL).sum()), 3))
+    pout = rng.random(nb)
+    u_prev = phi_prev = None
+    alpha = 1.0
+    if relax:
+        u_prev = u + 0.02 * rng.standard_normal((nc, 3))
+        phi_prev = np.concatenate([flux, np.einsum("ij,ij->i", u[own[nif:]], mesh.sf[nif:])])
+        phi_prev = phi_prev + 0.01 * rng.standard_normal(nf)
+        alpha = 0.7
+    c64 = lambda a: np.ascontiguousarray(a, np.float64)
+    return PressureInputs(c64(upper), c64(lower), c64(diag), c64(source), c64(u), c64(dcoef), kind, c64(fixed),
+                          c64(pout), None if u_prev is None else c64(u_prev),
+                          None if phi_prev is None else c64(phi_prev), alpha, True)
 
 
 def _d(a, n=None) -> np.ndarray:
@@ -292,6 +383,29 @@ class DeviceFv:
                int(bool(limited)), int(bool(venkat)), L.dptr(e), float(central), int(bool(transpose_term)),
                L.dptr(out))
         return out.reshape(m.n_cells, 3)
+
+    def pressure_equation(self, inp: PressureInputs) -> PressureEquation:
+        """Assembles the pressure equation on the device (kept there for
+        DeviceSolver.assemble_pressure) and returns it."""
+        m = self._mesh
+        nc, nif, nf, nb = m.n_cells, m.n_internal_faces, m.n_faces, m.n_boundary
+        keep = [_d(inp.upper, nif), _d(inp.lower, nif), _d(inp.diag, 3 * nc), _d(inp.source, 3 * nc),
+                _d(inp.u, 3 * nc), _d(inp.delta_coeffs, nf)]
+        kind = np.ascontiguousarray(inp.boundary_kind, np.int32).reshape(-1)
+        if kind.shape[0] != nb:
+            raise L.InvalidArgument(1, f"boundary_kind has {kind.shape[0]} values where {nb} are expected")
+        fixed, pout = _d(inp.fixed_velocity, 3 * nb), _d(inp.outlet_pressure, nb)
+        up = None if inp.u_prev is None else _d(inp.u_prev, 3 * nc)
+        pp = None if inp.phi_prev is None else _d(inp.phi_prev, nf)
+        c = L.PressureInputs(*[L.dptr(a) for a in keep], L.iptr(kind), L.dptr(fixed), L.dptr(pout),
+                             L.dptr(up) if up is not None else None, L.dptr(pp) if pp is not None else None,
+                             float(inp.alpha_u), int(bool(inp.pin_reference)))
+        eq = PressureEquation(np.empty(nc), np.empty(nif), np.empty(nif), np.empty(nc), np.empty(nf),
+                              np.empty(nf), np.empty((nc, 3)), np.empty(nc))
+        o = L.PressureOutputs(*[L.dptr(a) for a in (eq.diag, eq.upper, eq.lower, eq.source, eq.phi_hbya,
+                                                    eq.rau_f, eq.hbya, eq.rau)])
+        L.call("cfdg_fv_pressure_equation", self._h, C.byref(c), C.byref(o))
+        return eq
 
     def on_device(self) -> bool:
         v = C.c_int()

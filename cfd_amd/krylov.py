@@ -275,6 +275,11 @@ class DeviceSolver:
         diag, upper, lower = _f64(diag), _f64(upper), _f64(lower)
         L.call("cfdg_system_assemble", self._h, L.dptr(diag), L.dptr(upper), L.dptr(lower))
 
+    def assemble_pressure(self, fv) -> None:
+        """Matrix and right-hand side from the pressure equation last assembled by `fv` (a
+        DeviceFv on this system's mesh), copied within HBM."""
+        L.call("cfdg_system_assemble_pressure", self._h, fv._h)
+
     def set_rhs(self, b) -> None:
         b = _f64(b)
         L.call("cfdg_system_set_rhs", self._h, L.dptr(b))

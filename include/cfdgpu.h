@@ -334,6 +334,59 @@ cfdg_status cfdg_fv_momentum_corrections(cfdg_fv* fv, const double* flux, const 
                                          int linear_upwind, int limited, int venkat,
                                          const double* eps2, double central, int transpose_term,
                                          double* source);
+/* Pressure equation of a SIMPLE / PISO pressure correction, assembled on the device: replaces
+ * the host loops of FlowSolver::pressure_correction (flow_solver.cpp:688-796): H(U)/A and
+ * rAU = V / mean(A) from the momentum matrix (LduMatrix::off_diagonal_product,
+ * ldu_matrix.cpp:39-49), the predicted face flux phiHbyA and rAU_f with the boundary kinds, the
+ * optional relaxation-independent flux correction (Majumdar), the Laplacian coefficients and
+ * div(phiHbyA) of gpu::assemble_convection_diffusion with convection 0 (krylov_openmp.cpp:257-301,
+ * in face order), the boundary terms and the pin LduMatrix::set_reference(0, 0)
+ * (ldu_matrix.cpp:105-109). Boundary kinds follow FlowBoundary (flow_solver.hpp:16). */
+enum {
+    CFDG_BC_VELOCITY_INLET = 0,
+    CFDG_BC_WALL = 1,
+    CFDG_BC_PRESSURE_OUTLET = 2,
+    CFDG_BC_SYMMETRY = 3,
+    CFDG_BC_EMPTY = 4
+};
+
+typedef struct cfdg_pressure_inputs {
+    const double* upper;           /* n_internal_faces: momentum matrix off-diagonals (sys.base) */
+    const double* lower;           /* n_internal_faces */
+    const double* diag;            /* 3 n_cells: per-component diagonal (sys.diag), x y z per cell */
+    const double* source;          /* 3 n_cells: per-component source without grad p (sys.source) */
+    const double* u;               /* 3 n_cells: velocity */
+    const double* delta_coeffs;    /* n_faces */
+    const int* boundary_kind;      /* n_boundary faces: CFDG_BC_* */
+    const double* fixed_velocity;  /* 3 n_boundary: inlet / wall velocity (fixed_velocity_) */
+    const double* outlet_pressure; /* n_boundary: outlet pressure (BoundaryCondition::pressure) */
+    const double* u_prev;          /* 3 n_cells, or NULL: with phi_prev and alpha_u < 1 the */
+    const double* phi_prev;        /* n_faces, or NULL:   relaxation-independent face flux */
+    double alpha_u;
+    int pin_reference;             /* 1: set_reference(0, 0.0) (no fixed-pressure boundary) */
+} cfdg_pressure_inputs;
+
+typedef struct cfdg_pressure_outputs { /* every pointer may be NULL: not copied back */
+    double* diag;     /* n_cells: pressure matrix */
+    double* upper;    /* n_internal_faces */
+    double* lower;    /* n_internal_faces */
+    double* source;   /* n_cells */
+    double* phi_hbya; /* n_faces */
+    double* rau_f;    /* n_faces */
+    double* hbya;     /* 3 n_cells */
+    double* rau;      /* n_cells */
+} cfdg_pressure_outputs;
+
+/* Assembles the pressure equation on the device; the results stay resident in the object (for
+ * cfdg_system_assemble_pressure) and are copied to the non-NULL outputs. Results equal the
+ * serial face loops bit for bit. CFDG_ERR_INVALID_ARGUMENT for a missing array or a boundary
+ * kind out of range. */
+cfdg_status cfdg_fv_pressure_equation(cfdg_fv* fv, const cfdg_pressure_inputs* in,
+                                      cfdg_pressure_outputs* out);
+/* No counterpart in the reference: the system's matrix and right-hand side become the pressure
+ * equation last assembled by fv, copied within HBM (no PCIe transfer). The system must have been
+ * created from the pattern of fv's mesh (same cells and internal faces, in the same order). */
+cfdg_status cfdg_system_assemble_pressure(cfdg_system* sys, const cfdg_fv* fv);
 /* Replaces DeviceFv::on_device (fv_device.hpp:68): 1. */
 cfdg_status cfdg_fv_on_device(const cfdg_fv* fv, int* on_device);
 /* No counterpart in the reference: device time (ms) of the kernels of the last call on this

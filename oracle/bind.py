@@ -93,6 +93,8 @@ def orc() -> C.CDLL:
             "orc_fv_scalar_corrections": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, C.c_int, P_DBL, P_DBL,
                                                  C.c_int, P_DBL]),
             "orc_fv_limiter": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, P_DBL, C.c_int, P_DBL, P_DBL]),
+            "orc_pressure_equation": (None, [C.POINTER(OrcFvMesh)] + [P_DBL] * 6 + [P_INT, P_DBL, P_DBL, P_DBL,
+                                             P_DBL, C.c_double, C.c_int] + [P_DBL] * 8),
             "orc_fv_momentum_corrections": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, P_DBL, P_DBL, P_DBL,
                                                    C.c_int, C.c_int, C.c_int, P_DBL, C.c_double, C.c_int,
                                                    P_DBL]),
@@ -415,6 +417,26 @@ class FvOracle:
         phi, bv, flux, gamma = _d(phi), _d(bv), _d(flux), _d(gamma)
         orc().orc_fv_scalar_corrections(C.byref(self.m), dp(phi), dp(bv), int(scheme), dp(flux), dp(gamma),
                                         int(bool(linear_upwind)), dp(out))
+        return out
+
+    def pressure_equation(self, inp):
+        """orc_pressure_equation on inputs with PressureInputs' fields; returns a dict of the
+        outputs (diag, upper, lower, source, phi_hbya, rau_f, hbya, rau)."""
+        m = self.mesh
+        nc, nif, nf = m.n_cells, m.n_internal_faces, m.n_faces
+        a = [_d(v).reshape(-1) for v in (inp.upper, inp.lower, inp.diag, inp.source, inp.u, inp.delta_coeffs)]
+        kind = _i(np.asarray(inp.boundary_kind).reshape(-1))
+        fixed, pout = _d(inp.fixed_velocity).reshape(-1), _d(inp.outlet_pressure).reshape(-1)
+        up = None if inp.u_prev is None else _d(inp.u_prev).reshape(-1)
+        pp = None if inp.phi_prev is None else _d(inp.phi_prev).reshape(-1)
+        out = {k: np.zeros(n) for k, n in (("diag", nc), ("upper", nif), ("lower", nif), ("source", nc),
+                                           ("phi_hbya", nf), ("rau_f", nf), ("hbya", 3 * nc), ("rau", nc))}
+        orc().orc_pressure_equation(C.byref(self.m), *[dp(v) for v in a], ip(kind), dp(fixed), dp(pout),
+                                    dp(up) if up is not None else None, dp(pp) if pp is not None else None,
+                                    C.c_double(inp.alpha_u), int(bool(inp.pin_reference)),
+                                    *[dp(out[k]) for k in ("diag", "upper", "lower", "source", "phi_hbya",
+                                                           "rau_f", "hbya", "rau")])
+        out["hbya"] = out["hbya"].reshape(nc, 3)
         return out
 
     def limiter(self, u, ub, grad, venkat, eps2):

@@ -1429,6 +1429,131 @@ void orc_fv_scalar_corrections(const orc_fv_mesh* m, const double* phi, const do
     free(g);
 }
 
+/* Eigen's Vec3 dot and sum (fixed size 3, SSE2 linear-vectorized reduction: the first two
+ * terms as a packet, then the third). Parity with Eigen itself is unpinned (Eigen is absent);
+ * the order is fixed here and shared with the device kernels. */
+static double dot3(double a0, double a1, double a2, double b0, double b1, double b2) {
+    return (a0 * b0 + a1 * b1) + a2 * b2;
+}
+
+/* FlowSolver::pressure_correction, flow_solver.cpp:688-796, GpuCg branch:
+ * H(U)/A and rAU (:688-706, off-diagonal product of ldu_matrix.cpp:39-49), face flux and rAU_f
+ * (:708-733), Majumdar correction (:735-753), assemble_convection_diffusion with convection 0
+ * (krylov_openmp.cpp:286-300), source -= divergence (:760-769), boundary terms (:782-791) and
+ * set_reference(0, 0) (:792-794, ldu_matrix.cpp:105-109). */
+void orc_pressure_equation(const orc_fv_mesh* m, const double* upper, const double* lower, const double* diag3,
+                           const double* source3, const double* u, const double* dc, const int* kind,
+                           const double* fixed_velocity, const double* outlet_p, const double* u_prev,
+                           const double* phi_prev, double alpha_u, int pin, double* p_diag, double* p_upper,
+                           double* p_lower, double* p_source, double* phi_hbya, double* rau_f, double* hbya,
+                           double* rau) {
+    const int nc = m->nc, nif = m->nif, nf = m->nf;
+    double* off = (double*)malloc(sizeof(double) * (size_t)(nc > 0 ? nc : 1));
+    double* comp = (double*)malloc(sizeof(double) * (size_t)(nc > 0 ? nc : 1));
+    double* divergence = (double*)malloc(sizeof(double) * (size_t)(nc > 0 ? nc : 1));
+    for (int i = 0; i < 3; ++i) {
+        for (int c = 0; c < nc; ++c) {
+            comp[c] = u[3 * c + i];
+            off[c] = 0.0;
+        }
+        for (int f = 0; f < nif; ++f) {
+            const int p = m->own[f], n = m->nei[f];
+            off[p] += upper[f] * comp[n];
+            off[n] += lower[f] * comp[p];
+        }
+        for (int c = 0; c < nc; ++c) {
+            hbya[3 * c + i] = (source3[3 * c + i] - off[c]) / diag3[3 * c + i];
+        }
+    }
+    for (int c = 0; c < nc; ++c) {
+        const double mean = ((diag3[3 * c] + diag3[3 * c + 1]) + diag3[3 * c + 2]) / 3.0;
+        rau[c] = m->vol[c] / mean;
+    }
+    for (int f = 0; f < nif; ++f) {
+        const int p = m->own[f], n = m->nei[f];
+        const double w = m->w[f];
+        const double v0 = w * hbya[3 * p] + (1.0 - w) * hbya[3 * n];
+        const double v1 = w * hbya[3 * p + 1] + (1.0 - w) * hbya[3 * n + 1];
+        const double v2 = w * hbya[3 * p + 2] + (1.0 - w) * hbya[3 * n + 2];
+        phi_hbya[f] = dot3(v0, v1, v2, m->sf[3 * f], m->sf[3 * f + 1], m->sf[3 * f + 2]);
+        rau_f[f] = w * rau[p] + (1.0 - w) * rau[n];
+    }
+    for (int f = nif; f < nf; ++f) {
+        const int p = m->own[f], b = f - nif;
+        const double* s = m->sf + 3 * f;
+        rau_f[f] = rau[p];
+        switch (kind[b]) {
+        case 0:
+        case 1:
+            phi_hbya[f] = dot3(fixed_velocity[3 * b], fixed_velocity[3 * b + 1], fixed_velocity[3 * b + 2], s[0], s[1],
+                               s[2]);
+            break;
+        case 2:
+            phi_hbya[f] = dot3(hbya[3 * p], hbya[3 * p + 1], hbya[3 * p + 2], s[0], s[1], s[2]);
+            break;
+        default:
+            phi_hbya[f] = 0.0;
+            break;
+        }
+    }
+    if (u_prev != NULL && phi_prev != NULL && alpha_u < 1.0) {
+        const double beta = 1.0 - alpha_u;
+        for (int f = 0; f < nif; ++f) {
+            const int p = m->own[f], n = m->nei[f];
+            const double w = m->w[f];
+            const double* s = m->sf + 3 * f;
+            const double uf0 = w * u_prev[3 * p] + (1.0 - w) * u_prev[3 * n];
+            const double uf1 = w * u_prev[3 * p + 1] + (1.0 - w) * u_prev[3 * n + 1];
+            const double uf2 = w * u_prev[3 * p + 2] + (1.0 - w) * u_prev[3 * n + 2];
+            phi_hbya[f] += beta * (phi_prev[f] - dot3(uf0, uf1, uf2, s[0], s[1], s[2]));
+        }
+        for (int f = nif; f < nf; ++f) {
+            if (kind[f - nif] == 2) {
+                const int p = m->own[f];
+                const double* s = m->sf + 3 * f;
+                phi_hbya[f] += beta * (phi_prev[f] - dot3(u_prev[3 * p], u_prev[3 * p + 1], u_prev[3 * p + 2], s[0],
+                                                          s[1], s[2]));
+            }
+        }
+    }
+    for (int c = 0; c < nc; ++c) {
+        p_diag[c] = 0.0;
+        p_source[c] = 0.0;
+        divergence[c] = 0.0;
+    }
+    for (int f = 0; f < nif; ++f) {
+        const double fl = phi_hbya[f];
+        const double g = rau_f[f] * dc[f];
+        const double pos = fl > 0.0 ? 0.0 * fl : 0.0;
+        const double neg = fl < 0.0 ? 0.0 * fl : 0.0;
+        p_upper[f] = neg - g;
+        p_lower[f] = -pos - g;
+        p_diag[m->own[f]] += pos + g;
+        p_diag[m->nei[f]] += -neg + g;
+        divergence[m->own[f]] += fl;
+        divergence[m->nei[f]] -= fl;
+    }
+    for (int c = 0; c < nc; ++c) {
+        p_source[c] -= divergence[c];
+    }
+    for (int f = nif; f < nf; ++f) {
+        const int p = m->own[f], b = f - nif;
+        p_source[p] -= phi_hbya[f];
+        if (kind[b] == 2) {
+            const double g = rau_f[f] * dc[f];
+            p_diag[p] += g;
+            p_source[p] += g * outlet_p[b];
+        }
+    }
+    if (pin && nc > 0) {
+        p_source[0] += p_diag[0] * 0.0;
+        p_diag[0] += p_diag[0];
+    }
+    free(off);
+    free(comp);
+    free(divergence);
+}
+
 /* Barth-Jespersen / Venkatakrishnan limiter (fv_cell.inc:123-164). */
 void orc_fv_limiter(const orc_fv_mesh* m, const double* u, const double* ub, const double* grad, int venkat,
                     const double* eps2, double* lim) {

@@ -107,6 +107,20 @@ typedef struct orc_fv_mesh {
     const double* lsq;
 } orc_fv_mesh;
 
+/* Pressure equation of FlowSolver::pressure_correction (flow_solver.cpp:688-796) with the
+ * accelerator assembly path (gpu::assemble_convection_diffusion, convection 0), as face loops.
+ * Inputs: momentum off-diagonals upper / lower (nif), per-component diag / source / u (3 per
+ * cell), delta coefficients (nf), boundary kinds 0 inlet, 1 wall, 2 outlet, 3 symmetry, 4 empty,
+ * fixed velocity (3 per boundary face), outlet pressure, optional u_prev / phi_prev (NULL: off),
+ * alpha_u and the pin flag. Outputs: pressure diag / upper / lower / source, phi_hbya and rau_f
+ * (nf), hbya (3 per cell) and rau. */
+void orc_pressure_equation(const orc_fv_mesh* m, const double* upper, const double* lower, const double* diag3,
+                           const double* source3, const double* u, const double* dc, const int* kind,
+                           const double* fixed_velocity, const double* outlet_p, const double* u_prev,
+                           const double* phi_prev, double alpha_u, int pin, double* p_diag, double* p_upper,
+                           double* p_lower, double* p_source, double* phi_hbya, double* rau_f, double* hbya,
+                           double* rau);
+
 /* DeviceFv::gradient of a scalar field: scheme 0 Green-Gauss, 1 least squares, 2 corrected. */
 void orc_fv_gradient(const orc_fv_mesh* m, const double* phi, const double* bv, int scheme,
                      double* grad);
