@@ -325,6 +325,11 @@ def fv_bytes(m, op: str) -> float:
         mom = inner + own_nei + 8 * nif + 24 * nif + 24 * nc + 72 * nc + 24 * nc + 24 * nc + 8 * nif + 8 * nif \
             + 24 * nif + 24 * nif + 48 * nc
         return float(limiter + mom)
+    if op == "pressure_equation":  # with the relaxation-independent flux term
+        hbya = inner + own_nei + 16 * nif + 24 * nc + 48 * nc + 8 * nc + 32 * nc
+        press = lists + own_nei + 8 * nif + 24 * nf + 8 * nf + 32 * nc + 36 * nb + 24 * nc + 8 * nf + 16 * nf \
+            + 16 * nif + 16 * nc
+        return float(hbya + press)
     raise KeyError(op)
 
 
@@ -333,7 +338,7 @@ def fv_section(n_side: int = 128, reps: int = 10, cpu: bool = True) -> dict:
     operation's kernels (HIP events in the library), algorithmic GB/s against the 8 TB/s HBM
     roofline, the PCIe-inclusive time of the call, and the reference's gpu::DeviceFv host path on
     the same mesh (oracle/_ref, all host cores) as the CPU baseline."""
-    from cfd_amd.fv import DeviceFv, fv_box
+    from cfd_amd.fv import DeviceFv, fv_box, synthetic_pressure_inputs
 
     t0 = time.perf_counter()
     mesh = fv_box(n_side, n_side, n_side, jitter=0.2, seed=11)
@@ -354,6 +359,8 @@ def fv_section(n_side: int = 128, reps: int = 10, cpu: bool = True) -> dict:
         "scalar_corrections": lambda d: d.scalar_corrections(phi, bv, 1, flux, nu, True, src),
         "momentum_corrections": lambda d: d.momentum_corrections(flux, nu, True, True, False, eps2, 0.0, True, src3),
     }
+    pin = synthetic_pressure_inputs(mesh, 21, relax=True)
+    ops["pressure_equation"] = lambda d: d.pressure_equation(pin)
     out = {"workload": f"{n_side}^3 jittered hex box ({nc} cells, {mesh.n_faces} faces), FP64",
            "roofline_peak_gbs": 8000.0, "ops": {}}
     with DeviceFv(mesh) as dev:
@@ -377,12 +384,42 @@ def fv_section(n_side: int = 128, reps: int = 10, cpu: bool = True) -> dict:
                                 "frac": round(gbs / 8000.0, 4), "call_ms_with_pcie": round(1e3 * wall, 2)}
             log(f"fv {name}: {ms:.3f} ms device, {gbs:.0f} GB/s ({gbs / 80.0:.1f}% of HBM), "
                 f"{1e3 * wall:.1f} ms with PCIe")
+        out["pressure_handover"] = pressure_handover(mesh, dev, reps)
     dom = max(out["ops"], key=lambda k: out["ops"][k]["device_ms"])
     out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": out["ops"][dom]["achieved_gbs"], "peak": 8000.0,
                        "unit": "GB/s", "frac": out["ops"][dom]["frac"]}
     if cpu:
         out["cpu_baseline"] = fv_cpu_baseline(mesh, ops)
     return out
+
+
+def pressure_handover(mesh, dev, reps: int) -> dict:
+    """A solver system takes the pressure equation the DeviceFv object assembled, within HBM
+    (cfdg_system_assemble_pressure), against the same equation as host arrays (assemble +
+    set_rhs over PCIe): wall ms per hand-over."""
+    import cfd_amd
+    from cfd_amd import cases
+    from cfd_amd.fv import synthetic_pressure_inputs
+
+    nif = mesh.n_internal_faces
+    pat = cases.csr_pattern(mesh.n_cells, mesh.owner[:nif].astype(np.int32), mesh.neighbour.astype(np.int32))
+    eq = dev.pressure_equation(synthetic_pressure_inputs(mesh, 21, relax=True))
+    sysd = cfd_amd.DeviceSolver(pat)
+    try:
+        res = {}
+        for name, f in (("resident_ms", lambda: sysd.assemble_pressure(dev)),
+                        ("host_arrays_ms", lambda: (sysd.assemble(eq.diag, eq.upper, eq.lower),
+                                                    sysd.set_rhs(eq.source)))):
+            f()
+            t = time.perf_counter()
+            for _ in range(reps):
+                f()
+            res[name] = round(1e3 * (time.perf_counter() - t) / reps, 3)
+        res["host_bytes"] = int(8 * (2 * mesh.n_cells + 2 * nif))
+    finally:
+        sysd.close()
+    log(f"pressure hand-over: resident {res['resident_ms']} ms, host arrays {res['host_arrays_ms']} ms")
+    return res
 
 
 def fv_cpu_baseline(mesh, ops) -> dict:
@@ -395,14 +432,32 @@ def fv_cpu_baseline(mesh, ops) -> dict:
     host = ob.FvReference(mesh) if kind == "reference" else None
     res = {}
     for name, f in ops.items():
-        if host is None:
-            break
+        if host is None or name == "pressure_equation":
+            continue
         t = time.perf_counter()
         f(host)
         res[name] = round(1e3 * (time.perf_counter() - t), 2)
     cores = os.cpu_count() or 1
-    return {"unit": "ms per call", "ms": res, "cores": int(os.environ.get("OMP_NUM_THREADS", cores)), "kind": kind,
-            "sample": "one call of each operation on the same mesh (host arrays, no transfers)"}
+    out = {"unit": "ms per call", "ms": res, "cores": int(os.environ.get("OMP_NUM_THREADS", cores)), "kind": kind,
+           "sample": "one call of each operation on the same mesh (host arrays, no transfers)"}
+    if "pressure_equation" in ops:
+        # The reference's flow solver needs Eigen (absent): its face loops are timed as the
+        # oracle's single-threaded C restatement.
+        port = ob.FvOracle(mesh)
+        t = time.perf_counter()
+        ops["pressure_equation"](_OracleAsDevice(port))
+        out["pressure_equation"] = {"ms": round(1e3 * (time.perf_counter() - t), 2), "cores": 1, "kind": "port"}
+    return out
+
+
+class _OracleAsDevice:
+    """Lets the pressure-equation op run on the oracle (same call shape as DeviceFv)."""
+
+    def __init__(self, port):
+        self.port = port
+
+    def pressure_equation(self, inp):
+        return self.port.pressure_equation(inp)
 
 
 def main(argv=None) -> int:
