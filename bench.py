@@ -647,8 +647,8 @@ def main(argv=None) -> int:
         "setup_ms": round(hier.setup_ms, 1),
         "setup_ms_device_products": round(setup_dev_ms, 1),
         "footprint_bytes": dev.footprint(),
-        "roofline": roof(probe_m, "k_jacobi<float,2>: last level-0 FP32 Jacobi sweep + r.z, z.q",
-                         "k_jacobi<float, 2>@8192"),
+        "roofline": roof(probe_m, "k_jacobi_pair<float,2>: last level-0 FP32 Jacobi sweep + r.z, z.q",
+                         "k_jacobi_pair<float, 2>@8192"),
         "roofline_fp64_sweep": roof(probe_d, "k_jacobi<double,1>: last level-0 FP64 Jacobi sweep + r.z",
                                     "k_jacobi<double, 1>@8192"),
         "roofline_fp64_spmv": roof(probe_spmv, "k_spmv_pz<double>: FP64 q = A p_new, p_new = z + beta p_old, p.q",

@@ -285,6 +285,133 @@ __device__ __forceinline__ T ell_row_x1(const EllView& m, int slice, int lane, c
     return ell_row_f<T>(m, slice, lane, [&](int c) { return wd[c] * b[c]; });
 }
 
+// Row sums of two slices with equal widths and one column format, their loads issued together
+// (k_ell_group entries of each slice, then the sums in storage order per row): twice the loads in
+// flight per wavefront, the same arithmetic as two ell_sum calls.
+template <class T, class V, class C, class F>
+__device__ __forceinline__ void ell_sum2(const C& ca, const C& cb, const gmem_t<const V>* __restrict__ va,
+                                         const gmem_t<const V>* __restrict__ vb, int w, F load, T& sa, T& sb) {
+    constexpr int G = k_ell_group;
+    sa = T(0);
+    sb = T(0);
+    int k = 0;
+    for (; k + G <= w; k += G) {
+        int c0[G];
+        int c1[G];
+        T a0[G];
+        T a1[G];
+        T x0[G];
+        T x1[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            c0[g] = ca((k + g) * k_wave);
+            c1[g] = cb((k + g) * k_wave);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            a0[g] = value_at<T, V>(va, (k + g) * k_wave);
+            a1[g] = value_at<T, V>(vb, (k + g) * k_wave);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            x0[g] = load(c0[g]);
+            x1[g] = load(c1[g]);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            sa = madd(a0[g], x0[g], sa);
+            sb = madd(a1[g], x1[g], sb);
+        }
+    }
+    const int rem = w - k;
+    if (rem > 0) {
+        int c0[G - 1];
+        int c1[G - 1];
+        T a0[G - 1];
+        T a1[G - 1];
+        T x0[G - 1];
+        T x1[G - 1];
+#pragma unroll
+        for (int g = 0; g < G - 1; ++g) {
+            const int kk = g < rem ? k + g : k;
+            c0[g] = ca(kk * k_wave);
+            c1[g] = cb(kk * k_wave);
+        }
+#pragma unroll
+        for (int g = 0; g < G - 1; ++g) {
+            const int kk = g < rem ? k + g : k;
+            a0[g] = value_at<T, V>(va, kk * k_wave);
+            a1[g] = value_at<T, V>(vb, kk * k_wave);
+        }
+#pragma unroll
+        for (int g = 0; g < G - 1; ++g) {
+            x0[g] = load(c0[g]);
+            x1[g] = load(c1[g]);
+        }
+        sa = madd(a0[0], x0[0], sa);
+        sb = madd(a1[0], x1[0], sb);
+#pragma unroll
+        for (int g = 1; g < G - 1; ++g) {
+            if (g < rem) {
+                sa = madd(a0[g], x0[g], sa);
+                sb = madd(a1[g], x1[g], sb);
+            }
+        }
+    }
+}
+
+// Rows s0 * 64 + lane and s0 * 64 + 64 + lane of a square sliced-ELL matrix with one lane per
+// row (slices s0 and s0 + 1; a slice past the end gives 0): equal to ell_row_f on each row, bit
+// for bit. Slices of different widths are summed one after the other.
+template <class T, class F>
+__device__ __forceinline__ void ell_row_pair(const EllView& m, int s0, int lane, F load, T& ya, T& yb) {
+    ya = T(0);
+    yb = T(0);
+    if (s0 >= m.n_slices) {
+        return;
+    }
+    if (s0 + 1 >= m.n_slices) {
+        ya = ell_row_f<T>(m, s0, lane, load);
+        return;
+    }
+    const int b0 = m.sp[s0];
+    const int b1 = m.sp[s0 + 1];
+    const int w = (b1 - b0) / k_wave;
+    if (w != (m.sp[s0 + 2] - b1) / k_wave || m.cbase != nullptr) {
+        ya = ell_row_f<T>(m, s0, lane, load);
+        yb = ell_row_f<T>(m, s0 + 1, lane, load);
+        return;
+    }
+    const int r0 = s0 * k_wave + lane;
+    const int r1 = r0 + k_wave;
+    auto pair = [&](const auto& c0, const auto& c1) {
+        if constexpr (sizeof(T) == 4) {
+            if (m.fmt == 1) {
+                const auto val = gm(static_cast<const unsigned short*>(m.val));
+                T sa;
+                T sb;
+                ell_sum2<T, unsigned short>(c0, c1, val + b0 + lane, val + b1 + lane, w, load, sa, sb);
+                const T dxa = r0 < m.n_rows ? gm(m.diag)[r0] * load(r0) : T(0);
+                const T dxb = r1 < m.n_rows ? gm(m.diag)[r1] * load(r1) : T(0);
+                ya = madd(gm(m.rscale)[r0], sa, dxa);
+                yb = madd(gm(m.rscale)[r1], sb, dxb);
+                return;
+            }
+        }
+        const auto val = gm(static_cast<const T*>(m.val));
+        ell_sum2<T, T>(c0, c1, val + b0 + lane, val + b1 + lane, w, load, ya, yb);
+    };
+    if (m.soff != nullptr) {
+        const int t0 = __builtin_amdgcn_readfirstlane(b0 / k_wave);
+        const int t1 = __builtin_amdgcn_readfirstlane(b1 / k_wave);
+        pair(ColStencil{gm(m.soff) + t0, r0, m.n_rows - 1}, ColStencil{gm(m.soff) + t1, r1, m.n_rows - 1});
+    } else if (m.dcol != nullptr) {
+        pair(ColOff{gm(m.dcol) + b0 + lane, r0}, ColOff{gm(m.dcol) + b1 + lane, r1});
+    } else {
+        pair(ColAbs{gm(m.col) + b0 + lane}, ColAbs{gm(m.col) + b1 + lane});
+    }
+}
+
 // Sum over groups of tpr consecutive lanes (tpr a power of two <= 64); every lane of the
 // wavefront must take part.
 template <class T>
@@ -905,6 +1032,91 @@ __global__ void __launch_bounds__(k_block) k_jacobi(const SysDev* __restrict__ s
         if (threadIdx.x == 0) {
             S.partials[SLOT_RZ * S.nblk0 + bk_.y] = o[0];
             S.partials[SLOT_ZQ * S.nblk0 + bk_.y] = o[1];
+        }
+    }
+}
+
+// Level-0 FP32 post-sweep with two slices per wavefront (the finest operator is sliced ELL with
+// one lane per row): a workgroup of k_block_pair threads covers the same 256 rows as a k_block
+// one, wave w serving slices 4 y + 2 w and 4 y + 2 w + 1, so every table entry keeps its meaning.
+// Each row is computed as in k_jacobi and the block partials add the four slices' wavefront sums
+// in slice order: the results are bit-identical to k_jacobi's.
+constexpr int k_block_pair = k_block / 2;
+
+#define CFDG_PAIR_PROLOGUE(sysarr, blocks, level)                                                 \
+    const int2 bk_ = (blocks)[xcd_block()];                                                       \
+    const SysDev& S = (sysarr)[bk_.x];                                                            \
+    if (S.st->status != ST_RUNNING) {                                                             \
+        return;                                                                                   \
+    }                                                                                             \
+    const LevelDev& L = S.lv[level];                                                              \
+    if (L.coarsest) {                                                                             \
+        return;                                                                                   \
+    }                                                                                             \
+    const EllView& M = L.a;                                                                       \
+    const int lane = static_cast<int>(threadIdx.x) % k_wave;                                      \
+    const int wave_ = static_cast<int>(threadIdx.x) / k_wave;                                     \
+    const int s0 = bk_.y * k_slices_per_block + 2 * wave_;                                        \
+    const int row0 = s0 * k_wave + lane;                                                          \
+    const int row1 = row0 + k_wave;
+
+// Jacobi sweep v[out] = v[in] + wd (b - A v[in]) of level 0 with the partial dots of k_jacobi.
+template <class T, int Dots>
+__global__ void __launch_bounds__(k_block_pair) k_jacobi_pair(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in,
+                                                              int out) {
+    CFDG_PAIR_PROLOGUE(sys, blocks, level)
+    const T* x = static_cast<const T*>(L.v[in]);
+    const auto gx = gm(x);
+    const auto b = gm(static_cast<const T*>(L.b));
+    const auto wd = gm(static_cast<const T*>(L.wd));
+    T ya;
+    T yb;
+    ell_row_pair<T>(M, s0, lane, [&](int c) { return gx[c]; }, ya, yb);
+    const auto o = gm(static_cast<T*>(L.v[out]));
+    double d1[2] = {0.0, 0.0};
+    double d2[2] = {0.0, 0.0};
+    const int rows[2] = {row0, row1};
+    const T ys[2] = {ya, yb};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int row = rows[h];
+        if (row < L.n) {
+            const T z = gx[row] + wd[row] * (b[row] - ys[h]);
+            o[row] = z;
+            if (Dots == 1) {
+                d1[h] = static_cast<double>(b[row]) * static_cast<double>(z);
+            }
+            if (Dots == 2) {
+                const double zd = static_cast<double>(z);
+                d1[h] = gm(S.r)[row] * zd;
+                d2[h] = zd * gm(S.q)[row];
+            }
+        }
+    }
+    if (Dots != 0) {
+        __shared__ double red[2][k_slices_per_block];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const double w1 = wave_sum(d1[h]);
+            const double w2 = Dots == 2 ? wave_sum(d2[h]) : 0.0;
+            if (lane == 0) {
+                red[0][2 * wave_ + h] = w1;
+                red[1][2 * wave_ + h] = w2;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t1 = red[0][0];
+            double t2 = red[1][0];
+#pragma unroll
+            for (int w = 1; w < k_slices_per_block; ++w) {
+                t1 += red[0][w];
+                t2 += red[1][w];
+            }
+            S.partials[SLOT_RZ * S.nblk0 + bk_.y] = t1;
+            if (Dots == 2) {
+                S.partials[SLOT_ZQ * S.nblk0 + bk_.y] = t2;
+            }
         }
     }
 }

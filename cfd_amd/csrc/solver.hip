@@ -1321,13 +1321,26 @@ void probe_end(int kind, hipStream_t s) {
     ++g_probe->used;
 }
 
+#define CFDG_LAUNCH_PAIR(kernel, grid, stream, ...)                                               \
+    do {                                                                                          \
+        if ((grid) > 0) {                                                                         \
+            kernel<<<(grid), k_block_pair, 0, (stream)>>>(__VA_ARGS__);                           \
+        }                                                                                         \
+    } while (0)
+
+// Level-0 FP32 post-sweep with two slices per wavefront (bit-identical; CFDG_NO_PAIR_SWEEPS,
+// read whenever a V-cycle is queued, keeps one slice per wavefront, for comparisons).
+bool pair_sweeps() { return std::getenv("CFDG_NO_PAIR_SWEEPS") == nullptr; }
+
 int final_index(int sweeps) {
     const int cur = (sweeps >= 2 && (sweeps - 2) % 2 == 1) ? 2 : 0;
     return sweeps % 2 == 0 ? cur : 1;
 }
 
+// pair: the last level-0 post-sweep runs two slices per wavefront (FP32 values only; measured
+// slower on BF16 rows and on the residual, whose entries gather two values each).
 template <class T, int Dots>
-void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s) {
+void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s, bool pair) {
     const int fin = final_index(sweeps);
     const int L = tb.l_max;
     CFDG_LAUNCH(k_coarse<T>, tb.cn[0], s, d, tb.c[0].get(), 0, fin);
@@ -1406,7 +1419,11 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
             const bool last = k == sweeps - 1;
             if (l == 0 && last) {
                 probe_begin(PROBE_L0_SWEEP, s);
-                CFDG_LAUNCH((k_jacobi<T, Dots>), ga, s, d, ba, l, c, o);
+                if (pair) {
+                    CFDG_LAUNCH_PAIR((k_jacobi_pair<T, Dots>), ga, s, d, ba, l, c, o);
+                } else {
+                    CFDG_LAUNCH((k_jacobi<T, Dots>), ga, s, d, ba, l, c, o);
+                }
                 probe_end(PROBE_L0_SWEEP, s);
             } else {
                 CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, c, o);
@@ -1433,6 +1450,7 @@ struct Plan {
     bool amg;
     bool fp32;
     int sweeps;
+    bool bf16 = false;
 };
 
 void enqueue_init(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t s) {
@@ -1453,12 +1471,12 @@ void enqueue_init(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t
     if (pl.fp32) {
         CFDG_LAUNCH(k_residual0<true>, g0, s, d, b0);
         fin<FIN_AMG_INIT>(d, tb, s);
-        enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s);
+        enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s, !pl.bf16 && pair_sweeps());
         fin<FIN_BETA_FLEX>(d, tb, s);
     } else {
         CFDG_LAUNCH(k_residual0<false>, g0, s, d, b0);
         fin<FIN_AMG_INIT>(d, tb, s);
-        enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s);
+        enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s, false);
         fin<FIN_BETA>(d, tb, s);
     }
 }
@@ -1503,12 +1521,12 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
     if (pl.fp32) {
         CFDG_LAUNCH(k_update_xr<2>, g0, s, d, b0, parity);
         fin<FIN_RES>(d, tb, s);
-        enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s);
+        enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s, !pl.bf16 && pair_sweeps());
         fin<FIN_BETA_FLEX>(d, tb, s, status_out);
     } else {
         CFDG_LAUNCH(k_update_xr<0>, g0, s, d, b0, parity);
         fin<FIN_RES>(d, tb, s);
-        enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s);
+        enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s, false);
         fin<FIN_BETA>(d, tb, s, status_out);
     }
 }
@@ -1541,6 +1559,7 @@ public:
             uids_.push_back(sys_[i]->uid());
         }
         pl_ = Plan{opt.method, amg, fp32, sweeps};
+        pl_.bf16 = amg && opt.precision == CFDG_PREC_MIXED_BF16;
         precision_ = amg ? opt.precision : CFDG_PREC_FP64;
         const bool bf16 = amg && opt.precision == CFDG_PREC_MIXED_BF16;
         build_tables(tb_, sys_, amg, s_);
@@ -1800,6 +1819,7 @@ SlotRunner::SlotRunner(int slots, const BatchOptions& opt, int sweeps, hipStream
     const bool amg = opt.use_amg && opt.method == CFDG_METHOD_CG;
     const bool fp32 = amg && opt.precision != CFDG_PREC_FP64;
     m.pl = Plan{opt.method, amg, fp32, fp32 ? mixed_sweeps() : sweeps};
+    m.pl.bf16 = amg && opt.precision == CFDG_PREC_MIXED_BF16;
     m.precision = amg ? opt.precision : CFDG_PREC_FP64;
     m.s = s;
     const std::size_t k = static_cast<std::size_t>(slots);
