@@ -119,3 +119,10 @@ def test_pressure_errors():
                 dev2.assemble_pressure(fv)  # another mesh
         finally:
             dev2.close()
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 1, 1), (1, 1, 3), (65, 1, 1)])
+def test_pressure_equation_tiny_meshes(shape):
+    """No internal faces, one or two of them, and a partial second slice of 64 cells."""
+    mesh = fv_box(*shape, jitter=0.0, seed=1)
+    _check(mesh, synthetic_pressure_inputs(mesh, 3, random_kinds=True, relax=True))
