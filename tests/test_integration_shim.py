@@ -1,5 +1,6 @@
-"""The reference-side binding shown in INTEGRATION.md compiles against the reference's own
-cfd::gpu header and links against libcfdgpu (skipped where the reference checkout is absent)."""
+"""The reference-side binding of the integration guide at the repository root (its section 2
+C++ block) compiles against the reference's own cfd::gpu header and links against libcfdgpu
+(skipped where the reference checkout is absent)."""
 from __future__ import annotations
 
 import re
