@@ -360,9 +360,9 @@ __device__ __forceinline__ void ell_sum2(const C& ca, const C& cb, const gmem_t<
     }
 }
 
-// Rows s0 * 64 + lane and s0 * 64 + 64 + lane of a square sliced-ELL matrix with one lane per
-// row (slices s0 and s0 + 1; a slice past the end gives 0): equal to ell_row_f on each row, bit
-// for bit. Slices of different widths are summed one after the other.
+// Rows s0 * 64 + lane and s0 * 64 + 64 + lane of a sliced-ELL matrix with one lane per row
+// (slices s0 and s0 + 1; a slice past the end gives 0): equal to ell_row_f on each row, bit for
+// bit. Slices of different widths are summed one after the other.
 template <class T, class F>
 __device__ __forceinline__ void ell_row_pair(const EllView& m, int s0, int lane, F load, T& ya, T& yb) {
     ya = T(0);
@@ -377,7 +377,7 @@ __device__ __forceinline__ void ell_row_pair(const EllView& m, int s0, int lane,
     const int b0 = m.sp[s0];
     const int b1 = m.sp[s0 + 1];
     const int w = (b1 - b0) / k_wave;
-    if (w != (m.sp[s0 + 2] - b1) / k_wave || m.cbase != nullptr) {
+    if (w != (m.sp[s0 + 2] - b1) / k_wave) {
         ya = ell_row_f<T>(m, s0, lane, load);
         yb = ell_row_f<T>(m, s0 + 1, lane, load);
         return;
@@ -405,6 +405,9 @@ __device__ __forceinline__ void ell_row_pair(const EllView& m, int s0, int lane,
         const int t0 = __builtin_amdgcn_readfirstlane(b0 / k_wave);
         const int t1 = __builtin_amdgcn_readfirstlane(b1 / k_wave);
         pair(ColStencil{gm(m.soff) + t0, r0, m.n_rows - 1}, ColStencil{gm(m.soff) + t1, r1, m.n_rows - 1});
+    } else if (m.cbase != nullptr) {
+        const auto dc = gm(reinterpret_cast<const unsigned short*>(m.dcol));
+        pair(ColBase{dc + b0 + lane, gm(m.cbase)[s0]}, ColBase{dc + b1 + lane, gm(m.cbase)[s0 + 1]});
     } else if (m.dcol != nullptr) {
         pair(ColOff{gm(m.dcol) + b0 + lane, r0}, ColOff{gm(m.dcol) + b1 + lane, r1});
     } else {
@@ -1118,6 +1121,56 @@ __global__ void __launch_bounds__(k_block_pair) k_jacobi_pair(const SysDev* __re
                 S.partials[SLOT_ZQ * S.nblk0 + bk_.y] = t2;
             }
         }
+    }
+}
+
+// Level-0 prolongation v_0[io] (+)= P_0 v_1[coarse_buf] with the k_block_pair workgroup of
+// k_jacobi_pair: two slices per wavefront when P_0 is sliced ELL with one lane per row (the
+// same arithmetic as k_prolong), otherwise k_prolong's rows for the two halves of the k_block
+// lanes the table entry stands for.
+template <class T, bool X1>
+__global__ void __launch_bounds__(k_block_pair) k_prolong_pair(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int io,
+                                                               int coarse_buf) {
+    const int2 bk_ = blocks[xcd_block()];
+    const SysDev& S = sys[bk_.x];
+    if (S.st->status != ST_RUNNING) {
+        return;
+    }
+    const LevelDev& L = S.lv[level];
+    if (L.coarsest) {
+        return;
+    }
+    const LevelDev& N = S.lv[level + 1];
+    const EllView& M = L.p;
+    if (M.tpr == 1 && M.csr == 0) {
+        const int lane = static_cast<int>(threadIdx.x) % k_wave;
+        const int s0 = bk_.y * k_slices_per_block + 2 * (static_cast<int>(threadIdx.x) / k_wave);
+        const auto e = gm(static_cast<const T*>(N.v[coarse_buf]));
+        T ta;
+        T tb;
+        ell_row_pair<T>(M, s0, lane, [&](int c) { return e[c]; }, ta, tb);
+        const auto x = gm(static_cast<T*>(L.v[io]));
+        const int rows[2] = {s0 * k_wave + lane, s0 * k_wave + k_wave + lane};
+        const T ts[2] = {ta, tb};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = rows[h];
+            if (row < L.n) {
+                if constexpr (X1) {
+                    x[row] = gm(static_cast<const T*>(L.wd))[row] * gm(static_cast<const T*>(L.b))[row] + ts[h];
+                } else {
+                    x[row] = x[row] + ts[h];
+                }
+            }
+        }
+        return;
+    }
+    const int tpr = M.tpr;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int vt = static_cast<int>(threadIdx.x) + h * k_block_pair;
+        const int row = bk_.y * (k_block / tpr) + vt / tpr;
+        op_prolong<T, X1>(L, N, M, row, vt & (tpr - 1), io, coarse_buf);
     }
 }
 

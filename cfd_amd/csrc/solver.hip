@@ -1337,8 +1337,8 @@ int final_index(int sweeps) {
     return sweeps % 2 == 0 ? cur : 1;
 }
 
-// pair: the last level-0 post-sweep runs two slices per wavefront (FP32 values only; measured
-// slower on BF16 rows and on the residual, whose entries gather two values each).
+// pair: the level-0 prolongation and last post-sweep run two slices per wavefront (FP32 values
+// only; measured slower on BF16 rows and on the residual, whose entries gather two values each).
 template <class T, int Dots>
 void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s, bool pair) {
     const int fin = final_index(sweeps);
@@ -1409,7 +1409,11 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
         const int2* ba = tb.tm[0][li].get();
         const int c0 = cur[li];
         if (sweeps == 1) {
-            CFDG_LAUNCH((k_prolong<T, true>), tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
+            if (l == 0 && pair) {
+                CFDG_LAUNCH_PAIR((k_prolong_pair<T, true>), tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
+            } else {
+                CFDG_LAUNCH((k_prolong<T, true>), tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
+            }
         } else {
             CFDG_LAUNCH((k_prolong<T, false>), tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
         }

@@ -69,3 +69,48 @@ def test_pair_sweeps_bitwise_alone_and_batched(monkeypatch, stencil, prec):
         for a, b in zip(results[True][which], results[False][which]):
             assert a[0] == b[0] and a[1] == b[1]
             assert np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("lanes,csr", [(2, 0), (4, 1), (1, 1)])
+def test_pair_prolongation_with_other_transfer_layouts(monkeypatch, lanes, csr):
+    """P_0 laid out with several lanes per row or as CSR (forced by a layout rule) takes the
+    paired prolongation's general path; solves stay bit-identical to the one-slice kernels."""
+    import json
+    from pathlib import Path
+
+    from cfd_amd import autotune
+
+    rule = {"min_rows": 1000, "max_rows": 20000, "min_mean": 3.0, "max_mean": 6.0, "lanes": lanes, "csr": csr}
+    autotune.apply({"layout_rules": [rule]})
+    try:
+        systems = [_system(shape, 70 + i) for i, shape in enumerate(SHAPES[1:])]
+        results = {}
+        for pair in (True, False):
+            if pair:
+                monkeypatch.delenv("CFDG_NO_PAIR_SWEEPS", raising=False)
+            else:
+                monkeypatch.setenv("CFDG_NO_PAIR_SWEEPS", "1")
+            devs = [_build(*t) for t in systems]
+            try:
+                ctls = [SolveControl(1e-9 * np.linalg.norm(t[0].source), 300) for t in systems]
+                out = []
+                for d, c in zip(devs, ctls):
+                    d.set_x(None)
+                    st = d.solve_resident(Method.Cg, True, Precision.MIXED_FP32, c)
+                    assert st.converged
+                    out.append((st.iterations, d.get_x()))
+                with cfd_amd.Batch(devs, Method.Cg, True, Precision.MIXED_FP32) as batch:
+                    for d in devs:
+                        d.set_x(None)
+                    stats = batch.solve(ctls)
+                    out += [(st.iterations, d.get_x()) for st, d in zip(stats, devs)]
+                results[pair] = out
+            finally:
+                for d in devs:
+                    d.close()
+        for a, b in zip(results[True], results[False]):
+            assert a[0] == b[0]
+            assert np.array_equal(a[1], b[1])
+    finally:
+        path = Path(cfd_amd.__file__).resolve().parent / "tuning.json"
+        autotune.apply(json.loads(path.read_text()) if path.exists() else {"layout_rules": []})
