@@ -360,9 +360,10 @@ __device__ __forceinline__ void ell_sum2(const C& ca, const C& cb, const gmem_t<
     }
 }
 
-// Rows s0 * 64 + lane and s0 * 64 + 64 + lane of a sliced-ELL matrix with one lane per row
-// (slices s0 and s0 + 1; a slice past the end gives 0): equal to ell_row_f on each row, bit for
-// bit. Slices of different widths are summed one after the other.
+// The lane's share of its rows in slices s0 and s0 + 1 of a sliced-ELL matrix (a slice past the
+// end gives 0; with tpr > 1 the caller adds the row groups): equal to ell_row_f on each slice,
+// bit for bit. Slices of different widths are summed one after the other. Stencil columns and
+// BF16 rows occur with one lane per row only.
 template <class T, class F>
 __device__ __forceinline__ void ell_row_pair(const EllView& m, int s0, int lane, F load, T& ya, T& yb) {
     ya = T(0);
@@ -382,8 +383,9 @@ __device__ __forceinline__ void ell_row_pair(const EllView& m, int s0, int lane,
         yb = ell_row_f<T>(m, s0 + 1, lane, load);
         return;
     }
-    const int r0 = s0 * k_wave + lane;
-    const int r1 = r0 + k_wave;
+    const int lg = __builtin_ctz(m.tpr);
+    const int r0 = s0 * (k_wave >> lg) + (lane >> lg);
+    const int r1 = r0 + (k_wave >> lg);
     auto pair = [&](const auto& c0, const auto& c1) {
         if constexpr (sizeof(T) == 4) {
             if (m.fmt == 1) {
@@ -1124,54 +1126,71 @@ __global__ void __launch_bounds__(k_block_pair) k_jacobi_pair(const SysDev* __re
     }
 }
 
-// Level-0 prolongation v_0[io] (+)= P_0 v_1[coarse_buf] with the k_block_pair workgroup of
-// k_jacobi_pair: two slices per wavefront when P_0 is sliced ELL with one lane per row (the
-// same arithmetic as k_prolong), otherwise k_prolong's rows for the two halves of the k_block
-// lanes the table entry stands for.
-template <class T, bool X1>
-__global__ void __launch_bounds__(k_block_pair) k_prolong_pair(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int io,
-                                                               int coarse_buf) {
-    const int2 bk_ = blocks[xcd_block()];
-    const SysDev& S = sys[bk_.x];
-    if (S.st->status != ST_RUNNING) {
-        return;
-    }
-    const LevelDev& L = S.lv[level];
-    if (L.coarsest) {
-        return;
-    }
-    const LevelDev& N = S.lv[level + 1];
-    const EllView& M = L.p;
-    if (M.tpr == 1 && M.csr == 0) {
-        const int lane = static_cast<int>(threadIdx.x) % k_wave;
-        const int s0 = bk_.y * k_slices_per_block + 2 * (static_cast<int>(threadIdx.x) / k_wave);
-        const auto e = gm(static_cast<const T*>(N.v[coarse_buf]));
-        T ta;
-        T tb;
-        ell_row_pair<T>(M, s0, lane, [&](int c) { return e[c]; }, ta, tb);
-        const auto x = gm(static_cast<T*>(L.v[io]));
-        const int rows[2] = {s0 * k_wave + lane, s0 * k_wave + k_wave + lane};
-        const T ts[2] = {ta, tb};
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int row = rows[h];
-            if (row < L.n) {
-                if constexpr (X1) {
-                    x[row] = gm(static_cast<const T*>(L.wd))[row] * gm(static_cast<const T*>(L.b))[row] + ts[h];
-                } else {
-                    x[row] = x[row] + ts[h];
-                }
-            }
+// Rows of one table entry (the 256 / tpr rows a k_block workgroup covers) for a k_block_pair
+// workgroup: sliced ELL reads slices 4 y + 2 w and 4 y + 2 w + 1 together in wave w; CSR rows run
+// as the two halves of the k_block lanes. epi(row, sub, value) gets every lane's row product
+// (complete in every lane of the row's group); rows past the end give 0. Bit-identical to
+// mat_row on the same rows.
+template <class T, class F, class E>
+__device__ __forceinline__ void pair_rows(const EllView& m, int y, F load, E epi) {
+    const int lane = static_cast<int>(threadIdx.x) % k_wave;
+    const int tpr = m.tpr;
+    if (m.csr == 0) {
+        const int lg = __builtin_ctz(tpr);
+        const int s0 = y * k_slices_per_block + 2 * (static_cast<int>(threadIdx.x) / k_wave);
+        T ya;
+        T yb;
+        ell_row_pair<T>(m, s0, lane, load, ya, yb);
+        if (tpr > 1) {
+            ya = group_sum(ya, tpr);
+            yb = group_sum(yb, tpr);
         }
+        const int r0 = s0 * (k_wave >> lg) + (lane >> lg);
+        epi(r0, lane & (tpr - 1), ya);
+        epi(r0 + (k_wave >> lg), lane & (tpr - 1), yb);
         return;
     }
-    const int tpr = M.tpr;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int vt = static_cast<int>(threadIdx.x) + h * k_block_pair;
-        const int row = bk_.y * (k_block / tpr) + vt / tpr;
-        op_prolong<T, X1>(L, N, M, row, vt & (tpr - 1), io, coarse_buf);
+        const int row = y * (k_block / tpr) + vt / tpr;
+        T v = T(0);
+        if (row < m.n_rows) {
+            v = csr_lane_sum<T>(gm(m.col), gm(static_cast<const T*>(m.val)), m.sp[row] + (vt & (tpr - 1)),
+                                m.sp[row + 1], tpr, load);
+        }
+        epi(row, vt & (tpr - 1), group_sum(v, tpr));
     }
+}
+
+#define CFDG_PAIR_LEVEL_PROLOGUE(sysarr, blocks, level)                                           \
+    const int2 bk_ = (blocks)[xcd_block()];                                                       \
+    const SysDev& S = (sysarr)[bk_.x];                                                            \
+    if (S.st->status != ST_RUNNING) {                                                             \
+        return;                                                                                   \
+    }                                                                                             \
+    const LevelDev& L = S.lv[level];                                                              \
+    if (L.coarsest) {                                                                             \
+        return;                                                                                   \
+    }
+
+// k_prolong with pair_rows: v_l[io] += P_l v_{l+1}[coarse_buf]; X1: v_l[io] = wd b + P_l v_{l+1}.
+template <class T, bool X1>
+__global__ void __launch_bounds__(k_block_pair) k_prolong_pair(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int io,
+                                                               int coarse_buf) {
+    CFDG_PAIR_LEVEL_PROLOGUE(sys, blocks, level)
+    const LevelDev& N = S.lv[level + 1];
+    const auto e = gm(static_cast<const T*>(N.v[coarse_buf]));
+    const auto x = gm(static_cast<T*>(L.v[io]));
+    pair_rows<T>(L.p, bk_.y, [&](int c) { return e[c]; }, [&](int row, int sub, T t) {
+        if (sub == 0 && row < L.n) {
+            if constexpr (X1) {
+                x[row] = gm(static_cast<const T*>(L.wd))[row] * gm(static_cast<const T*>(L.b))[row] + t;
+            } else {
+                x[row] = x[row] + t;
+            }
+        }
+    });
 }
 
 // v[out] = b - A v[in].

@@ -1328,6 +1328,17 @@ void probe_end(int kind, hipStream_t s) {
         }                                                                                         \
     } while (0)
 
+// One of two kernels of the same arithmetic: the pair variant (k_block_pair threads, two slices
+// per wavefront) or the one-slice kernel.
+#define CFDG_LAUNCH_EITHER(pair, kpair, kone, grid, stream, ...)                                  \
+    do {                                                                                          \
+        if (pair) {                                                                               \
+            CFDG_LAUNCH_PAIR(kpair, grid, stream, __VA_ARGS__);                                   \
+        } else {                                                                                  \
+            CFDG_LAUNCH(kone, grid, stream, __VA_ARGS__);                                         \
+        }                                                                                         \
+    } while (0)
+
 // Level-0 FP32 post-sweep with two slices per wavefront (bit-identical; CFDG_NO_PAIR_SWEEPS,
 // read whenever a V-cycle is queued, keeps one slice per wavefront, for comparisons).
 bool pair_sweeps() { return std::getenv("CFDG_NO_PAIR_SWEEPS") == nullptr; }
@@ -1338,7 +1349,9 @@ int final_index(int sweeps) {
 }
 
 // pair: the level-0 prolongation and last post-sweep run two slices per wavefront (FP32 values
-// only; measured slower on BF16 rows and on the residual, whose entries gather two values each).
+// only). Measured slower elsewhere: on BF16 rows, on the x1-forming residual (two gathers per
+// entry), and on every coarser level or multi-lane operator (the halved workgroups cost more
+// than the extra loads in flight gain there).
 template <class T, int Dots>
 void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s, bool pair) {
     const int fin = final_index(sweeps);
@@ -1409,13 +1422,11 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
         const int2* ba = tb.tm[0][li].get();
         const int c0 = cur[li];
         if (sweeps == 1) {
-            if (l == 0 && pair) {
-                CFDG_LAUNCH_PAIR((k_prolong_pair<T, true>), tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
-            } else {
-                CFDG_LAUNCH((k_prolong<T, true>), tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
-            }
+            CFDG_LAUNCH_EITHER(pair && l == 0, (k_prolong_pair<T, true>), (k_prolong<T, true>), tb.tmn[1][li], s, d,
+                               tb.tm[1][li].get(), l, c0, fin);
         } else {
-            CFDG_LAUNCH((k_prolong<T, false>), tb.tmn[1][li], s, d, tb.tm[1][li].get(), l, c0, fin);
+            CFDG_LAUNCH_EITHER(pair && l == 0, (k_prolong_pair<T, false>), (k_prolong<T, false>), tb.tmn[1][li], s,
+                               d, tb.tm[1][li].get(), l, c0, fin);
         }
         int c = c0;
         for (int k = 0; k < sweeps; ++k) {
