@@ -9,6 +9,11 @@
  * repository). Arithmetic is written in the reference's order and the library is built with
  * -ffp-contract=off, so a serial run reproduces the reference's host results bit for bit where
  * the reference itself is serial (n <= 20000, src/gpu/src/krylov_openmp.cpp:21-25).
+ *
+ * Parity unpinned (no reference build or fixture can check it here): orc_sgs_pcg's exact
+ * iteration counts (Eigen's vectorised norms), bit-exact coarse patterns against Eigen's own
+ * sparse products at full size, and orc_pressure_equation's Vec3 dot / mean order against
+ * Eigen's evaluation (Eigen is absent; each is pinned by the mathematical checks of the tests).
  */
 #ifndef CFD_ORACLE_H
 #define CFD_ORACLE_H
