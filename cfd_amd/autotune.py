@@ -99,33 +99,48 @@ def tune_layouts():
     return rules, log
 
 
-def tune_smoother(n_side: int = 96, reps: int = 5):
+def tune_smoother(n_side: int = 96, reps: int = 3, sample: int = 12):
+    """Mixed-precision smoother: damped-Jacobi sweeps per side and the ratio of the damping
+    interval, timed as the total mixed solve time of one n_side^3 jittered box and `sample`
+    systems of the batch workload's shapes (cases.batch_shapes)."""
     from . import AmgHierarchy, AmgSettings, DeviceSolver, Method, Precision, SolveControl, cases
 
-    mesh = cases.box_mesh(n_side, n_side, n_side, jitter=0.2, seed=7)
-    s = cases.pressure_system(mesh, rhs_seed=8)
-    pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
-    a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
-    view = AmgHierarchy(a, AmgSettings(strength_decay=0.5)).view()
-    tol = 1e-8 * float(np.linalg.norm(s.source))
+    systems = []
+    shapes = [dict(nx=n_side, ny=n_side, nz=n_side, lz=1.0, jitter=0.2, seed=7)]
+    shapes += cases.batch_shapes(256)[::max(1, 256 // sample)][:sample]
+    for sh in shapes:
+        mesh = cases.box_mesh(sh["nx"], sh["ny"], sh["nz"], hi=(1.0, 1.0, sh["lz"]), jitter=sh["jitter"],
+                              seed=sh["seed"])
+        s = cases.pressure_system(mesh, rhs_seed=sh["seed"] + 1)
+        pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
+        a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
+        systems.append((s, pat, AmgHierarchy(a, AmgSettings(strength_decay=0.5)).view()))
     result = {}
-    for sweeps in (1, 2):
-        L.call("cfdg_set_mixed_smoother", sweeps, 10.0)
-        dev = DeviceSolver(pat)
-        dev.assemble(s.diag, s.upper, s.lower)
-        dev.set_amg(view)
-        dev.set_rhs(s.source)
-        times = []
-        for _ in range(reps + 2):
-            dev.set_x(None)
-            st = dev.solve_resident(Method.Cg, True, Precision.MIXED_FP32, SolveControl(tol, 500))
-            times.append(st.solve_ms)
-        result[sweeps] = (float(np.median(times[2:])), st.iterations)
-        dev.close()
-        print(f"mixed smoother, {sweeps} sweep(s): {result[sweeps][0]:.3f} ms, {st.iterations} iterations",
+    for sweeps, ratio in ((1, 10.0), (1, 8.0), (1, 6.0), (1, 5.0), (2, 10.0), (2, 6.0)):
+        L.call("cfdg_set_mixed_smoother", sweeps, ratio)
+        total = 0.0
+        its = 0
+        for s, pat, view in systems:
+            dev = DeviceSolver(pat)
+            dev.assemble(s.diag, s.upper, s.lower)
+            dev.set_amg(view)  # the weights follow the smoother set before
+            dev.set_rhs(s.source)
+            tol = 1e-8 * float(np.linalg.norm(s.source))
+            times = []
+            for _ in range(reps + 1):
+                dev.set_x(None)
+                st = dev.solve_resident(Method.Cg, True, Precision.MIXED_FP32, SolveControl(tol, 500))
+                times.append(st.solve_ms)
+            total += float(np.median(times[1:]))
+            its += st.iterations
+            dev.close()
+        result[(sweeps, ratio)] = (total, its)
+        print(f"mixed smoother, {sweeps} sweep(s), ratio {ratio:4.1f}: {total:.3f} ms, {its} iterations in all",
               flush=True)
     best = min(result, key=lambda k: result[k][0])
-    return {"sweeps": best, "ratio": 10.0, "timings_ms": {str(k): v[0] for k, v in result.items()}}
+    return {"sweeps": best[0], "ratio": best[1],
+            "timings_ms": {f"{k[0]}/{k[1]:g}": round(v[0], 4) for k, v in result.items()},
+            "iterations": {f"{k[0]}/{k[1]:g}": v[1] for k, v in result.items()}}
 
 
 def apply(table: dict) -> None:
@@ -144,7 +159,16 @@ def apply(table: dict) -> None:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(DEFAULT_PATH))
+    ap.add_argument("--smoother-only", action="store_true",
+                    help="re-tune the mixed smoother only and keep the table's layout rules")
     args = ap.parse_args(argv)
+    if args.smoother_only:
+        table = json.loads(Path(args.out).read_text()) if Path(args.out).exists() else {"layout_rules": []}
+        apply({"layout_rules": table.get("layout_rules", [])})
+        table["mixed_smoother"] = tune_smoother()
+        Path(args.out).write_text(json.dumps(table, indent=1) + "\n")
+        print(f"wrote {args.out}")
+        return 0
     L.call("cfdg_set_layout_rules", None, 0)  # compare with the built-in rule
     t0 = time.time()
     rules, log = tune_layouts()

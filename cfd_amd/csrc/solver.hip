@@ -407,7 +407,7 @@ void check_csr(const cfdg_csr& m, int rows, int cols, const char* what) {
 } // namespace
 
 // Smoother of the mixed-precision V-cycle: sweeps per side (default 1, or CFDG_MIXED_SWEEPS)
-// and the ratio of the damping interval (default 10, or CFDG_MIXED_RATIO; see mixed_weight),
+// and the ratio of the damping interval (default 6, or CFDG_MIXED_RATIO; see mixed_weight),
 // process-wide, settable with cfdg_set_mixed_smoother.
 namespace {
 std::atomic<int> g_mixed_sweeps{-1};
@@ -430,8 +430,8 @@ double mixed_ratio() {
     double v = g_mixed_ratio.load();
     if (v < 0.0) {
         const char* e = std::getenv("CFDG_MIXED_RATIO");
-        const double r = e != nullptr ? std::atof(e) : 10.0;
-        v = r > 1.0 ? r : 10.0;
+        const double r = e != nullptr ? std::atof(e) : 6.0;
+        v = r > 1.0 ? r : 6.0;
         double expected = -1.0;
         g_mixed_ratio.compare_exchange_strong(expected, v);
         v = g_mixed_ratio.load();
