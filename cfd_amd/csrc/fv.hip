@@ -566,20 +566,121 @@ struct PressureOut {
     double* rau_f;
 };
 
-// Pressure equation of one cell (flow_solver.cpp:708-796, accelerator branch): over its faces in
-// face order, the predicted flux phiHbyA = (w HbyA_P + (1 - w) HbyA_N) . S and rAU_f (internal),
-// the boundary kinds' fluxes, the Majumdar term beta (phi_prev - u_prev,f . S), and the
-// convection-0 assembly of assemble_convection_diffusion (krylov_openmp.cpp:286-300): diagonal
-// sum of rAU_f dc, divergence of phiHbyA, then source = -divergence and the boundary terms, and
-// set_reference(0, 0) for cell 0. A face's two cells compute its flux identically; the owner
-// writes the face's outputs. Nothing is shared, nothing is atomic.
-__global__ void __launch_bounds__(k_fv_block) k_fv_pressure(FvSlots m, PressureIn in, PressureOut out) {
+// Face-major copies of the weight and area vector of every face, taken from the owner's slots
+// (each listed face has exactly one): the pressure assembly's face pass reads them coalesced.
+__global__ void __launch_bounds__(k_fv_block) k_fv_faces_from_slots(FvSlots m, double* __restrict__ w,
+                                                                    double* __restrict__ sf) {
     const int c = fv_cell();
     if (c >= m.nc) {
         return;
     }
+    int first = 0;
+    int count = 0;
+    slot_range(m, c, first, count);
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        const int f = g(m.face)[t];
+        if (f < 0) {
+            break;
+        }
+        if (g(m.owns)[t] == 0) {
+            continue;
+        }
+        if (f < m.nif) {
+            w[f] = g(m.w)[t];
+        }
+        sf[3LL * f] = g(m.sf0)[t];
+        sf[3LL * f + 1] = g(m.sf1)[t];
+        sf[3LL * f + 2] = g(m.sf2)[t];
+    }
+}
+
+struct PressureFaces {
+    int nif;
+    int nf;
+    const int* own;
+    const int* nei;
+    const double* w;
+    const double* sf; // 3 per face
+};
+
+// Face pass of the pressure equation (flow_solver.cpp:708-753 and the face loop of
+// assemble_convection_diffusion, krylov_openmp.cpp:289-295, convection 0), one thread per face:
+// phiHbyA = (w HbyA_P + (1 - w) HbyA_N) . S and rAU_f (internal), the boundary kinds' fluxes
+// with rAU_f = rAU_P, the Majumdar term beta (phi_prev - u_prev,f . S), the Laplacian
+// coefficient g = rAU_f dc (outlets only on the boundary; 0 elsewhere) and upper / lower.
+__global__ void __launch_bounds__(k_fv_block) k_fv_face_flux(PressureFaces fa, PressureIn in, PressureOut out,
+                                                             double* __restrict__ gface) {
+    const int f = fv_cell();
+    if (f >= fa.nf) {
+        return;
+    }
     const auto hb = g(in.hbya);
     const auto ra = g(in.rau);
+    const int p = g(fa.own)[f];
+    const double s0 = g(fa.sf)[3LL * f];
+    const double s1 = g(fa.sf)[3LL * f + 1];
+    const double s2 = g(fa.sf)[3LL * f + 2];
+    double phi = 0.0;
+    double rf = 0.0;
+    double gc = 0.0;
+    if (f < fa.nif) {
+        const int n = g(fa.nei)[f];
+        const double w = g(fa.w)[f];
+        const double v0 = w * hb[3 * p] + (1.0 - w) * hb[3 * n];
+        const double v1 = w * hb[3 * p + 1] + (1.0 - w) * hb[3 * n + 1];
+        const double v2 = w * hb[3 * p + 2] + (1.0 - w) * hb[3 * n + 2];
+        phi = dot3(v0, v1, v2, s0, s1, s2);
+        rf = w * ra[p] + (1.0 - w) * ra[n];
+        if (in.majumdar != 0) {
+            const auto up = g(in.u_prev);
+            const double uf0 = w * up[3 * p] + (1.0 - w) * up[3 * n];
+            const double uf1 = w * up[3 * p + 1] + (1.0 - w) * up[3 * n + 1];
+            const double uf2 = w * up[3 * p + 2] + (1.0 - w) * up[3 * n + 2];
+            phi += in.beta * (g(in.phi_prev)[f] - dot3(uf0, uf1, uf2, s0, s1, s2));
+        }
+        gc = rf * g(in.dc)[f];
+        const double pos = phi > 0.0 ? 0.0 * phi : 0.0;
+        const double neg = phi < 0.0 ? 0.0 * phi : 0.0;
+        out.upper[f] = neg - gc;
+        out.lower[f] = -pos - gc;
+    } else {
+        const int b = f - fa.nif;
+        const int kind = g(in.kind)[b];
+        if (kind == CFDG_BC_VELOCITY_INLET || kind == CFDG_BC_WALL) {
+            const auto fu = g(in.fixed_u);
+            phi = dot3(fu[3 * b], fu[3 * b + 1], fu[3 * b + 2], s0, s1, s2);
+        } else if (kind == CFDG_BC_PRESSURE_OUTLET) {
+            phi = dot3(hb[3 * p], hb[3 * p + 1], hb[3 * p + 2], s0, s1, s2);
+            if (in.majumdar != 0) {
+                const auto up = g(in.u_prev);
+                phi += in.beta * (g(in.phi_prev)[f] - dot3(up[3 * p], up[3 * p + 1], up[3 * p + 2], s0, s1, s2));
+            }
+        }
+        rf = ra[p];
+        if (kind == CFDG_BC_PRESSURE_OUTLET) {
+            gc = rf * g(in.dc)[f];
+        }
+    }
+    out.phi[f] = phi;
+    out.rau_f[f] = rf;
+    gface[f] = gc;
+}
+
+// Cell pass of the pressure equation (assemble_convection_diffusion's accumulation, then
+// flow_solver.cpp:760-796): over the cell's faces in face order, the diagonal sum of the
+// faces' coefficients (pos + g as owner, -neg + g as neighbour, convection 0) and the divergence
+// of phiHbyA, then source = -divergence before the first boundary face, the boundary terms
+// (-phi; outlets: diagonal + g, source + g p_out) and set_reference(0, 0) for cell 0. The faces'
+// values come from the face pass, so every cell adds the same numbers in the order of the
+// serial loops.
+__global__ void __launch_bounds__(k_fv_block) k_fv_pressure_cells(FvSlots m, const double* __restrict__ phi_f,
+                                                                  const double* __restrict__ gface, PressureIn in,
+                                                                  PressureOut out) {
+    const int c = fv_cell();
+    if (c >= m.nc) {
+        return;
+    }
     double diag = 0.0;
     double div = 0.0;
     double src = 0.0;
@@ -594,37 +695,15 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_pressure(FvSlots m, PressureI
             break;
         }
         const int o = g(m.other)[t];
-        const double s0 = g(m.sf0)[t];
-        const double s1 = g(m.sf1)[t];
-        const double s2 = g(m.sf2)[t];
+        const double phi = g(phi_f)[f];
+        const double gc = g(gface)[f];
         if (o >= 0) {
-            const bool mine = g(m.owns)[t] != 0;
-            const int p = mine ? c : o;
-            const int n = mine ? o : c;
-            const double w = g(m.w)[t];
-            const double v0 = w * hb[3 * p] + (1.0 - w) * hb[3 * n];
-            const double v1 = w * hb[3 * p + 1] + (1.0 - w) * hb[3 * n + 1];
-            const double v2 = w * hb[3 * p + 2] + (1.0 - w) * hb[3 * n + 2];
-            double phi = dot3(v0, v1, v2, s0, s1, s2);
-            const double rf = w * ra[p] + (1.0 - w) * ra[n];
-            if (in.majumdar != 0) {
-                const auto up = g(in.u_prev);
-                const double uf0 = w * up[3 * p] + (1.0 - w) * up[3 * n];
-                const double uf1 = w * up[3 * p + 1] + (1.0 - w) * up[3 * n + 1];
-                const double uf2 = w * up[3 * p + 2] + (1.0 - w) * up[3 * n + 2];
-                phi += in.beta * (g(in.phi_prev)[f] - dot3(uf0, uf1, uf2, s0, s1, s2));
-            }
-            const double gc = rf * g(in.dc)[f];
-            const double pos = phi > 0.0 ? 0.0 * phi : 0.0;
-            const double neg = phi < 0.0 ? 0.0 * phi : 0.0;
-            if (mine) {
+            if (g(m.owns)[t] != 0) {
+                const double pos = phi > 0.0 ? 0.0 * phi : 0.0;
                 diag += pos + gc;
                 div += phi;
-                out.upper[f] = neg - gc;
-                out.lower[f] = -pos - gc;
-                out.phi[f] = phi;
-                out.rau_f[f] = rf;
             } else {
+                const double neg = phi < 0.0 ? 0.0 * phi : 0.0;
                 diag += -neg + gc;
                 div -= phi;
             }
@@ -635,24 +714,8 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_pressure(FvSlots m, PressureI
             boundary = true;
         }
         const int b = -1 - o;
-        const int kind = g(in.kind)[b];
-        double phi = 0.0;
-        if (kind == CFDG_BC_VELOCITY_INLET || kind == CFDG_BC_WALL) {
-            const auto fu = g(in.fixed_u);
-            phi = dot3(fu[3 * b], fu[3 * b + 1], fu[3 * b + 2], s0, s1, s2);
-        } else if (kind == CFDG_BC_PRESSURE_OUTLET) {
-            phi = dot3(hb[3 * c], hb[3 * c + 1], hb[3 * c + 2], s0, s1, s2);
-            if (in.majumdar != 0) {
-                const auto up = g(in.u_prev);
-                phi += in.beta * (g(in.phi_prev)[f] - dot3(up[3 * c], up[3 * c + 1], up[3 * c + 2], s0, s1, s2));
-            }
-        }
-        const double rf = ra[c];
-        out.phi[f] = phi;
-        out.rau_f[f] = rf;
         src -= phi;
-        if (kind == CFDG_BC_PRESSURE_OUTLET) {
-            const double gc = rf * g(in.dc)[f];
+        if (g(in.kind)[b] == CFDG_BC_PRESSURE_OUTLET) {
             diag += gc;
             src += gc * g(in.outlet_p)[b];
         }
@@ -816,8 +879,6 @@ public:
             // face-indexed copies, needed only to fill the slots
             DevBuf<int> start;
             DevBuf<int> faces;
-            DevBuf<int> own;
-            DevBuf<int> nei;
             DevBuf<double> w;
             DevBuf<double> sf;
             DevBuf<double> lsq_d;
@@ -827,14 +888,14 @@ public:
             const double zero_d = 0.0;
             start.upload(v.cell_offsets, c + 1, s);
             faces.upload(n_list > 0 ? v.cell_faces : &zero_i, at_least_one(n_list), s);
-            own.upload(v.owner, f, s);
-            nei.upload(nif_ > 0 ? v.neighbour : &zero_i, at_least_one(nif_), s);
+            fown_.upload(v.owner, f, s); // kept: checked indices for the pressure face pass
+            fnei_.upload(nif_ > 0 ? v.neighbour : &zero_i, at_least_one(nif_), s);
             w.upload(nif_ > 0 ? v.weight : &zero_d, at_least_one(nif_), s);
             sf.upload(v.sf, 3 * f, s);
             lsq_d.upload(v.lsq_d, 3 * f, s);
             cf.upload(v.face_centre, 3 * f, s);
             kv.upload(v.correction, 3 * f, s);
-            const FvFaceArrays a{nc_, nif_, start.get(), faces.get(), own.get(), nei.get(), w.get(),
+            const FvFaceArrays a{nc_, nif_, start.get(), faces.get(), fown_.get(), fnei_.get(), w.get(),
                                  sf.get(), lsq_d.get(), cf.get(), kv.get()};
             FvSlotOut o{sface_.get(), sother_.get(), sowns_.get(), sw_.get(), {}};
             for (int j = 0; j < 12; ++j) {
@@ -1012,7 +1073,23 @@ public:
                              1.0 - in.alpha_u, majumdar ? 1 : 0,
                              in.pin_reference != 0 ? 1 : 0};
         const PressureOut pout{pdiag_.get(), pupper_.get(), plower_.get(), psrc_.get(), phih_.get(), rauf_.get()};
-        launch([&](hipStream_t s) { k_fv_pressure<<<blocks(), k_fv_block, 0, s>>>(m, pin, pout); });
+        if (fw_.size() == 0) { // face-major weights and area vectors, once (zero for unlisted faces)
+            fw_.resize(at_least_one(fi));
+            fsf_.resize(at_least_one(3 * f));
+            gface_.resize(at_least_one(f));
+            fw_.zero(stream_.get());
+            fsf_.zero(stream_.get());
+            k_fv_faces_from_slots<<<blocks(), k_fv_block, 0, stream_.get()>>>(m, fw_.get(), fsf_.get());
+            hip_check(hipGetLastError(), "k_fv_faces_from_slots launch");
+        }
+        const PressureFaces fa{nif_, nf_, fown_.get(), fnei_.get(), fw_.get(), fsf_.get()};
+        const int face_blocks = static_cast<int>((f + k_fv_block - 1) / k_fv_block);
+        launch([&](hipStream_t s) {
+            k_fv_face_flux<<<face_blocks, k_fv_block, 0, s>>>(fa, pin, pout, gface_.get());
+        });
+        launch([&](hipStream_t s) {
+            k_fv_pressure_cells<<<blocks(), k_fv_block, 0, s>>>(m, phih_.get(), gface_.get(), pin, pout);
+        });
         have_pressure_ = true;
         const auto back = [&](const DevBuf<double>& buf, double* dst, long long n) {
             if (dst != nullptr && n > 0) {
@@ -1140,6 +1217,8 @@ private:
     DevBuf<double> mupper_, mlower_, mdiag_, msrc_, mu_, dc_, fixed_u_, outlet_p_, uprev_, phiprev_;
     DevBuf<int> kind_;
     DevBuf<double> hbya_, rau_, phih_, rauf_, pdiag_, pupper_, plower_, psrc_;
+    DevBuf<int> fown_, fnei_;         // face-major owner / neighbour (pressure face pass)
+    DevBuf<double> fw_, fsf_, gface_; // face-major weight, area vector; Laplacian coefficients
     bool have_pressure_ = false;
     Event e0_;
     Event e1_;
