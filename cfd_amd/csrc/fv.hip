@@ -6,7 +6,10 @@
 //
 // One thread per cell runs over the cell's faces in ascending face order, with the reference's
 // operation order and no contraction of multiply-adds, so every result equals the serial CPU face
-// loops bit for bit; nothing is scattered, nothing is atomic. The face data a cell needs is laid
+// loops bit for bit; nothing is atomic. Quantities of a face that both its cells need (the
+// explicit correction fluxes, the pressure equation's face flux and coefficient) are computed
+// once per face by a face pass on face-major arrays and summed per cell in face order. The face
+// data a cell needs is laid
 // out once, at creation, in per-cell slots, slot-major over the 64 cells of a wavefront and one
 // array per component (the sliced layout of the solver's matrices): every kernel streams it
 // coalesced instead of gathering 3-vectors of faces shared with other cells through the face list
@@ -365,19 +368,138 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_limiter(FvSlots m, const doub
     }
 }
 
-// Explicit internal-face fluxes of a scalar equation into source (fv_cell.inc:167-195):
-// e_f = -F_f g_up . (x_f - C_up) [linear upwind] + gamma_f k_f . (w g_P + (1 - w) g_N), added for
-// the owner and subtracted for the neighbour.
-__global__ void __launch_bounds__(k_fv_block) k_fv_scalar_corrections(FvSlots m, const double* grad, const double* flux,
-                                                                      const double* gamma, int linear_upwind,
-                                                                      double* source) {
+
+// Face-major mesh data for the face passes: owner / neighbour (checked at creation) and the
+// weight, area vector, face centre and correction vector copied from the owners' slots.
+struct FvFaces {
+    int nif;
+    const int* own;
+    const int* nei;
+    const double* w;
+    const double* sf; // 3 per face
+    const double* cf; // 3 per face
+    const double* kv; // 3 per face
+};
+
+// Explicit flux of an internal face for a scalar equation (fv_cell.inc:167-195), one thread per
+// face: e_f = -F_f g_up . (x_f - C_up) [linear upwind] + gamma_f k_f . (w g_P + (1 - w) g_N); the
+// cells add it with k_fv_face_sum (+ for the owner, - for the neighbour).
+__global__ void __launch_bounds__(k_fv_block) k_fv_scalar_face(FvFaces fa, const double* __restrict__ cc,
+                                                               const double* __restrict__ grad,
+                                                               const double* __restrict__ flux,
+                                                               const double* __restrict__ gamma, int linear_upwind,
+                                                               double* __restrict__ e_out) {
+    const int f = fv_cell();
+    if (f >= fa.nif) {
+        return;
+    }
+    const auto gg = g(grad);
+    const auto cg = g(cc);
+    const int p = g(fa.own)[f];
+    const int n = g(fa.nei)[f];
+    const double fl = g(flux)[f];
+    double e = 0.0;
+    if (linear_upwind != 0) {
+        const int up = fl >= 0.0 ? p : n;
+        const double r0 = g(fa.cf)[3LL * f] - cg[3 * up];
+        const double r1 = g(fa.cf)[3LL * f + 1] - cg[3 * up + 1];
+        const double r2 = g(fa.cf)[3LL * f + 2] - cg[3 * up + 2];
+        e -= fl * (gg[3 * up] * r0 + gg[3 * up + 1] * r1 + gg[3 * up + 2] * r2);
+    }
+    const double wf = g(fa.w)[f];
+    const double gf0 = wf * gg[3 * p] + (1.0 - wf) * gg[3 * n];
+    const double gf1 = wf * gg[3 * p + 1] + (1.0 - wf) * gg[3 * n + 1];
+    const double gf2 = wf * gg[3 * p + 2] + (1.0 - wf) * gg[3 * n + 2];
+    e += g(gamma)[f] * (g(fa.kv)[3LL * f] * gf0 + g(fa.kv)[3LL * f + 1] * gf1 + g(fa.kv)[3LL * f + 2] * gf2);
+    e_out[f] = e;
+}
+
+// Explicit flux of an internal face for the momentum equation (fv_cell.inc:199-260), one thread
+// per face, per component: -F_f lim_up (g_up^T (x_f - C_up)) [linear upwind, blended with the
+// linear face value when central > 0, limited when lim is given] + nu_f (g_f^T k_f)
+// [+ nu_f g_f S_f], g_f the interpolated 3x3 gradient; the cells add it with k_fv_face_sum.
+__global__ void __launch_bounds__(k_fv_block) k_fv_momentum_face(FvFaces fa, const double* __restrict__ cc,
+                                                                 const double* __restrict__ grad,
+                                                                 const double* __restrict__ lim, int limited,
+                                                                 const double* __restrict__ flux,
+                                                                 const double* __restrict__ nu_face,
+                                                                 int linear_upwind, double central,
+                                                                 const double* __restrict__ u, int transpose_term,
+                                                                 double* __restrict__ e_out) {
+    const int f = fv_cell();
+    if (f >= fa.nif) {
+        return;
+    }
+    const auto gg = g(grad);
+    const auto fu = g(u);
+    const auto cg = g(cc);
+    const int p = g(fa.own)[f];
+    const int n = g(fa.nei)[f];
+    const double fl = g(flux)[f];
+    const double wf = g(fa.w)[f];
+    double e[3] = {0.0, 0.0, 0.0};
+    if (linear_upwind != 0) {
+        const int up = fl >= 0.0 ? p : n;
+        const double r0 = g(fa.cf)[3LL * f] - cg[3 * up];
+        const double r1 = g(fa.cf)[3LL * f + 1] - cg[3 * up + 1];
+        const double r2 = g(fa.cf)[3LL * f + 2] - cg[3 * up + 2];
+        const auto gu = gg + 9LL * up;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double d = gu[3 * i] * r0 + gu[3 * i + 1] * r1 + gu[3 * i + 2] * r2;
+            if (central > 0.0) {
+                const double uf = wf * fu[3 * p + i] + (1.0 - wf) * fu[3 * n + i];
+                d = (1.0 - central) * d + central * (uf - fu[3 * up + i]);
+            }
+            if (limited != 0) {
+                d = g(lim)[3 * up + i] * d;
+            }
+            e[i] -= fl * d;
+        }
+    }
+    double gf[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        gf[j] = wf * gg[9LL * p + j] + (1.0 - wf) * gg[9LL * n + j];
+    }
+    const double nu = g(nu_face)[f];
+    const double k0 = g(fa.kv)[3LL * f];
+    const double kk1 = g(fa.kv)[3LL * f + 1];
+    const double k2 = g(fa.kv)[3LL * f + 2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        e[i] += nu * (gf[3 * i] * k0 + gf[3 * i + 1] * kk1 + gf[3 * i + 2] * k2);
+    }
+    if (transpose_term != 0) {
+        const double a0 = g(fa.sf)[3LL * f];
+        const double a1 = g(fa.sf)[3LL * f + 1];
+        const double a2 = g(fa.sf)[3LL * f + 2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            e[i] += nu * (gf[i] * a0 + gf[i + 3] * a1 + gf[i + 6] * a2);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        e_out[3LL * f + i] = e[i];
+    }
+}
+
+// source_c += sum over the cell's internal faces in face order of +e_f (owner) or -e_f
+// (neighbour), NC components, in the order of the reference's face loops; each face's flux comes
+// from its face pass.
+template <int NC>
+__global__ void __launch_bounds__(k_fv_block) k_fv_face_sum(FvSlots m, const double* __restrict__ e_f,
+                                                            double* __restrict__ source) {
     const int c = fv_cell();
     if (c >= m.nc) {
         return;
     }
-    const auto gg = g(grad);
-    const auto cc = g(m.cc);
-    double s = 0.0;
+    double s[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        s[i] = 0.0;
+    }
     int first = 0;
     int count = 0;
     slot_range(m, c, first, count);
@@ -389,109 +511,15 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_scalar_corrections(FvSlots m,
             break; // boundary faces follow the internal ones
         }
         const bool mine = g(m.owns)[t] != 0;
-        const int p = mine ? c : o;
-        const int n = mine ? o : c;
-        const double fl = g(flux)[f];
-        double e = 0.0;
-        if (linear_upwind != 0) {
-            const int up = fl >= 0.0 ? p : n;
-            const double r0 = g(m.cf0)[t] - cc[3 * up];
-            const double r1 = g(m.cf1)[t] - cc[3 * up + 1];
-            const double r2 = g(m.cf2)[t] - cc[3 * up + 2];
-            e -= fl * (gg[3 * up] * r0 + gg[3 * up + 1] * r1 + gg[3 * up + 2] * r2);
-        }
-        const double wf = g(m.w)[t];
-        const double gf0 = wf * gg[3 * p] + (1.0 - wf) * gg[3 * n];
-        const double gf1 = wf * gg[3 * p + 1] + (1.0 - wf) * gg[3 * n + 1];
-        const double gf2 = wf * gg[3 * p + 2] + (1.0 - wf) * gg[3 * n + 2];
-        e += g(gamma)[f] * (g(m.kv0)[t] * gf0 + g(m.kv1)[t] * gf1 + g(m.kv2)[t] * gf2);
-        s = mine ? s + e : s - e;
-    }
-    source[c] += s;
-}
-
-// Explicit internal-face fluxes of the momentum equation (fv_cell.inc:199-260): per component
-// -F_f lim_up (g_up^T (x_f - C_up)) [linear upwind, blended with the linear face value when
-// central > 0, limited when lim is given] + nu_f (g_f^T k_f) [+ nu_f g_f S_f], g_f the
-// interpolated 3x3 gradient.
-__global__ void __launch_bounds__(k_fv_block) k_fv_momentum_corrections(FvSlots m, const double* grad, const double* lim,
-                                                                        int limited, const double* flux,
-                                                                        const double* nu_face, int linear_upwind,
-                                                                        double central, const double* u,
-                                                                        int transpose_term, double* source) {
-    const int c = fv_cell();
-    if (c >= m.nc) {
-        return;
-    }
-    const auto gg = g(grad);
-    const auto fu = g(u);
-    const auto cc = g(m.cc);
-    double s[3] = {0.0, 0.0, 0.0};
-    int first = 0;
-    int count = 0;
-    slot_range(m, c, first, count);
-    for (int k = 0; k < count; ++k) {
-        const int t = first + (k << 6);
-        const int o = g(m.other)[t];
-        const int f = g(m.face)[t];
-        if (f < 0 || o < 0) {
-            break;
-        }
-        const bool mine = g(m.owns)[t] != 0;
-        const int p = mine ? c : o;
-        const int n = mine ? o : c;
-        const double fl = g(flux)[f];
-        const double wf = g(m.w)[t];
-        double e[3] = {0.0, 0.0, 0.0};
-        if (linear_upwind != 0) {
-            const int up = fl >= 0.0 ? p : n;
-            const double r0 = g(m.cf0)[t] - cc[3 * up];
-            const double r1 = g(m.cf1)[t] - cc[3 * up + 1];
-            const double r2 = g(m.cf2)[t] - cc[3 * up + 2];
-            const auto gu = gg + 9LL * up;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                double d = gu[3 * i] * r0 + gu[3 * i + 1] * r1 + gu[3 * i + 2] * r2;
-                if (central > 0.0) {
-                    const double uf = wf * fu[3 * p + i] + (1.0 - wf) * fu[3 * n + i];
-                    d = (1.0 - central) * d + central * (uf - fu[3 * up + i]);
-                }
-                if (limited != 0) {
-                    d = g(lim)[3 * up + i] * d;
-                }
-                e[i] -= fl * d;
-            }
-        }
-        double gf[9];
-#pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            gf[j] = wf * gg[9LL * p + j] + (1.0 - wf) * gg[9LL * n + j];
-        }
-        const double nu = g(nu_face)[f];
-        const double k0 = g(m.kv0)[t];
-        const double kk1 = g(m.kv1)[t];
-        const double k2 = g(m.kv2)[t];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            e[i] += nu * (gf[3 * i] * k0 + gf[3 * i + 1] * kk1 + gf[3 * i + 2] * k2);
-        }
-        if (transpose_term != 0) {
-            const double a0 = g(m.sf0)[t];
-            const double a1 = g(m.sf1)[t];
-            const double a2 = g(m.sf2)[t];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                e[i] += nu * (gf[i] * a0 + gf[i + 3] * a1 + gf[i + 6] * a2);
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            s[i] = mine ? s[i] + e[i] : s[i] - e[i];
+        for (int i = 0; i < NC; ++i) {
+            const double e = g(e_f)[NC * static_cast<long long>(f) + i];
+            s[i] = mine ? s[i] + e : s[i] - e;
         }
     }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        source[3 * c + i] += s[i];
+    for (int i = 0; i < NC; ++i) {
+        source[NC * c + i] += s[i];
     }
 }
 
@@ -566,10 +594,12 @@ struct PressureOut {
     double* rau_f;
 };
 
-// Face-major copies of the weight and area vector of every face, taken from the owner's slots
-// (each listed face has exactly one): the pressure assembly's face pass reads them coalesced.
+// Face-major copies of the weight, area vector, face centre and correction vector of every
+// face, taken from the owner's slots (each listed face has exactly one): the face passes read
+// them coalesced.
 __global__ void __launch_bounds__(k_fv_block) k_fv_faces_from_slots(FvSlots m, double* __restrict__ w,
-                                                                    double* __restrict__ sf) {
+                                                                    double* __restrict__ sf, double* __restrict__ cf,
+                                                                    double* __restrict__ kv) {
     const int c = fv_cell();
     if (c >= m.nc) {
         return;
@@ -592,6 +622,12 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_faces_from_slots(FvSlots m, d
         sf[3LL * f] = g(m.sf0)[t];
         sf[3LL * f + 1] = g(m.sf1)[t];
         sf[3LL * f + 2] = g(m.sf2)[t];
+        cf[3LL * f] = g(m.cf0)[t];
+        cf[3LL * f + 1] = g(m.cf1)[t];
+        cf[3LL * f + 2] = g(m.cf2)[t];
+        kv[3LL * f] = g(m.kv0)[t];
+        kv[3LL * f + 1] = g(m.kv1)[t];
+        kv[3LL * f + 2] = g(m.kv2)[t];
     }
 }
 
@@ -961,10 +997,18 @@ public:
         put(flux_, flux, nif_);
         put(coef_, gamma, nif_);
         put(src_, source, nc_);
-        launch([&](hipStream_t s) {
-            k_fv_scalar_corrections<<<blocks(), k_fv_block, 0, s>>>(mesh(), sgrad_.get(), flux_.get(), coef_.get(),
-                                                                    linear_upwind ? 1 : 0, src_.get());
-        });
+        ensure_faces();
+        if (eface_.size() == 0) {
+            eface_.resize(at_least_one(3LL * nif_));
+        }
+        if (nif_ > 0) { // no internal face: nothing to add
+            launch([&](hipStream_t s) {
+                k_fv_scalar_face<<<face_blocks(nif_), k_fv_block, 0, s>>>(faces(), cc_.get(), sgrad_.get(),
+                                                                          flux_.get(), coef_.get(),
+                                                                          linear_upwind ? 1 : 0, eface_.get());
+            });
+        }
+        launch([&](hipStream_t s) { k_fv_face_sum<1><<<blocks(), k_fv_block, 0, s>>>(mesh(), eface_.get(), src_.get()); });
         get(src_, source, nc_);
     }
 
@@ -990,11 +1034,18 @@ public:
                                                              venkat ? 1 : 0, e0, e1, e2, lim_.get());
             });
         }
-        launch([&](hipStream_t s) {
-            k_fv_momentum_corrections<<<blocks(), k_fv_block, 0, s>>>(
-                mesh(), vgrad_.get(), lim_.get(), use_lim ? 1 : 0, flux_.get(), coef_.get(), linear_upwind ? 1 : 0,
-                central, vphi_.get(), transpose_term ? 1 : 0, src_.get());
-        });
+        ensure_faces();
+        if (eface_.size() == 0) {
+            eface_.resize(at_least_one(3LL * nif_));
+        }
+        if (nif_ > 0) { // no internal face: nothing to add
+            launch([&](hipStream_t s) {
+                k_fv_momentum_face<<<face_blocks(nif_), k_fv_block, 0, s>>>(
+                    faces(), cc_.get(), vgrad_.get(), lim_.get(), use_lim ? 1 : 0, flux_.get(), coef_.get(),
+                    linear_upwind ? 1 : 0, central, vphi_.get(), transpose_term ? 1 : 0, eface_.get());
+            });
+        }
+        launch([&](hipStream_t s) { k_fv_face_sum<3><<<blocks(), k_fv_block, 0, s>>>(mesh(), eface_.get(), src_.get()); });
         get(src_, source, 3LL * nc_);
     }
 
@@ -1073,20 +1124,16 @@ public:
                              1.0 - in.alpha_u, majumdar ? 1 : 0,
                              in.pin_reference != 0 ? 1 : 0};
         const PressureOut pout{pdiag_.get(), pupper_.get(), plower_.get(), psrc_.get(), phih_.get(), rauf_.get()};
-        if (fw_.size() == 0) { // face-major weights and area vectors, once (zero for unlisted faces)
-            fw_.resize(at_least_one(fi));
-            fsf_.resize(at_least_one(3 * f));
+        ensure_faces();
+        if (gface_.size() == 0) {
             gface_.resize(at_least_one(f));
-            fw_.zero(stream_.get());
-            fsf_.zero(stream_.get());
-            k_fv_faces_from_slots<<<blocks(), k_fv_block, 0, stream_.get()>>>(m, fw_.get(), fsf_.get());
-            hip_check(hipGetLastError(), "k_fv_faces_from_slots launch");
         }
         const PressureFaces fa{nif_, nf_, fown_.get(), fnei_.get(), fw_.get(), fsf_.get()};
-        const int face_blocks = static_cast<int>((f + k_fv_block - 1) / k_fv_block);
-        launch([&](hipStream_t s) {
-            k_fv_face_flux<<<face_blocks, k_fv_block, 0, s>>>(fa, pin, pout, gface_.get());
-        });
+        if (f > 0) {
+            launch([&](hipStream_t s) {
+                k_fv_face_flux<<<face_blocks(f), k_fv_block, 0, s>>>(fa, pin, pout, gface_.get());
+            });
+        }
         launch([&](hipStream_t s) {
             k_fv_pressure_cells<<<blocks(), k_fv_block, 0, s>>>(m, phih_.get(), gface_.get(), pin, pout);
         });
@@ -1124,6 +1171,29 @@ public:
 
 private:
     [[nodiscard]] int nb() const { return nf_ - nif_; }
+    [[nodiscard]] int face_blocks(long long n) const { return static_cast<int>((n + k_fv_block - 1) / k_fv_block); }
+
+    // Face-major weights, area vectors, face centres and correction vectors, once (zero for a
+    // face no cell lists).
+    void ensure_faces() {
+        if (fw_.size() != 0) {
+            return;
+        }
+        hipStream_t s = stream_.get();
+        fw_.resize(at_least_one(nif_));
+        fsf_.resize(at_least_one(3LL * nf_));
+        fcf_.resize(at_least_one(3LL * nf_));
+        fkv_.resize(at_least_one(3LL * nf_));
+        for (DevBuf<double>* b : {&fw_, &fsf_, &fcf_, &fkv_}) {
+            b->zero(s);
+        }
+        k_fv_faces_from_slots<<<blocks(), k_fv_block, 0, s>>>(mesh(), fw_.get(), fsf_.get(), fcf_.get(), fkv_.get());
+        hip_check(hipGetLastError(), "k_fv_faces_from_slots launch");
+    }
+
+    [[nodiscard]] FvFaces faces() const {
+        return FvFaces{nif_, fown_.get(), fnei_.get(), fw_.get(), fsf_.get(), fcf_.get(), fkv_.get()};
+    }
     [[nodiscard]] int blocks() const { return (nc_ + k_fv_block - 1) / k_fv_block; }
 
     static void check_scheme(int scheme) {
@@ -1218,7 +1288,8 @@ private:
     DevBuf<int> kind_;
     DevBuf<double> hbya_, rau_, phih_, rauf_, pdiag_, pupper_, plower_, psrc_;
     DevBuf<int> fown_, fnei_;         // face-major owner / neighbour (pressure face pass)
-    DevBuf<double> fw_, fsf_, gface_; // face-major weight, area vector; Laplacian coefficients
+    DevBuf<double> fw_, fsf_, fcf_, fkv_; // face-major weight, area vector, face centre, correction
+    DevBuf<double> gface_, eface_;        // per face: Laplacian coefficient; explicit correction flux
     bool have_pressure_ = false;
     Event e0_;
     Event e1_;
