@@ -188,87 +188,6 @@ __device__ __forceinline__ void cell_gradient(const FvSlots& m, int c, const dou
     }
 }
 
-// Skewness-corrected Green-Gauss gradient with the least-squares gradients lg (component i of cell
-// c at lg[c * 3 NC + 3 i]): face values w phi_P + (1 - w) phi_N + (w g_P + (1 - w) g_N) . (x_f - x_ip),
-// x_ip = w C_P + (1 - w) C_N (fv_cell.inc:77-118).
-template <int NC>
-__device__ __forceinline__ void cell_gradient_corrected(const FvSlots& m, int c, const double* phi, const double* bv,
-                                                        int stride, const double* lg, double* out, int ostride) {
-    const auto fp = g(phi);
-    const auto gl = g(lg);
-    const auto cc = g(m.cc);
-    double g0[NC];
-    double g1[NC];
-    double g2[NC];
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        g0[i] = 0.0;
-        g1[i] = 0.0;
-        g2[i] = 0.0;
-    }
-    int first = 0;
-    int count = 0;
-    slot_range(m, c, first, count);
-    for (int k = 0; k < count; ++k) {
-        const int t = first + (k << 6);
-        if (g(m.face)[t] < 0) {
-            break;
-        }
-        const int o = g(m.other)[t];
-        const double s0 = g(m.sf0)[t];
-        const double s1 = g(m.sf1)[t];
-        const double s2 = g(m.sf2)[t];
-        if (o >= 0) {
-            const bool mine = g(m.owns)[t] != 0;
-            const int p = mine ? c : o;
-            const int n = mine ? o : c;
-            const double wf = g(m.w)[t];
-            const double cfj[3] = {g(m.cf0)[t], g(m.cf1)[t], g(m.cf2)[t]};
-            double r[3];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const double xip = wf * cc[3 * p + j] + (1.0 - wf) * cc[3 * n + j];
-                r[j] = cfj[j] - xip;
-            }
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                double pf = wf * fp[p * stride + i] + (1.0 - wf) * fp[n * stride + i];
-                double dot = 0.0;
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double gf = wf * gl[3 * NC * p + 3 * i + j] + (1.0 - wf) * gl[3 * NC * n + 3 * i + j];
-                    dot = j == 0 ? gf * r[0] : dot + gf * r[j];
-                }
-                pf += dot;
-                if (mine) {
-                    g0[i] += s0 * pf;
-                    g1[i] += s1 * pf;
-                    g2[i] += s2 * pf;
-                } else {
-                    g0[i] -= s0 * pf;
-                    g1[i] -= s1 * pf;
-                    g2[i] -= s2 * pf;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                const double b = g(bv)[(-1 - o) * stride + i];
-                g0[i] += s0 * b;
-                g1[i] += s1 * b;
-                g2[i] += s2 * b;
-            }
-        }
-    }
-    double* out_c = out + static_cast<long long>(c) * ostride;
-    const double v = g(m.vol)[c];
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        out_c[3 * i] = g0[i] / v;
-        out_c[3 * i + 1] = g1[i] / v;
-        out_c[3 * i + 2] = g2[i] / v;
-    }
-}
 
 template <bool Lsq, int NC>
 __global__ void __launch_bounds__(k_fv_block) k_fv_gradient(FvSlots m, const double* phi, const double* bv, int stride,
@@ -279,15 +198,6 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_gradient(FvSlots m, const dou
     }
 }
 
-template <int NC>
-__global__ void __launch_bounds__(k_fv_block) k_fv_gradient_corrected(FvSlots m, const double* phi, const double* bv,
-                                                                      int stride, const double* lg, double* out,
-                                                                      int ostride) {
-    const int c = fv_cell();
-    if (c < m.nc) {
-        cell_gradient_corrected<NC>(m, c, phi, bv, stride, lg, out, ostride);
-    }
-}
 
 // Limiter of the three components of u (fv_cell.inc:123-164): bounds from the cell and its face
 // neighbours (boundary values for boundary faces), then per face and component the ratio of the
@@ -520,6 +430,112 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_face_sum(FvSlots m, const dou
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         source[NC * c + i] += s[i];
+    }
+}
+
+// Face pass of the skewness-corrected Gauss gradient (fv_cell.inc:77-118): the corrected face
+// value of each internal face, NC components, phi_f = w phi_P + (1 - w) phi_N + g_f . (x_f - x_f'),
+// g_f the interpolated least-squares gradient lg and x_f' = w C_P + (1 - w) C_N, as each cell
+// of the face would form it.
+template <int NC>
+__global__ void __launch_bounds__(k_fv_block) k_fv_face_value(FvFaces fa, const double* __restrict__ cc,
+                                                              const double* __restrict__ phi, int stride,
+                                                              const double* __restrict__ lg,
+                                                              double* __restrict__ pf_out) {
+    const int f = fv_cell();
+    if (f >= fa.nif) {
+        return;
+    }
+    const auto fp = g(phi);
+    const auto gl = g(lg);
+    const auto cg = g(cc);
+    const int p = g(fa.own)[f];
+    const int n = g(fa.nei)[f];
+    const double wf = g(fa.w)[f];
+    double r[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const double xip = wf * cg[3 * p + j] + (1.0 - wf) * cg[3 * n + j];
+        r[j] = g(fa.cf)[3LL * f + j] - xip;
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        double pf = wf * fp[p * stride + i] + (1.0 - wf) * fp[n * stride + i];
+        double dot = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double gf = wf * gl[3 * NC * p + 3 * i + j] + (1.0 - wf) * gl[3 * NC * n + 3 * i + j];
+            dot = j == 0 ? gf * r[0] : dot + gf * r[j];
+        }
+        pf_out[NC * static_cast<long long>(f) + i] = pf + dot;
+    }
+}
+
+// Cell pass of the corrected Gauss gradient: sum over the cell's faces in face order of
+// +-S_f phi_f (internal faces, values from k_fv_face_value) and S_f phi_b (boundary faces),
+// divided by the volume.
+template <int NC>
+__global__ void __launch_bounds__(k_fv_block) k_fv_gauss_from_faces(FvSlots m, const double* __restrict__ pf_f,
+                                                                    const double* __restrict__ bv, int stride,
+                                                                    double* __restrict__ out, int ostride) {
+    const int c = fv_cell();
+    if (c >= m.nc) {
+        return;
+    }
+    double g0[NC];
+    double g1[NC];
+    double g2[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        g0[i] = 0.0;
+        g1[i] = 0.0;
+        g2[i] = 0.0;
+    }
+    int first = 0;
+    int count = 0;
+    slot_range(m, c, first, count);
+    for (int k = 0; k < count; ++k) {
+        const int t = first + (k << 6);
+        const int f = g(m.face)[t];
+        if (f < 0) {
+            break;
+        }
+        const int o = g(m.other)[t];
+        const double s0 = g(m.sf0)[t];
+        const double s1 = g(m.sf1)[t];
+        const double s2 = g(m.sf2)[t];
+        if (o >= 0) {
+            const bool mine = g(m.owns)[t] != 0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const double pf = g(pf_f)[NC * static_cast<long long>(f) + i];
+                if (mine) {
+                    g0[i] += s0 * pf;
+                    g1[i] += s1 * pf;
+                    g2[i] += s2 * pf;
+                } else {
+                    g0[i] -= s0 * pf;
+                    g1[i] -= s1 * pf;
+                    g2[i] -= s2 * pf;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const double b = g(bv)[(-1 - o) * stride + i];
+                g0[i] += s0 * b;
+                g1[i] += s1 * b;
+                g2[i] += s2 * b;
+            }
+        }
+    }
+    double* out_c = out + static_cast<long long>(c) * ostride;
+    const double v = g(m.vol)[c];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        out_c[3 * i] = g0[i] / v;
+        out_c[3 * i + 1] = g1[i] / v;
+        out_c[3 * i + 2] = g2[i] / v;
     }
 }
 
@@ -1218,8 +1234,18 @@ private:
             launch([&](hipStream_t s) {
                 k_fv_gradient<true, NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, tmp, 3 * NC);
             });
+            ensure_faces();
+            if (eface_.size() < static_cast<std::size_t>(at_least_one(3LL * nif_))) {
+                eface_.resize(at_least_one(3LL * nif_));
+            }
+            if (nif_ > 0) {
+                launch([&](hipStream_t s) {
+                    k_fv_face_value<NC><<<face_blocks(nif_), k_fv_block, 0, s>>>(faces(), cc_.get(), phi, NC, tmp,
+                                                                                 eface_.get());
+                });
+            }
             launch([&](hipStream_t s) {
-                k_fv_gradient_corrected<NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, tmp, out, ostride);
+                k_fv_gauss_from_faces<NC><<<blocks(), k_fv_block, 0, s>>>(m, eface_.get(), bv, NC, out, ostride);
             });
             return;
         }
