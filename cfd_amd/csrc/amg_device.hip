@@ -241,59 +241,104 @@ __global__ void k_fill_one(int n, double* __restrict__ a) {
     }
 }
 
-// Dense-row products for small, dense levels: thread i accumulates row i of A B in its own
-// n_cols-wide slice of scratch (mark: first touch), in the order of the host setup, then counts
-// (rp == nullptr) or writes its entries with ascending columns.
-__global__ void k_dense_rows(int n, int n_cols, const int* __restrict__ arp, const int* __restrict__ aci,
-                             const double* __restrict__ ava, const int* __restrict__ brp, const int* __restrict__ bci,
-                             const double* __restrict__ bva, const double* __restrict__ lscale,
-                             double* __restrict__ acc_all, int* __restrict__ mark_all, long long* __restrict__ cnt) {
-    const int i = blockIdx.x * k_blk + threadIdx.x;
+// Dense-row products for small levels with long product lists: one wavefront per row of A B,
+// the row accumulated in LDS (a value and a first-touch mark per column; lane l owns the columns
+// j = l mod 64). The row's products are taken 64 at a time in the host's order and applied one
+// after the other, each by the lane that owns its column, so every column's sum has the host's
+// operand order (first product, then +=). The wavefront then writes the row's values and marks
+// to its scratch row (coalesced) and its count of columns. k_wave_dense = 64 threads per block.
+constexpr int k_wave_dense = 64;
+constexpr int k_dense_max_cols = 5400; // 12 bytes of LDS per column, within 64 KiB
+
+__global__ void __launch_bounds__(k_wave_dense) k_dense_rows(int n, int n_cols, const int* __restrict__ arp,
+                                                             const int* __restrict__ aci,
+                                                             const double* __restrict__ ava,
+                                                             const int* __restrict__ brp,
+                                                             const int* __restrict__ bci,
+                                                             const double* __restrict__ bva,
+                                                             const double* __restrict__ lscale,
+                                                             double* __restrict__ acc_all, int* __restrict__ mark_all,
+                                                             long long* __restrict__ cnt) {
+    extern __shared__ double lds[];
+    double* acc = lds;
+    int* mark = reinterpret_cast<int*>(lds + n_cols);
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
     if (i >= n) {
         return;
     }
-    double* acc = acc_all + static_cast<long long>(i) * n_cols;
-    int* mark = mark_all + static_cast<long long>(i) * n_cols;
-    for (int j = 0; j < n_cols; ++j) {
+    for (int j = lane; j < n_cols; j += k_wave_dense) {
         mark[j] = 0;
     }
+    __syncthreads();
     const double s = lscale != nullptr ? lscale[i] : 1.0;
     for (int ka = arp[i]; ka < arp[i + 1]; ++ka) {
         const int k = aci[ka];
         const double y = lscale != nullptr ? s * ava[ka] : ava[ka];
-        for (int kb = brp[k]; kb < brp[k + 1]; ++kb) {
-            const int j = bci[kb];
-            const double prod = bva[kb] * y;
-            if (mark[j] == 0) {
-                mark[j] = 1;
-                acc[j] = prod;
-            } else {
-                acc[j] += prod;
+        for (int kb0 = brp[k]; kb0 < brp[k + 1]; kb0 += k_wave_dense) {
+            const int m = min(k_wave_dense, brp[k + 1] - kb0);
+            int jl = 0;
+            double pl = 0.0;
+            if (lane < m) {
+                jl = bci[kb0 + lane];
+                pl = bva[kb0 + lane] * y;
+            }
+            for (int t = 0; t < m; ++t) { // in order; the owner lane of column j applies it
+                const int j = __shfl(jl, t, k_wave_dense);
+                const double prod = __shfl(pl, t, k_wave_dense);
+                if ((j & (k_wave_dense - 1)) == lane) {
+                    if (mark[j] == 0) {
+                        mark[j] = 1;
+                        acc[j] = prod;
+                    } else {
+                        acc[j] += prod;
+                    }
+                }
             }
         }
     }
+    __syncthreads();
+    double* acc_row = acc_all + static_cast<long long>(i) * n_cols;
+    int* mark_row = mark_all + static_cast<long long>(i) * n_cols;
     long long c = 0;
-    for (int j = 0; j < n_cols; ++j) {
+    for (int j = lane; j < n_cols; j += k_wave_dense) {
+        mark_row[j] = mark[j];
+        acc_row[j] = mark[j] != 0 ? acc[j] : 0.0;
         c += mark[j];
     }
-    cnt[i] = c;
+    for (int off = 32; off > 0; off >>= 1) {
+        c += __shfl_xor(c, off, k_wave_dense);
+    }
+    if (lane == 0) {
+        cnt[i] = c;
+    }
 }
 
-__global__ void k_dense_write(int n, int n_cols, const double* __restrict__ acc_all, const int* __restrict__ mark_all,
-                              const long long* __restrict__ off, Key* __restrict__ key, double* __restrict__ val) {
-    const int i = blockIdx.x * k_blk + threadIdx.x;
+// The marked columns of each scratch row, ascending, at the row's offset: one wavefront per
+// row, 64 columns per step, positions by ballot.
+__global__ void __launch_bounds__(k_wave_dense) k_dense_write(int n, int n_cols, const double* __restrict__ acc_all,
+                                                              const int* __restrict__ mark_all,
+                                                              const long long* __restrict__ off,
+                                                              Key* __restrict__ key, double* __restrict__ val) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
     if (i >= n) {
         return;
     }
     const double* acc = acc_all + static_cast<long long>(i) * n_cols;
     const int* mark = mark_all + static_cast<long long>(i) * n_cols;
     long long q = off[i];
-    for (int j = 0; j < n_cols; ++j) {
-        if (mark[j] != 0) {
-            key[q] = make_key(i, j);
-            val[q] = acc[j];
-            ++q;
+    const unsigned long long below = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+    for (int j0 = 0; j0 < n_cols; j0 += k_wave_dense) {
+        const int j = j0 + lane;
+        const bool on = j < n_cols && mark[j] != 0;
+        const unsigned long long bits = __ballot(on);
+        if (on) {
+            const long long pos = q + __popcll(bits & below);
+            key[pos] = make_key(i, j);
+            val[pos] = acc[j];
         }
+        q += __popcll(bits);
     }
 }
 
@@ -430,18 +475,19 @@ void spgemm(const DCsr& a, const DCsr& b, const double* lscale, bool prune, DCsr
     const long long total = last_of(off.get(), cnt.get(), n, s);
     // Small level with long product lists: dense accumulation per row instead of sorting them.
     const long long cells = static_cast<long long>(n) * b.n_cols;
-    if (cells <= (1LL << 25) && total >= 32LL * n) {
+    if (n > 0 && cells <= (1LL << 25) && b.n_cols <= k_dense_max_cols && total >= 32LL * n) {
         DevBuf<double> acc(static_cast<std::size_t>(cells > 0 ? cells : 1));
         DevBuf<int> mark(static_cast<std::size_t>(cells > 0 ? cells : 1));
-        k_dense_rows<<<blocks_for(n), k_blk, 0, s>>>(n, b.n_cols, a.rp.get(), a.ci.get(), a.va.get(), b.rp.get(),
-                                                     b.ci.get(), b.va.get(), lscale, acc.get(), mark.get(), cnt.get());
+        const std::size_t lds = static_cast<std::size_t>(b.n_cols) * (sizeof(double) + sizeof(int));
+        k_dense_rows<<<n, k_wave_dense, lds, s>>>(n, b.n_cols, a.rp.get(), a.ci.get(), a.va.get(), b.rp.get(),
+                                                  b.ci.get(), b.va.get(), lscale, acc.get(), mark.get(), cnt.get());
         hip_check(hipGetLastError(), "dense rows launch");
         exclusive_sum(cnt.get(), off.get(), n, s);
         const long long kept = last_of(off.get(), cnt.get(), n, s);
         DevBuf<Key> dkey(static_cast<std::size_t>(kept > 0 ? kept : 1));
         DevBuf<double> dval(static_cast<std::size_t>(kept > 0 ? kept : 1));
-        k_dense_write<<<blocks_for(n), k_blk, 0, s>>>(n, b.n_cols, acc.get(), mark.get(), off.get(), dkey.get(),
-                                                      dval.get());
+        k_dense_write<<<n, k_wave_dense, 0, s>>>(n, b.n_cols, acc.get(), mark.get(), off.get(), dkey.get(),
+                                                 dval.get());
         hip_check(hipGetLastError(), "dense write launch");
         hip_check(hipStreamSynchronize(s), "dense rows sync"); // the scratch goes back to the pool
         entries_to_csr(dkey, dval, static_cast<int>(kept), n, b.n_cols, prune, c, s);
