@@ -342,6 +342,161 @@ __global__ void __launch_bounds__(k_wave_dense) k_dense_write(int n, int n_cols,
     }
 }
 
+// ---- Aggregation (amg.cpp:12-78), passes 1 and 2 ----
+
+// Strength flags per entry, |a_ij| > theta sqrt(|d_i d_j|) with i != j (amg.cpp:16-18), the
+// count of strong entries per row and, for the transposed pattern, keys (j, i) of them.
+__global__ void k_strength(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ va, const double* __restrict__ d, double theta,
+                           unsigned char* __restrict__ strong, long long* __restrict__ cnt) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const double di = d[i];
+    long long c = 0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        const int j = ci[k];
+        const bool st = i != j && fabs(va[k]) > theta * sqrt(fabs(di * d[j]));
+        strong[k] = st ? 1 : 0;
+        c += st ? 1 : 0;
+    }
+    cnt[i] = c;
+}
+
+__global__ void k_strong_keys(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                              const unsigned char* __restrict__ strong, const long long* __restrict__ off,
+                              Key* __restrict__ key) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    long long q = off[i];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        if (strong[k] != 0) {
+            key[q++] = make_key(ci[k], i);
+        }
+    }
+}
+
+// Roots of pass 1 in rounds (amg.cpp:22-46). Node i is a root iff it has a strong neighbour and
+// no root s < i claimed i or one of its strong neighbours j before i's turn, i.e. no root s < i in
+// T(i), S(i) or T(S(i)) (S: strong neighbours, T: the nodes whose strong set holds the node). A
+// node decides as soon as such a root is known (not a root) or every s < i there is decided (a
+// root). Statuses: 0 undecided, 1 root, 2 not a root; a round reads `in` and writes `out`, so each
+// round is deterministic. The smallest undecided node always decides, so the rounds end, and the
+// roots are those of the sequential loop.
+__global__ void k_root_round(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                             const unsigned char* __restrict__ strong, const int* __restrict__ trp,
+                             const int* __restrict__ tci, const int* __restrict__ in, int* __restrict__ out) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const int st = in[i];
+    if (st != 0) {
+        out[i] = st;
+        return;
+    }
+    bool known = true;
+    bool claimed = false;
+    auto look = [&](int v) {
+        if (v < i) {
+            const int x = in[v];
+            claimed = claimed || x == 1;
+            known = known && x != 0;
+        }
+    };
+    for (int t = trp[i]; t < trp[i + 1] && !claimed; ++t) {
+        look(tci[t]);
+    }
+    for (int k = rp[i]; k < rp[i + 1] && !claimed; ++k) {
+        if (strong[k] == 0) {
+            continue;
+        }
+        const int j = ci[k];
+        look(j);
+        for (int t = trp[j]; t < trp[j + 1] && !claimed; ++t) {
+            look(tci[t]);
+        }
+    }
+    out[i] = claimed ? 2 : (known ? 1 : 0);
+}
+
+// Per-block count of undecided nodes.
+__global__ void k_count_undecided(int n, const int* __restrict__ st, int* __restrict__ part) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    const int c = __syncthreads_count(i < n && st[i] == 0);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = c;
+    }
+}
+
+__global__ void k_root_flags(int n, const int* __restrict__ st, int* __restrict__ flag) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    if (i < n) {
+        flag[i] = st[i] == 1 ? 1 : 0;
+    }
+}
+
+// Pass-1 aggregates: a root's index among the roots, given to it and its strong neighbours (the
+// strong sets of roots are disjoint); -1 elsewhere (agg1 filled with -1 before).
+__global__ void k_claim(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                        const unsigned char* __restrict__ strong, const int* __restrict__ st,
+                        const int* __restrict__ id, int* __restrict__ agg1) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    if (i >= n || st[i] != 1) {
+        return;
+    }
+    const int a = id[i];
+    agg1[i] = a;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        if (strong[k] != 0) {
+            agg1[ci[k]] = a;
+        }
+    }
+}
+
+// Pass 2 (amg.cpp:48-63): a node left free takes the pass-1 aggregate of its strongest strong
+// neighbour (first maximum of |a_ij| in row order), from the pass-1 snapshot.
+__global__ void k_pass2(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                        const double* __restrict__ va, const unsigned char* __restrict__ strong,
+                        const int* __restrict__ agg1, int* __restrict__ agg) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    if (agg1[i] >= 0) {
+        agg[i] = agg1[i];
+        return;
+    }
+    double best = 0.0;
+    int choice = -1;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        const int j = ci[k];
+        if (strong[k] != 0 && agg1[j] >= 0 && fabs(va[k]) > best) {
+            best = fabs(va[k]);
+            choice = agg1[j];
+        }
+    }
+    agg[i] = choice;
+}
+
+// Undecided (0), or not a root (2) for a row without a strong entry.
+__global__ void k_init_status(int n, const long long* __restrict__ cnt, int* __restrict__ st) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    if (i < n) {
+        st[i] = cnt[i] > 0 ? 0 : 2;
+    }
+}
+
+__global__ void k_fill_int(int n, int* __restrict__ x, int v) {
+    const int i = blockIdx.x * k_blk + threadIdx.x;
+    if (i < n) {
+        x[i] = v;
+    }
+}
+
 template <class T>
 void exclusive_sum(const T* in, T* out, long long n, hipStream_t s) {
     std::size_t bytes = 0;
@@ -531,6 +686,91 @@ DeviceSetup::DeviceSetup(const Csr& a0) : cur_(std::make_unique<DCsr>()) {
 }
 
 DeviceSetup::~DeviceSetup() { (void)hipStreamSynchronize(stream_.get()); }
+
+std::vector<int> DeviceSetup::aggregate12(const std::vector<double>& d, double theta, int& n_roots) {
+    hipStream_t s = stream_.get();
+    const DCsr& a = *cur_;
+    const int n = a.n_rows;
+    require(static_cast<int>(d.size()) == n, "device aggregation: diagonal does not match the level");
+    std::vector<int> agg(static_cast<std::size_t>(n), -1);
+    n_roots = 0;
+    if (n == 0) {
+        return agg;
+    }
+    DevBuf<double> dd;
+    dd.upload(d.data(), d.size(), s);
+    DevBuf<unsigned char> strong(static_cast<std::size_t>(a.nnz > 0 ? a.nnz : 1));
+    DevBuf<long long> cnt(static_cast<std::size_t>(n));
+    DevBuf<long long> off(static_cast<std::size_t>(n));
+    k_strength<<<blocks_for(n), k_blk, 0, s>>>(n, a.rp.get(), a.ci.get(), a.va.get(), dd.get(), theta, strong.get(),
+                                               cnt.get());
+    hip_check(hipGetLastError(), "strength launch");
+    exclusive_sum(cnt.get(), off.get(), n, s);
+    const long long m = last_of(off.get(), cnt.get(), n, s);
+    require(m < static_cast<long long>(INT_MAX), "device aggregation: too many strong entries");
+    // T, the transposed strength pattern: rows j listing the i with j in S(i), ascending.
+    DCsr t;
+    {
+        DevBuf<Key> key(static_cast<std::size_t>(m > 0 ? m : 1));
+        DevBuf<double> val(static_cast<std::size_t>(m > 0 ? m : 1));
+        k_strong_keys<<<blocks_for(n), k_blk, 0, s>>>(n, a.rp.get(), a.ci.get(), strong.get(), off.get(), key.get());
+        hip_check(hipGetLastError(), "strong keys launch");
+        sort_pairs(key, val, m, 32 + bits_for(n), s);
+        entries_to_csr(key, val, static_cast<int>(m), n, n, false, t, s);
+    }
+    // Pass 1 in rounds; the nodes without a strong neighbour are decided (not roots) at once.
+    DevBuf<int> st0(static_cast<std::size_t>(n));
+    DevBuf<int> st1(static_cast<std::size_t>(n));
+    const int nb = blocks_for(n);
+    k_init_status<<<nb, k_blk, 0, s>>>(n, cnt.get(), st0.get());
+    hip_check(hipGetLastError(), "status launch");
+    DevBuf<int> part(static_cast<std::size_t>(nb));
+    std::vector<int> part_h(static_cast<std::size_t>(nb));
+    long long rounds = 0;
+    constexpr int k_rounds_per_check = 16;
+    for (;;) {
+        for (int r = 0; r < k_rounds_per_check; ++r) {
+            k_root_round<<<nb, k_blk, 0, s>>>(n, a.rp.get(), a.ci.get(), strong.get(), t.rp.get(), t.ci.get(), st0.get(),
+                                              st1.get());
+            std::swap(st0, st1);
+        }
+        hip_check(hipGetLastError(), "root round launch");
+        rounds += k_rounds_per_check;
+        k_count_undecided<<<nb, k_blk, 0, s>>>(n, st0.get(), part.get());
+        hip_check(hipMemcpyAsync(part_h.data(), part.get(), part_h.size() * sizeof(int), hipMemcpyDeviceToHost, s),
+                  "D2H");
+        hip_check(hipStreamSynchronize(s), "root rounds sync");
+        long long undecided = 0;
+        for (int v : part_h) {
+            undecided += v;
+        }
+        if (undecided == 0) {
+            break;
+        }
+        // every round decides at least the smallest undecided node
+        require(rounds <= static_cast<long long>(n) + k_rounds_per_check, "device aggregation: rounds did not end");
+    }
+    DevBuf<int> flag(static_cast<std::size_t>(n));
+    DevBuf<int> id(static_cast<std::size_t>(n));
+    k_root_flags<<<nb, k_blk, 0, s>>>(n, st0.get(), flag.get());
+    hip_check(hipGetLastError(), "root flags launch");
+    exclusive_sum(flag.get(), id.get(), n, s);
+    int last_id = 0;
+    int last_flag = 0;
+    hip_check(hipMemcpyAsync(&last_id, id.get() + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipMemcpyAsync(&last_flag, flag.get() + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipStreamSynchronize(s), "roots sync");
+    n_roots = last_id + last_flag;
+    DevBuf<int> agg1(static_cast<std::size_t>(n));
+    DevBuf<int> agg2(static_cast<std::size_t>(n));
+    k_fill_int<<<nb, k_blk, 0, s>>>(n, agg1.get(), -1);
+    k_claim<<<nb, k_blk, 0, s>>>(n, a.rp.get(), a.ci.get(), strong.get(), st0.get(), id.get(), agg1.get());
+    k_pass2<<<nb, k_blk, 0, s>>>(n, a.rp.get(), a.ci.get(), a.va.get(), strong.get(), agg1.get(), agg2.get());
+    hip_check(hipGetLastError(), "aggregation launch");
+    hip_check(hipMemcpyAsync(agg.data(), agg2.get(), agg.size() * sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipStreamSynchronize(s), "aggregation sync"); // the buffers go back to the pool
+    return agg;
+}
 
 void DeviceSetup::level(const std::vector<double>& inv_diag, const std::vector<int>& agg, int n_agg, double omega,
                         Csr& p_out, Csr& r_out, Csr& coarse_out) {

@@ -23,6 +23,10 @@ public:
 
     void level(const std::vector<double>& inv_diag, const std::vector<int>& agg, int n_agg, double omega, Csr& p,
                Csr& r, Csr& coarse);
+    // Passes 1 and 2 of the greedy aggregation (amg.cpp:12-63) of the current level on the device,
+    // bit-identical to the sequential loops: the aggregate of every node, -1 for the nodes pass 3
+    // must place; n_roots = the aggregates of pass 1.
+    std::vector<int> aggregate12(const std::vector<double>& d, double theta, int& n_roots);
 
     struct DCsr;
 

@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
+#include <string>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -128,6 +130,42 @@ std::vector<int> aggregate(const Csr& a, const std::vector<double>& d, double th
         }
     }
     return agg;
+}
+
+// Whether the device setup aggregates a level on the device (passes 1 and 2 as parallel root
+// rounds, the same aggregates as the sequential passes): CFDG_DEVICE_AGGREGATION = "all" for every
+// level, "short" for levels whose rows hold at most 8 entries on average, anything else or unset
+// for none. The host passes are the default: on 96^3-160^3 boxes they build the hierarchy faster
+// than the rounds, whose two-hop root checks repeat until every node is decided.
+bool device_aggregation(const Csr& a) {
+    const char* e = std::getenv("CFDG_DEVICE_AGGREGATION");
+    const std::string mode = e != nullptr ? e : "";
+    if (mode == "all") {
+        return true;
+    }
+    return mode == "short" && a.n_rows > 0 && a.nnz() <= 8LL * a.n_rows;
+}
+
+// Pass 3 of the aggregation (amg.cpp:65-76) on the aggregates of passes 1 and 2: the nodes still
+// free, in order, found aggregates with their strong neighbours that are still free.
+void aggregate_pass3(const Csr& a, const std::vector<double>& d, double theta, std::vector<int>& agg, int& n_agg) {
+    const int n = a.n_rows;
+    for (int i = 0; i < n; ++i) {
+        if (agg[static_cast<std::size_t>(i)] >= 0) {
+            continue;
+        }
+        agg[static_cast<std::size_t>(i)] = n_agg;
+        const double di = d[static_cast<std::size_t>(i)];
+        for (int k = a.rp[static_cast<std::size_t>(i)]; k < a.rp[static_cast<std::size_t>(i) + 1]; ++k) {
+            const int j = a.ci[static_cast<std::size_t>(k)];
+            const bool strong = i != j && std::abs(a.va[static_cast<std::size_t>(k)]) >
+                                              theta * std::sqrt(std::abs(di * d[static_cast<std::size_t>(j)]));
+            if (strong && agg[static_cast<std::size_t>(j)] < 0) {
+                agg[static_cast<std::size_t>(j)] = n_agg;
+            }
+        }
+        ++n_agg;
+    }
 }
 
 // Row-wise sparse product C = A * B (Gustavson). Entry (i, j) is the left-to-right sum over the
@@ -469,8 +507,9 @@ std::vector<double> coarse_inverse(const Csr& a) {
     return inv;
 }
 
-// The setup loop (amg.cpp:128-190). With `device`, the sparse products of every level run on
-// the GPU (DeviceSetup, bit-identical results) and only the aggregation stays here.
+// The setup loop (amg.cpp:128-190). With `device`, the sparse products and the first two
+// aggregation passes of every level run on the GPU (DeviceSetup, bit-identical results); the
+// third pass, over the few nodes the first two leave, stays here.
 void build(AmgHierarchy& h, Csr a0, bool device = false) {
     const cfdg_amg_settings& s = h.settings;
     h.levels.clear();
@@ -494,7 +533,13 @@ void build(AmgHierarchy& h, Csr a0, bool device = false) {
         }
         int n_agg = 0;
         const double theta = s.strength_threshold * std::pow(s.strength_decay, static_cast<double>(l));
-        std::vector<int> agg = aggregate(current, d, theta, n_agg);
+        std::vector<int> agg;
+        if (dev && device_aggregation(current)) { // passes 1 and 2 on the device, pass 3 here
+            agg = dev->aggregate12(d, theta, n_agg);
+            aggregate_pass3(current, d, theta, agg, n_agg);
+        } else {
+            agg = aggregate(current, d, theta, n_agg);
+        }
         if (n_agg >= current.n_rows || n_agg == 0) { // amg.cpp:145-148
             h.levels.push_back(std::move(level));
             break;

@@ -1,7 +1,9 @@
 """Smoothed-aggregation setup with the sparse products on the GPU (cfdg_amg_build_device)
 against the host setup (cfdg_amg_build, pinned to the reference by tests/test_oracle_pins.py and
 the golden fixtures): every level's operator, prolongation, restriction and aggregates and the
-coarsest inverse are bit-identical, so the hierarchy, and every solve with it, is the same.
+coarsest inverse are bit-identical, so the hierarchy, and every solve with it, is the same. Each
+case runs with the aggregation on the host (the default) and with its first two passes as
+parallel root rounds on the GPU (CFDG_DEVICE_AGGREGATION=all).
 """
 from __future__ import annotations
 
@@ -11,6 +13,15 @@ import pytest
 from cfd_amd import AmgHierarchy, AmgSettings, cases
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=["host", "all"])
+def aggregation(request, monkeypatch):
+    if request.param == "host":
+        monkeypatch.delenv("CFDG_DEVICE_AGGREGATION", raising=False)
+    else:
+        monkeypatch.setenv("CFDG_DEVICE_AGGREGATION", request.param)
+    return request.param
 
 
 def same_hierarchy(h, d):
@@ -29,7 +40,7 @@ def same_hierarchy(h, d):
 
 @pytest.mark.parametrize("shape,jitter,decay", [((24, 20, 18), 0.2, 0.5), ((40, 40, 40), 0.0, 1.0),
                                                  ((64, 48, 30), 0.25, 0.5), ((7, 5, 3), 0.0, 0.5)])
-def test_device_setup_matches_host_setup(shape, jitter, decay):
+def test_device_setup_matches_host_setup(aggregation, shape, jitter, decay):
     mesh = cases.box_mesh(*shape, hi=(1.0, 1.0, shape[2] / shape[0]), jitter=jitter, seed=sum(shape))
     s = cases.pressure_system(mesh, rhs_seed=3)
     pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
@@ -37,11 +48,11 @@ def test_device_setup_matches_host_setup(shape, jitter, decay):
     settings = AmgSettings(strength_decay=decay)
     h = AmgHierarchy(a, settings)
     d = AmgHierarchy(a, settings, device=True)
-    print(f"{shape}: levels {h.level_sizes()}, host {h.setup_ms:.1f} ms, device {d.setup_ms:.1f} ms", flush=True)
+    print(f"{shape} ({aggregation} aggregation): levels {h.level_sizes()}, host {h.setup_ms:.1f} ms, device {d.setup_ms:.1f} ms", flush=True)
     same_hierarchy(h, d)
 
 
-def test_device_setup_of_a_variable_coefficient_operator():
+def test_device_setup_of_a_variable_coefficient_operator(aggregation):
     mesh = cases.box_mesh(30, 30, 30, jitter=0.2, seed=9)
     rng = np.random.default_rng(2)
     kappa = np.exp(rng.uniform(-3.0, 3.0, mesh.n_cells))  # strongly varying cell coefficients
