@@ -94,10 +94,80 @@ __device__ __forceinline__ void slot_range(const FvSlots& m, int c, int& first, 
     count = (g(m.base)[s + 1] - b0) >> 6;
 }
 
+// One slot's operands for a gradient: the face (-1 for padding), the slot's vector (least-squares
+// d / |d|^2 or the area vector), the weight and the other side's NC values (the neighbour's, or the
+// boundary face's). Every load stays in bounds for padding slots too: their `other` is cell 0 and
+// their vectors are zero, and boundary indices were validated on the host.
+template <bool Lsq, int NC>
+struct GradSlot {
+    int f;
+    bool mine;
+    bool internal;
+    double v0, v1, v2, w;
+    double u[NC];
+};
+
+template <bool Lsq, int NC>
+__device__ __forceinline__ GradSlot<Lsq, NC> load_grad_slot(const FvSlots& m, int t, const double* phi,
+                                                            const double* bv, int stride) {
+    GradSlot<Lsq, NC> sl;
+    sl.f = g(m.face)[t];
+    const int o = g(m.other)[t];
+    sl.mine = g(m.owns)[t] != 0;
+    sl.internal = o >= 0;
+    sl.v0 = Lsq ? g(m.ld0)[t] : g(m.sf0)[t];
+    sl.v1 = Lsq ? g(m.ld1)[t] : g(m.sf1)[t];
+    sl.v2 = Lsq ? g(m.ld2)[t] : g(m.sf2)[t];
+    sl.w = Lsq ? 0.0 : g(m.w)[t];
+    const long long at = o >= 0 ? static_cast<long long>(o) * stride : static_cast<long long>(-1 - o) * stride;
+    const double* src = o >= 0 ? phi : bv;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        sl.u[i] = g(src)[at + i];
+    }
+    return sl;
+}
+
+// Adds one face's term in the reference's arithmetic: dphi = phi_N - phi_P (the boundary value
+// minus the cell's for boundary faces) times d, or +-S_f phi_f with phi_f = w phi_P + (1 - w) phi_N.
+// pc holds the calling cell's values, which equal phi[c * stride + i] bit for bit.
+template <bool Lsq, int NC>
+__device__ __forceinline__ void add_grad_slot(const GradSlot<Lsq, NC>& sl, const double* pc, double* g0, double* g1,
+                                              double* g2) {
+    if (sl.f < 0) {
+        return; // padding
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        if (Lsq) {
+            const double dv = !sl.internal ? sl.u[i] - pc[i] : (sl.mine ? sl.u[i] - pc[i] : pc[i] - sl.u[i]);
+            g0[i] += sl.v0 * dv;
+            g1[i] += sl.v1 * dv;
+            g2[i] += sl.v2 * dv;
+        } else if (sl.internal) {
+            const double pf = sl.mine ? sl.w * pc[i] + (1.0 - sl.w) * sl.u[i] : sl.w * sl.u[i] + (1.0 - sl.w) * pc[i];
+            if (sl.mine) {
+                g0[i] += sl.v0 * pf;
+                g1[i] += sl.v1 * pf;
+                g2[i] += sl.v2 * pf;
+            } else {
+                g0[i] -= sl.v0 * pf;
+                g1[i] -= sl.v1 * pf;
+                g2[i] -= sl.v2 * pf;
+            }
+        } else {
+            g0[i] += sl.v0 * sl.u[i];
+            g1[i] += sl.v1 * sl.u[i];
+            g2[i] += sl.v2 * sl.u[i];
+        }
+    }
+}
+
 // Gradient of NC components of a field with `stride` values per cell and boundary face; component
 // i goes to out[c * ostride + 3 i + j]. Least squares: A (sum_f d_f dphi_f), A the cell's
 // column-major inverse; Green-Gauss: sum_f S_f phi_f / V with phi_f = w phi_P + (1 - w) phi_N
-// (fv_cell.inc:25-73).
+// (fv_cell.inc:25-73). Slots are loaded two at a time ahead of their terms, which are added in
+// slot (ascending face) order; padding slots add nothing.
 template <bool Lsq, int NC>
 __device__ __forceinline__ void cell_gradient(const FvSlots& m, int c, const double* phi, const double* bv, int stride,
                                               double* out, int ostride) {
@@ -116,55 +186,16 @@ __device__ __forceinline__ void cell_gradient(const FvSlots& m, int c, const dou
     int first = 0;
     int count = 0;
     slot_range(m, c, first, count);
-    for (int k = 0; k < count; ++k) {
+    int k = 0;
+    for (; k + 2 <= count; k += 2) {
         const int t = first + (k << 6);
-        if (g(m.face)[t] < 0) {
-            break; // padding follows the cell's faces
-        }
-        const int o = g(m.other)[t];
-        const bool mine = g(m.owns)[t] != 0;
-        const int p = mine ? c : o;
-        const int n = mine ? o : c;
-        if (Lsq) {
-            const double d0 = g(m.ld0)[t];
-            const double d1 = g(m.ld1)[t];
-            const double d2 = g(m.ld2)[t];
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                const double dv = o >= 0 ? fp[n * stride + i] - fp[p * stride + i] : g(bv)[(-1 - o) * stride + i] - pc[i];
-                g0[i] += d0 * dv;
-                g1[i] += d1 * dv;
-                g2[i] += d2 * dv;
-            }
-        } else {
-            const double s0 = g(m.sf0)[t];
-            const double s1 = g(m.sf1)[t];
-            const double s2 = g(m.sf2)[t];
-            if (o >= 0) {
-                const double wf = g(m.w)[t];
-#pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    const double pf = wf * fp[p * stride + i] + (1.0 - wf) * fp[n * stride + i];
-                    if (mine) {
-                        g0[i] += s0 * pf;
-                        g1[i] += s1 * pf;
-                        g2[i] += s2 * pf;
-                    } else {
-                        g0[i] -= s0 * pf;
-                        g1[i] -= s1 * pf;
-                        g2[i] -= s2 * pf;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    const double b = g(bv)[(-1 - o) * stride + i];
-                    g0[i] += s0 * b;
-                    g1[i] += s1 * b;
-                    g2[i] += s2 * b;
-                }
-            }
-        }
+        const auto a = load_grad_slot<Lsq, NC>(m, t, phi, bv, stride);
+        const auto b = load_grad_slot<Lsq, NC>(m, t + 64, phi, bv, stride);
+        add_grad_slot<Lsq, NC>(a, pc, g0, g1, g2);
+        add_grad_slot<Lsq, NC>(b, pc, g0, g1, g2);
+    }
+    if (k < count) {
+        add_grad_slot<Lsq, NC>(load_grad_slot<Lsq, NC>(m, first + (k << 6), phi, bv, stride), pc, g0, g1, g2);
     }
     double* out_c = out + static_cast<long long>(c) * ostride;
     if (Lsq) {
@@ -474,6 +505,56 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_face_value(FvFaces fa, const 
 // Cell pass of the corrected Gauss gradient: sum over the cell's faces in face order of
 // +-S_f phi_f (internal faces, values from k_fv_face_value) and S_f phi_b (boundary faces),
 // divided by the volume.
+// One slot's operands for k_fv_gauss_from_faces: the area vector and the face value (internal
+// faces) or the boundary value; padding loads no value.
+template <int NC>
+struct GaussSlot {
+    int f;
+    bool internal;
+    bool mine;
+    double s0, s1, s2;
+    double u[NC];
+};
+
+template <int NC>
+__device__ __forceinline__ GaussSlot<NC> load_gauss_slot(const FvSlots& m, int t, const double* pf_f, const double* bv,
+                                                         int stride) {
+    GaussSlot<NC> sl;
+    sl.f = g(m.face)[t];
+    const int o = g(m.other)[t];
+    sl.internal = o >= 0;
+    sl.mine = g(m.owns)[t] != 0;
+    sl.s0 = g(m.sf0)[t];
+    sl.s1 = g(m.sf1)[t];
+    sl.s2 = g(m.sf2)[t];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        sl.u[i] = sl.f < 0 ? 0.0
+                  : o >= 0 ? g(pf_f)[NC * static_cast<long long>(sl.f) + i]
+                           : g(bv)[static_cast<long long>(-1 - o) * stride + i];
+    }
+    return sl;
+}
+
+template <int NC>
+__device__ __forceinline__ void add_gauss_slot(const GaussSlot<NC>& sl, double* g0, double* g1, double* g2) {
+    if (sl.f < 0) {
+        return; // padding
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        if (sl.internal && !sl.mine) {
+            g0[i] -= sl.s0 * sl.u[i];
+            g1[i] -= sl.s1 * sl.u[i];
+            g2[i] -= sl.s2 * sl.u[i];
+        } else {
+            g0[i] += sl.s0 * sl.u[i];
+            g1[i] += sl.s1 * sl.u[i];
+            g2[i] += sl.s2 * sl.u[i];
+        }
+    }
+}
+
 template <int NC>
 __global__ void __launch_bounds__(k_fv_block) k_fv_gauss_from_faces(FvSlots m, const double* __restrict__ pf_f,
                                                                     const double* __restrict__ bv, int stride,
@@ -494,40 +575,17 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_gauss_from_faces(FvSlots m, c
     int first = 0;
     int count = 0;
     slot_range(m, c, first, count);
-    for (int k = 0; k < count; ++k) {
+    // Faces in ascending order, two slots loaded ahead of their terms.
+    int k = 0;
+    for (; k + 2 <= count; k += 2) {
         const int t = first + (k << 6);
-        const int f = g(m.face)[t];
-        if (f < 0) {
-            break;
-        }
-        const int o = g(m.other)[t];
-        const double s0 = g(m.sf0)[t];
-        const double s1 = g(m.sf1)[t];
-        const double s2 = g(m.sf2)[t];
-        if (o >= 0) {
-            const bool mine = g(m.owns)[t] != 0;
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                const double pf = g(pf_f)[NC * static_cast<long long>(f) + i];
-                if (mine) {
-                    g0[i] += s0 * pf;
-                    g1[i] += s1 * pf;
-                    g2[i] += s2 * pf;
-                } else {
-                    g0[i] -= s0 * pf;
-                    g1[i] -= s1 * pf;
-                    g2[i] -= s2 * pf;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                const double b = g(bv)[(-1 - o) * stride + i];
-                g0[i] += s0 * b;
-                g1[i] += s1 * b;
-                g2[i] += s2 * b;
-            }
-        }
+        const auto a = load_gauss_slot<NC>(m, t, pf_f, bv, stride);
+        const auto b = load_gauss_slot<NC>(m, t + 64, pf_f, bv, stride);
+        add_gauss_slot<NC>(a, g0, g1, g2);
+        add_gauss_slot<NC>(b, g0, g1, g2);
+    }
+    if (k < count) {
+        add_gauss_slot<NC>(load_gauss_slot<NC>(m, first + (k << 6), pf_f, bv, stride), g0, g1, g2);
     }
     double* out_c = out + static_cast<long long>(c) * ostride;
     const double v = g(m.vol)[c];
