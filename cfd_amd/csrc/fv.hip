@@ -163,14 +163,15 @@ __device__ __forceinline__ void add_grad_slot(const GradSlot<Lsq, NC>& sl, const
     }
 }
 
-// Gradient of NC components of a field with `stride` values per cell and boundary face; component
-// i goes to out[c * ostride + 3 i + j]. Least squares: A (sum_f d_f dphi_f), A the cell's
+// Gradient of NC components of a field with `stride` values per cell and boundary face; derivative
+// j of component i goes to out[c * ostride + (3 i + j) * qstride] (qstride 1: interleaved per cell;
+// ostride 1 and qstride n_cells: component-major). Least squares: A (sum_f d_f dphi_f), A the cell's
 // column-major inverse; Green-Gauss: sum_f S_f phi_f / V with phi_f = w phi_P + (1 - w) phi_N
 // (fv_cell.inc:25-73). Slots are loaded two at a time ahead of their terms, which are added in
 // slot (ascending face) order; padding slots add nothing.
 template <bool Lsq, int NC>
 __device__ __forceinline__ void cell_gradient(const FvSlots& m, int c, const double* phi, const double* bv, int stride,
-                                              double* out, int ostride) {
+                                              double* out, int ostride, int qstride) {
     const auto fp = g(phi);
     double g0[NC];
     double g1[NC];
@@ -204,28 +205,30 @@ __device__ __forceinline__ void cell_gradient(const FvSlots& m, int c, const dou
                      a8 = a[8];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            out_c[3 * i] = a0 * g0[i] + a3 * g1[i] + a6 * g2[i];
-            out_c[3 * i + 1] = a1 * g0[i] + a4 * g1[i] + a7 * g2[i];
-            out_c[3 * i + 2] = a2 * g0[i] + a5 * g1[i] + a8 * g2[i];
+            out_c[static_cast<long long>(3 * i) * qstride] = a0 * g0[i] + a3 * g1[i] + a6 * g2[i];
+            out_c[static_cast<long long>(3 * i + 1) * qstride] = a1 * g0[i] + a4 * g1[i] + a7 * g2[i];
+            out_c[static_cast<long long>(3 * i + 2) * qstride] = a2 * g0[i] + a5 * g1[i] + a8 * g2[i];
         }
     } else {
         const double v = g(m.vol)[c];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            out_c[3 * i] = g0[i] / v;
-            out_c[3 * i + 1] = g1[i] / v;
-            out_c[3 * i + 2] = g2[i] / v;
+            out_c[static_cast<long long>(3 * i) * qstride] = g0[i] / v;
+            out_c[static_cast<long long>(3 * i + 1) * qstride] = g1[i] / v;
+            out_c[static_cast<long long>(3 * i + 2) * qstride] = g2[i] / v;
         }
     }
 }
 
 
-template <bool Lsq, int NC>
+// CompMajor: the output is component-major with qstride between components; otherwise each cell's
+// 3 NC derivatives are contiguous (the component stride is the constant 1).
+template <bool Lsq, int NC, bool CompMajor = false>
 __global__ void __launch_bounds__(k_fv_block) k_fv_gradient(FvSlots m, const double* phi, const double* bv, int stride,
-                                                            double* out, int ostride) {
+                                                            double* out, int ostride, int qstride) {
     const int c = fv_cell();
     if (c < m.nc) {
-        cell_gradient<Lsq, NC>(m, c, phi, bv, stride, out, ostride);
+        cell_gradient<Lsq, NC>(m, c, phi, bv, stride, out, ostride, CompMajor ? qstride : 1);
     }
 }
 
@@ -471,8 +474,8 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_face_sum(FvSlots m, const dou
 template <int NC>
 __global__ void __launch_bounds__(k_fv_block) k_fv_face_value(FvFaces fa, const double* __restrict__ cc,
                                                               const double* __restrict__ phi, int stride,
-                                                              const double* __restrict__ lg,
-                                                              double* __restrict__ pf_out) {
+                                                              const double* __restrict__ lg, int gcs,
+                                                              int gqs, double* __restrict__ pf_out) {
     const int f = fv_cell();
     if (f >= fa.nif) {
         return;
@@ -495,7 +498,8 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_face_value(FvFaces fa, const 
         double dot = 0.0;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const double gf = wf * gl[3 * NC * p + 3 * i + j] + (1.0 - wf) * gl[3 * NC * n + 3 * i + j];
+            const long long q = static_cast<long long>(3 * i + j) * gqs;
+            const double gf = wf * gl[static_cast<long long>(p) * gcs + q] + (1.0 - wf) * gl[static_cast<long long>(n) * gcs + q];
             dot = j == 0 ? gf * r[0] : dot + gf * r[j];
         }
         pf_out[NC * static_cast<long long>(f) + i] = pf + dot;
@@ -1283,14 +1287,15 @@ private:
     }
 
     // Gradient of NC interleaved components (fv_openmp.cpp:257-311): least squares, Green-Gauss,
-    // or least squares into tmp followed by the corrected Gauss gradient.
+    // or least squares into tmp (component-major, so that the face pass gathers each derivative
+    // of the two cells coalesced) followed by the corrected Gauss gradient.
     template <int NC>
     void gradient_loop(const double* phi, const double* bv, int scheme, double* out, int ostride) {
         const FvSlots m = mesh();
         if (scheme == 2) {
             double* tmp = tmp_.get();
             launch([&](hipStream_t s) {
-                k_fv_gradient<true, NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, tmp, 3 * NC);
+                k_fv_gradient<true, NC, true><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, tmp, 1, nc_);
             });
             ensure_faces();
             if (eface_.size() < static_cast<std::size_t>(at_least_one(3LL * nif_))) {
@@ -1298,8 +1303,8 @@ private:
             }
             if (nif_ > 0) {
                 launch([&](hipStream_t s) {
-                    k_fv_face_value<NC><<<face_blocks(nif_), k_fv_block, 0, s>>>(faces(), cc_.get(), phi, NC, tmp,
-                                                                                 eface_.get());
+                    k_fv_face_value<NC><<<face_blocks(nif_), k_fv_block, 0, s>>>(faces(), cc_.get(), phi, NC, tmp, 1,
+                                                                                 nc_, eface_.get());
                 });
             }
             launch([&](hipStream_t s) {
@@ -1309,11 +1314,11 @@ private:
         }
         if (scheme == 1) {
             launch([&](hipStream_t s) {
-                k_fv_gradient<true, NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, out, ostride);
+                k_fv_gradient<true, NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, out, ostride, 1);
             });
         } else {
             launch([&](hipStream_t s) {
-                k_fv_gradient<false, NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, out, ostride);
+                k_fv_gradient<false, NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, out, ostride, 1);
             });
         }
     }
