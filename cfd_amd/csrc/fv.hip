@@ -361,7 +361,8 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_scalar_face(FvFaces fa, const
 // Explicit flux of an internal face for the momentum equation (fv_cell.inc:199-260), one thread
 // per face, per component: -F_f lim_up (g_up^T (x_f - C_up)) [linear upwind, blended with the
 // linear face value when central > 0, limited when lim is given] + nu_f (g_f^T k_f)
-// [+ nu_f g_f S_f], g_f the interpolated 3x3 gradient; the cells add it with k_fv_face_sum.
+// [+ nu_f g_f S_f], g_f the interpolated 3x3 gradient; component i goes to e_out[i * nif + f] and
+// the cells add it with k_fv_face_sum.
 __global__ void __launch_bounds__(k_fv_block) k_fv_momentum_face(FvFaces fa, const double* __restrict__ cc,
                                                                  const double* __restrict__ grad,
                                                                  const double* __restrict__ lim, int limited,
@@ -425,13 +426,13 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_momentum_face(FvFaces fa, con
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        e_out[3LL * f + i] = e[i];
+        e_out[f + static_cast<long long>(i) * fa.nif] = e[i];
     }
 }
 
 // source_c += sum over the cell's internal faces in face order of +e_f (owner) or -e_f
 // (neighbour), NC components, in the order of the reference's face loops; each face's flux comes
-// from its face pass.
+// from its face pass, component i of face f at e_f[i * nif + f].
 template <int NC>
 __global__ void __launch_bounds__(k_fv_block) k_fv_face_sum(FvSlots m, const double* __restrict__ e_f,
                                                             double* __restrict__ source) {
@@ -457,7 +458,7 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_face_sum(FvSlots m, const dou
         const bool mine = g(m.owns)[t] != 0;
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            const double e = g(e_f)[NC * static_cast<long long>(f) + i];
+            const double e = g(e_f)[f + static_cast<long long>(i) * m.nif];
             s[i] = mine ? s[i] + e : s[i] - e;
         }
     }
@@ -470,7 +471,8 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_face_sum(FvSlots m, const dou
 // Face pass of the skewness-corrected Gauss gradient (fv_cell.inc:77-118): the corrected face
 // value of each internal face, NC components, phi_f = w phi_P + (1 - w) phi_N + g_f . (x_f - x_f'),
 // g_f the interpolated least-squares gradient lg and x_f' = w C_P + (1 - w) C_N, as each cell
-// of the face would form it.
+// of the face would form it. Component i of face f goes to pf_out[i * nif + f]; derivative q of
+// cell c's gradient is lg[c * gcs + q * gqs].
 template <int NC>
 __global__ void __launch_bounds__(k_fv_block) k_fv_face_value(FvFaces fa, const double* __restrict__ cc,
                                                               const double* __restrict__ phi, int stride,
@@ -502,7 +504,7 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_face_value(FvFaces fa, const 
             const double gf = wf * gl[static_cast<long long>(p) * gcs + q] + (1.0 - wf) * gl[static_cast<long long>(n) * gcs + q];
             dot = j == 0 ? gf * r[0] : dot + gf * r[j];
         }
-        pf_out[NC * static_cast<long long>(f) + i] = pf + dot;
+        pf_out[f + static_cast<long long>(i) * fa.nif] = pf + dot;
     }
 }
 
@@ -510,7 +512,8 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_face_value(FvFaces fa, const 
 // +-S_f phi_f (internal faces, values from k_fv_face_value) and S_f phi_b (boundary faces),
 // divided by the volume.
 // One slot's operands for k_fv_gauss_from_faces: the area vector and the face value (internal
-// faces) or the boundary value; padding loads no value.
+// faces; component i of face f at pf_f[i * nif + f]) or the boundary value; padding loads no
+// value.
 template <int NC>
 struct GaussSlot {
     int f;
@@ -534,7 +537,7 @@ __device__ __forceinline__ GaussSlot<NC> load_gauss_slot(const FvSlots& m, int t
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         sl.u[i] = sl.f < 0 ? 0.0
-                  : o >= 0 ? g(pf_f)[NC * static_cast<long long>(sl.f) + i]
+                  : o >= 0 ? g(pf_f)[sl.f + static_cast<long long>(i) * m.nif]
                            : g(bv)[static_cast<long long>(-1 - o) * stride + i];
     }
     return sl;
