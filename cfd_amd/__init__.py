@@ -2,7 +2,7 @@
 
 The compute path is libcfdgpu.so (C++ host code + HIP kernels, C-ABI in include/cfdgpu.h);
 this package is its Python mirror of the reference's cfd::gpu interface. Importing it loads
-the library and fails loudly if it has not been built (python -m cfd_amd.build).
+the library and fails loudly if it has not been built (python cfd_amd/build.py).
 """
 from . import _lib
 

@@ -310,7 +310,7 @@ def lib() -> C.CDLL:
         path = Path(os.environ.get("CFDG_LIBRARY", LIB_PATH))
         if not path.exists():
             raise ImportError(
-                f"libcfdgpu.so not found at {path}; build it with `python -m cfd_amd.build` "
+                f"libcfdgpu.so not found at {path}; build it with `python cfd_amd/build.py` "
                 "(there is no fallback implementation)")
         h = C.CDLL(str(path))
         for name, (res, args) in _SIGNATURES.items():

@@ -1,9 +1,9 @@
 """Builds libcfdgpu.so for gfx950 in-tree (cfd_amd/lib/) and, for tests and the CPU baseline,
 the oracle under oracle/ (plus oracle/_ref from the reference checkout when it is present).
 
-    python -m cfd_amd.build            # library + oracle, every source compiled again
-    python -m cfd_amd.build --lib-only
-    python -m cfd_amd.build --incremental  # only what changed
+    python cfd_amd/build.py              # library + oracle, every source compiled again
+    python cfd_amd/build.py --lib-only
+    python cfd_amd/build.py --incremental   # only what changed
 """
 from __future__ import annotations
 
