@@ -223,28 +223,16 @@ def served(count: int = 24, rate: float = 200.0, seed: int = 20240607) -> dict:
     import cfd_amd
     from cfd_amd import AmgHierarchy, AmgSettings, Precision, cases
 
-    rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
+    gen, arrivals = cases.serving_requests(count, rate=rate, seed=seed)
     reqs, cells, solves = [], 0, 0
-    for i in range(count):
-        n = int(round(np.exp(rng.uniform(np.log(1e4), np.log(4e6)))))
-        m = max(4, int(round(n ** (1.0 / 3.0))))
-        nz = max(4, int(round(n / (m * m))))
-        mesh = cases.box_mesh(m, m, nz, hi=(1.0, 1.0, nz / m), jitter=0.2, seed=seed + 31 * i)
-        s = cases.pressure_system(mesh, rhs_seed=seed + 31 * i + 1)
-        pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
-        a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
-        view = AmgHierarchy(a, AmgSettings(strength_decay=0.5)).view()
-        steps = int(rng.integers(1, 21))
-        rhs = [s.source]
-        for _ in range(steps - 1):
-            noise = rng.uniform(-1.0, 1.0, s.n)
-            noise -= noise.mean()
-            rhs.append(rhs[-1] + 0.05 * np.linalg.norm(rhs[-1]) / np.linalg.norm(noise) * noise)
-        reqs.append(dict(pattern=pat, diag=s.diag, upper=s.upper, lower=s.lower, amg=view, rhs=rhs))
+    for g in gen:
+        s = g["system"]
+        view = AmgHierarchy(g["csr"], AmgSettings(strength_decay=0.5)).view()
+        reqs.append(dict(pattern=g["pattern"], diag=s.diag, upper=s.upper, lower=s.lower, amg=view,
+                         rhs=g["rhs"]))
         cells += s.n
-        solves += steps
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+        solves += len(g["rhs"])
     log(f"serve: {count} requests, {cells} cells, {solves} solves, set up in "
         f"{time.perf_counter() - t0:.1f} s")
 

@@ -227,3 +227,34 @@ def batch_shapes(count: int = 256, n_min: int = 4096, n_max: int = 500_000,
         # cell size h x h x r h: domain [0,1] x [0,1] x [0, nz r / m]
         shapes.append(dict(nx=m, ny=m, nz=nz, lz=nz * r / m, jitter=jitter, seed=seed + 17 * i + 1))
     return shapes
+
+
+def serving_requests(count: int, n_min: float = 1e4, n_max: float = 4e6, max_steps: int = 20,
+                     rate: float = 200.0, seed: int = 20240607):
+    """Serving workload: `count` requests, each a jittered box of log-uniform n_min..n_max cells
+    (m x m x nz cells on [0,1] x [0,1] x [0, nz / m]) with its pressure system and a run of
+    1..max_steps time-step right-hand sides, each the previous one plus 5% (in norm) of zero-mean
+    uniform noise; arrival times (seconds) of a Poisson process of the given rate. numpy's
+    default generator with the given seed draws sizes, step counts, noise and arrivals.
+
+    Returns (requests, arrivals); a request is a dict with the system's LduSystem `system`, its
+    LduPattern `pattern`, the assembled CsrView `csr` and the list `rhs`."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        m = max(4, int(round(n ** (1.0 / 3.0))))
+        nz = max(4, int(round(n / (m * m))))
+        mesh = box_mesh(m, m, nz, hi=(1.0, 1.0, nz / m), jitter=0.2, seed=seed + 31 * i)
+        s = pressure_system(mesh, rhs_seed=seed + 31 * i + 1)
+        pat = csr_pattern(s.n, s.owner, s.neighbour)
+        steps = int(rng.integers(1, max_steps + 1))
+        rhs = [s.source]
+        for _ in range(steps - 1):
+            noise = rng.uniform(-1.0, 1.0, s.n)
+            noise -= noise.mean()
+            rhs.append(rhs[-1] + 0.05 * np.linalg.norm(rhs[-1]) / np.linalg.norm(noise) * noise)
+        reqs.append(dict(system=s, pattern=pat, csr=ldu_to_csr(pat, s.diag, s.upper, s.lower),
+                         rhs=rhs))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    return reqs, arrivals

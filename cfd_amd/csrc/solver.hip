@@ -2180,9 +2180,11 @@ extern "C" cfdg_status cfdg_system_solve(cfdg_system* sys, const double* diag, c
         s.assemble(diag, upper, lower);
         s.set_rhs(b);
         s.set_x(x);
-        // The reference runs AMG only for the CG method (device_solver.cpp:138-149).
-        const bool amg = use_amg != 0 && method == CFDG_METHOD_CG && s.has_amg();
-        *stats = solve_one(s, method, amg, CFDG_PREC_FP64, *control);
+        // With use_amg and a hierarchy the reference runs AMG-preconditioned CG whatever the
+        // method (device_solver.cpp:138-147, krylov_cuda.cu:659-667); otherwise the method's
+        // Jacobi-preconditioned Krylov solver.
+        const bool amg = use_amg != 0 && s.has_amg();
+        *stats = solve_one(s, amg ? CFDG_METHOD_CG : method, amg, CFDG_PREC_FP64, *control);
         s.get_x(x);
     });
 }

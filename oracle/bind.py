@@ -341,6 +341,17 @@ class _LevelArrays:
                 ip(self.nnz_r), *self.r)
 
 
+def amg_cg_lists(a_list, p_list, r_list, coarse_inverse, b, x, tol, max_it, sweeps=2,
+                 weight=0.7) -> Stats:
+    """The oracle's Jacobi-V-cycle AMG-PCG on a hierarchy given as per-level CSR lists and a
+    dense row-major coarse inverse (krylov_kernels.inc:182-301)."""
+    arrs = _LevelArrays(a_list, p_list, r_list)
+    ci = _d(coarse_inverse).reshape(-1)
+    b = _d(b)
+    return _stats(orc().orc_amg_cg(len(a_list), ip(arrs.n), *arrs.a, *arrs.p, *arrs.r, dp(ci),
+                                   int(sweeps), float(weight), dp(b), dp(x), float(tol), int(max_it)))
+
+
 def ref_solve_amg_cg_host(a_list, p_list, r_list, coarse_inverse, b, x, tol, max_it,
                           sweeps=2, weight=0.7) -> Stats:
     """The reference's cfd::gpu::solve_amg_cg_host on the given hierarchy."""

@@ -226,3 +226,27 @@ def test_device_built_layouts_match_host_built(monkeypatch):
     for prec in (Precision.FP64, Precision.MIXED_FP32):
         assert out[(False, prec)][0] == out[(True, prec)][0]
         assert np.array_equal(out[(False, prec)][1], out[(True, prec)][1])
+
+
+def test_use_amg_runs_amg_cg_whatever_the_method(oracle):
+    """DeviceSolver::solve with use_amg and a hierarchy runs AMG-preconditioned CG whatever the
+    method (device_solver.cpp:138-147, krylov_cuda.cu:659-667): BiCGStab with use_amg gives the
+    oracle's AMG-PCG result; without a hierarchy the method's own Krylov solver runs."""
+    s, pat, a = _pressure_csr(12, 0.2, seed=21)
+    h = AmgHierarchy(a, AmgSettings(strength_decay=0.5))
+    b = s.source
+    tol = 1e-8 * np.linalg.norm(b)
+    oh = oracle.Hierarchy(oracle.Csr(a.n, a.n, a.row_ptr, a.col, a.val), decay=0.5)
+    xo = np.zeros(s.n)
+    so = oh.amg_cg(b, xo, tol, 300)
+    with cfd_amd.DeviceSolver(pat) as dev:
+        xb = np.zeros(s.n)
+        stb = dev.solve(s.diag, s.upper, s.lower, b, xb, Method.BiCgStab, True, SolveControl(tol, 300))
+        assert not np.array_equal(xb, np.zeros(s.n))
+    with cfd_amd.DeviceSolver(pat) as dev:
+        dev.set_amg(h.view())
+        x = np.zeros(s.n)
+        st = dev.solve(s.diag, s.upper, s.lower, b, x, Method.BiCgStab, True, SolveControl(tol, 300))
+    print(f"BiCGStab + use_amg: gpu it {st.iterations} oracle AMG-PCG it {so.iterations}; "
+          f"without hierarchy (Jacobi-BiCGStab) it {stb.iterations}")
+    assert st.converged and abs(st.iterations - so.iterations) <= 1 and _rel(x, xo) <= 1e-10

@@ -164,6 +164,19 @@ def test_setup_bit_exact_against_oracle(oracle, shape, jitter, decay, dirichlet)
     _same_hierarchy(h, oh)
 
 
+def test_setup_bit_exact_on_the_128_cube_benchmark_system(oracle):
+    """The hierarchy the single-system benchmark solves on: the 128^3 jittered pressure system
+    (2,097,152 cells, threshold halved per level). Every level's P, R and A_c, patterns and
+    values, and every aggregate map equal the oracle's bit for bit."""
+    _, p, _ = cases.config2(128)
+    pat = cases.csr_pattern(p.n, p.owner, p.neighbour)
+    a = cases.ldu_to_csr(pat, p.diag, p.upper, p.lower)
+    h = AmgHierarchy(a, AmgSettings(strength_decay=0.5))
+    oh = oracle.Hierarchy(oracle.Csr(a.n, a.n, a.row_ptr, a.col, a.val), decay=0.5)
+    assert h.level_sizes() == [2097152, 263534, 14687, 488, 22]
+    _same_hierarchy(h, oh)
+
+
 def test_setup_bit_exact_on_variable_coefficients(oracle):
     """Config-3-like contrast (rho 1000 sphere, harmonic faces) on a small jittered box."""
     mesh = cases.box_mesh(30, 30, 30, jitter=0.2, seed=5)
@@ -226,3 +239,16 @@ def test_tuning_table_is_well_formed():
         assert r["min_rows"] <= r["max_rows"] and r["min_mean"] <= r["max_mean"]
         assert r["lanes"] in (1, 2, 4, 8, 16, 32, 64) and r["csr"] in (0, 1)
     assert t["mixed_smoother"]["sweeps"] in (1, 2)
+
+
+def test_singular_coarsest_operator_falls_back_to_the_pseudo_inverse():
+    """A cell with no faces and a zero diagonal leaves an empty row in every coarse operator, so
+    the coarsest LDL^T meets an exactly zero pivot; the setup then returns the Moore-Penrose
+    pseudo-inverse (amg.cpp:185-189, 227-240: Eigen's complete orthogonal decomposition)."""
+    from .test_gpu_failure_paths import _singular_coarse_system
+
+    h = AmgHierarchy(_singular_coarse_system())
+    c = h.matrix(h.n_levels - 1).to_scipy().toarray()
+    assert h.n_levels == 2 and np.linalg.matrix_rank(c) == c.shape[0] - 1
+    pinv = np.linalg.pinv(c)
+    assert np.abs(h.coarse_inverse() - pinv).max() <= 1e-10 * np.abs(pinv).max()
