@@ -88,6 +88,9 @@ def orc() -> C.CDLL:
             "orc_amg_cg_h": (OrcStats, [C.c_void_p, P_INT, P_INT, P_DBL, C.c_int, C.c_double, P_DBL,
                                         P_DBL, C.c_double, C.c_int]),
             "orc_sgs_pcg": (OrcStats, [C.c_void_p, P_DBL, P_DBL, C.c_double, C.c_int]),
+            "orc_sgs_apply": (None, [C.c_void_p, P_DBL, P_DBL]),
+            "orc_spgemm": (C.c_int, [C.c_int, C.c_int, C.c_int, P_INT, P_INT, P_DBL, P_INT, P_INT, P_DBL,
+                                     C.c_int, P_INT, P_INT, P_DBL]),
             "orc_fv_gradient": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, C.c_int, P_DBL]),
             "orc_fv_vector_gradient": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, C.c_int, P_DBL]),
             "orc_fv_scalar_corrections": (None, [C.POINTER(OrcFvMesh), P_DBL, P_DBL, C.c_int, P_DBL, P_DBL,
@@ -308,6 +311,26 @@ class Hierarchy:
     def sgs_pcg(self, b, x, tol: float, max_it: int) -> Stats:
         b = _d(b)
         return _stats(orc().orc_sgs_pcg(self.h, dp(b), dp(x), tol, max_it))
+
+    def sgs_apply(self, r) -> np.ndarray:
+        """One SGS V-cycle z = M^-1 r from z = 0 (amg.cpp:218-225)."""
+        r = _d(r)
+        z = np.zeros_like(r)
+        orc().orc_sgs_apply(self.h, dp(r), dp(z))
+        return z
+
+
+def spgemm(a: Csr, b: Csr) -> Csr:
+    """The setup's conservative sparse product A B, explicit zeros kept."""
+    cap = max(1, a.n_rows * b.n_cols)
+    rp = np.zeros(a.n_rows + 1, np.int32)
+    ci = np.zeros(cap, np.int32)
+    va = np.zeros(cap)
+    nnz = orc().orc_spgemm(a.n_rows, a.n_cols, b.n_cols, ip(_i(a.rp)), ip(_i(a.ci)), dp(_d(a.va)), ip(_i(b.rp)),
+                           ip(_i(b.ci)), dp(_d(b.va)), cap, ip(rp), ip(ci), dp(va))
+    if nnz < 0:
+        raise RuntimeError("spgemm: output larger than rows x cols")
+    return Csr(a.n_rows, b.n_cols, rp, ci[:nnz], va[:nnz])
 
 
 class _LevelArrays:

@@ -1165,10 +1165,7 @@ static void orc_cycle(const orc_amg* h, const sgs_work* w, int level, const doub
     orc_gauss_seidel(&lv->a, w->inv_diag[level], b, x, 0);
 }
 
-/* pcg (amg.cpp:253-285) with apply (amg.cpp:218-225). */
-orc_stats orc_sgs_pcg(const orc_amg* h, const double* b, double* x, double tol, int max_it) {
-    const ocsr* a = &h->levels[0].a;
-    const int n = a->rows;
+static sgs_work sgs_work_alloc(const orc_amg* h) {
     const int nl = h->n_levels;
     sgs_work w;
     w.inv_diag = (double**)calloc((size_t)nl, sizeof(double*));
@@ -1189,6 +1186,37 @@ orc_stats orc_sgs_pcg(const orc_amg* h, const double* b, double* x, double tol, 
         w.rc[l] = (double*)calloc((size_t)nc + 1, sizeof(double));
         w.ec[l] = (double*)calloc((size_t)nc + 1, sizeof(double));
     }
+    return w;
+}
+
+static void sgs_work_free(const orc_amg* h, sgs_work* w) {
+    for (int l = 0; l < h->n_levels; ++l) {
+        free(w->inv_diag[l]);
+        free(w->res[l]);
+        free(w->rc[l]);
+        free(w->ec[l]);
+    }
+    free(w->inv_diag);
+    free(w->res);
+    free(w->rc);
+    free(w->ec);
+}
+
+/* apply (amg.cpp:218-225): z = M^-1 r, one SGS V-cycle from z = 0. */
+void orc_sgs_apply(const orc_amg* h, const double* r, double* z) {
+    sgs_work w = sgs_work_alloc(h);
+    for (int i = 0; i < h->levels[0].a.rows; ++i) {
+        z[i] = 0.0;
+    }
+    orc_cycle(h, &w, 0, r, z);
+    sgs_work_free(h, &w);
+}
+
+/* pcg (amg.cpp:253-285) with apply (amg.cpp:218-225). */
+orc_stats orc_sgs_pcg(const orc_amg* h, const double* b, double* x, double tol, int max_it) {
+    const ocsr* a = &h->levels[0].a;
+    const int n = a->rows;
+    sgs_work w = sgs_work_alloc(h);
     double* r = (double*)calloc((size_t)n + 1, sizeof(double));
     double* z = (double*)calloc((size_t)n + 1, sizeof(double));
     double* p = (double*)calloc((size_t)n + 1, sizeof(double));
@@ -1239,21 +1267,34 @@ orc_stats orc_sgs_pcg(const orc_amg* h, const double* b, double* x, double tol, 
             rz = rz_new;
         }
     }
-    for (int l = 0; l < nl; ++l) {
-        free(w.inv_diag[l]);
-        free(w.res[l]);
-        free(w.rc[l]);
-        free(w.ec[l]);
-    }
-    free(w.inv_diag);
-    free(w.res);
-    free(w.rc);
-    free(w.ec);
+    sgs_work_free(h, &w);
     free(r);
     free(z);
     free(p);
     free(ap);
     return st;
+}
+
+/* The conservative product C = A B of the setup (ocsr_mul) on caller arrays, explicit zeros
+ * kept; c_* must hold cap entries. Returns nnz(C), or -1 when cap is too small. */
+int orc_spgemm(int rows, int inner, int cols, const int* a_rp, const int* a_ci, const double* a_va,
+               const int* b_rp, const int* b_ci, const double* b_va, int cap, int* c_rp, int* c_ci,
+               double* c_va) {
+    ocsr a = ocsr_copy(rows, inner, a_rp, a_ci, a_va);
+    ocsr b = ocsr_copy(inner, cols, b_rp, b_ci, b_va);
+    ocsr c = ocsr_mul(&a, &b);
+    int nnz = c.nnz;
+    if (nnz <= cap) {
+        memcpy(c_rp, c.rp, sizeof(int) * ((size_t)rows + 1));
+        memcpy(c_ci, c.ci, sizeof(int) * (size_t)nnz);
+        memcpy(c_va, c.va, sizeof(double) * (size_t)nnz);
+    } else {
+        nnz = -1;
+    }
+    ocsr_free(&a);
+    ocsr_free(&b);
+    ocsr_free(&c);
+    return nnz;
 }
 
 

@@ -10,10 +10,14 @@
  * -ffp-contract=off, so a serial run reproduces the reference's host results bit for bit where
  * the reference itself is serial (n <= 20000, src/gpu/src/krylov_openmp.cpp:21-25).
  *
- * Parity unpinned (no reference build or fixture can check it here): orc_sgs_pcg's exact
- * iteration counts (Eigen's vectorised norms), bit-exact coarse patterns against Eigen's own
- * sparse products at full size, and orc_pressure_equation's Vec3 dot / mean order against
- * Eigen's evaluation (Eigen is absent; each is pinned by the mathematical checks of the tests).
+ * How the Eigen-based parts are pinned (Eigen, and so the reference's fv library, is absent):
+ * orc_sgs_pcg's V-cycle equals the dense two-level symmetric Gauss-Seidel operator and its outer
+ * loop equals scipy's preconditioned CG with that operator (tests/test_oracle_pins.py); the
+ * setup's sparse product follows hand-derived known answers in which any other summation order
+ * keeps or drops an exact zero, and the whole setup equals dense brute force. Still parity
+ * unpinned: bit-exact iteration counts against Eigen's vectorised norms in SGS-PCG, and
+ * orc_pressure_equation's Vec3 dot / mean order against Eigen's evaluation (pinned only by the
+ * mathematical checks of tests/test_pressure_oracle.py).
  */
 #ifndef CFD_ORACLE_H
 #define CFD_ORACLE_H
@@ -93,6 +97,13 @@ orc_stats orc_amg_cg_h(const orc_amg* h, const int* rp, const int* ci, const dou
 /* CPU AmgCg: PCG with the symmetric Gauss-Seidel V-cycle and a direct coarsest solve
  * (fv::pcg amg.cpp:253-285, cycle amg.cpp:192-216). Breakdown when p.Ap <= 0. */
 orc_stats orc_sgs_pcg(const orc_amg* h, const double* b, double* x, double tol, int max_it);
+/* apply (amg.cpp:218-225): z = M^-1 r, one SGS V-cycle from z = 0. */
+void orc_sgs_apply(const orc_amg* h, const double* r, double* z);
+/* The setup's conservative sparse product C = A B (amg.cpp:170, 176-177), explicit zeros kept;
+ * returns nnz(C), or -1 when cap is too small. */
+int orc_spgemm(int rows, int inner, int cols, const int* a_rp, const int* a_ci, const double* a_va,
+               const int* b_rp, const int* b_ci, const double* b_va, int cap, int* c_rp, int* c_ci,
+               double* c_va);
 
 /* ---- Finite-volume cell kernels (gpu::DeviceFv, src/gpu/src/fv_cell.inc, fv_openmp.cpp) ----
  * The mesh struct has FvMeshView's fields in its order (src/gpu/include/cfd/gpu/fv_device.hpp:16-32). */

@@ -276,6 +276,100 @@ def test_sgs_pcg_reference_bounds(oracle):
     assert iters[2] <= iters[0] + 10
 
 
+def _sgs_dense_operator(a, p, r, c):
+    """M^-1 of one two-level SGS V-cycle from x = 0 (amg.cpp:192-225) formed from dense matrices:
+    forward Gauss-Seidel x1 = (D + L)^-1 b, coarse correction x2 = x1 + P C^-1 R (b - A x1),
+    backward Gauss-Seidel x3 = x2 + (D + U)^-1 (b - A x2); the coarse solve is exact."""
+    import scipy.linalg as sl
+
+    n = a.shape[0]
+    eye = np.eye(n)
+    f = sl.solve_triangular(np.tril(a), eye, lower=True)
+    g = sl.solve_triangular(np.triu(a), eye, lower=False)
+    k = p @ np.linalg.solve(c, r)
+    t = f + k @ (eye - a @ f)
+    return t + g @ (eye - a @ t), f, g, k
+
+
+def test_sgs_vcycle_is_the_two_level_symmetric_gauss_seidel_operator(oracle):
+    """The oracle's SGS V-cycle (orc_cycle) applied to every unit vector equals the dense
+    operator of a two-level cycle with a forward pre-sweep, exact coarse solve and backward
+    post-sweep, and is symmetric (<M^-1 x, y> = <x, M^-1 y>). Swapping the sweep directions or
+    making both sweeps forward changes the operator by far more than the tolerance (checked)."""
+    _, a = _box_csr(oracle, 24, 20, 1, dirichlet=True)
+    h = oracle.Hierarchy(a)
+    assert h.n_levels == 2
+    ad = a.to_scipy().toarray()
+    p = h.matrix(0, 1).to_scipy().toarray()
+    r = h.matrix(0, 2).to_scipy().toarray()
+    c = h.matrix(1).to_scipy().toarray()
+    m_dense, f, g, k = _sgs_dense_operator(ad, p, r, c)
+    n = a.n_rows
+    m_orc = np.column_stack([h.sgs_apply(np.eye(n)[:, j]) for j in range(n)])
+    scale = np.abs(m_dense).max()
+    assert np.abs(m_orc - m_dense).max() <= 1e-12 * scale
+    rng = np.random.default_rng(3)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    assert abs(h.sgs_apply(x) @ y - x @ h.sgs_apply(y)) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(y) * scale
+    eye = np.eye(n)
+    t_sw = g + k @ (eye - ad @ g)          # backward pre-sweep, forward post-sweep
+    swapped = t_sw + f @ (eye - ad @ t_sw)
+    t_ff = f + k @ (eye - ad @ f)          # forward both times
+    forward = t_ff + f @ (eye - ad @ t_ff)
+    assert np.abs(swapped - m_dense).max() > 1e-6 * scale
+    assert np.abs(forward - m_dense).max() > 1e-6 * scale
+
+
+def test_sgs_pcg_is_preconditioned_cg_with_that_operator(oracle):
+    """orc_sgs_pcg (amg.cpp:253-285) against scipy's preconditioned CG with the dense V-cycle
+    operator as preconditioner, from x = 0 and stopping at |r| <= tol: same iteration count
+    (+-1) and solution to 1e-10."""
+    import scipy.sparse.linalg as sla
+
+    _, a = _box_csr(oracle, 24, 20, 1, dirichlet=True)
+    h = oracle.Hierarchy(a)
+    ad = a.to_scipy().toarray()
+    m_dense, _, _, _ = _sgs_dense_operator(ad, h.matrix(0, 1).to_scipy().toarray(),
+                                           h.matrix(0, 2).to_scipy().toarray(),
+                                           h.matrix(1).to_scipy().toarray())
+    n = a.n_rows
+    b = np.cos(0.37 * np.arange(n))
+    tol = 1e-10 * np.linalg.norm(b)
+    x = np.zeros(n)
+    st = h.sgs_pcg(b, x, tol, 100)
+    count = []
+    xs, info = sla.cg(a.to_scipy(), b, x0=np.zeros(n), rtol=0.0, atol=tol, maxiter=100,
+                      M=sla.aslinearoperator(m_dense), callback=lambda _: count.append(1))
+    assert info == 0 and st.converged
+    assert abs(st.iterations - len(count)) <= 1
+    assert np.linalg.norm(x - xs) <= 1e-10 * np.linalg.norm(xs)
+
+
+def test_conservative_product_known_answers(oracle):
+    """The setup's sparse product (amg.cpp:170, 176-177) adds the terms of an entry in the order
+    k runs through row i of A, starting from the first product, and keeps exact zeros for the
+    later prune(0.0). Hand-derived in binary64, where 1 + 1e16 = 1e16 and 1 - 1e16 = -1e16:
+      C(0, 0): terms (1, 1e16, -1e16) -> (1 + 1e16) - 1e16 = 0, kept (reversed order gives 1);
+      C(0, 1): terms (1e16, -1e16, 1) -> 1                       (reversed order gives 0);
+      C(1, 0): terms (1, -1e16) -> -1e16;  C(1, 1): terms (1e16, 1) -> 1e16;
+      C(2, :) = -1 x row 1 of B = (-1e16, 1e16);
+    and a single product -1 x 0 is stored as -0.0 (an accumulator started at 0.0 would give +0.0)."""
+    a = oracle.Csr(3, 3, np.array([0, 3, 5, 6], np.int32), np.array([0, 1, 2, 0, 2, 1], np.int32),
+                   np.array([1.0, 1.0, 1.0, 1.0, 1.0, -1.0]))
+    b = oracle.Csr(3, 2, np.array([0, 2, 4, 6], np.int32), np.array([0, 1, 0, 1, 0, 1], np.int32),
+                   np.array([1.0, 1e16, 1e16, -1e16, -1e16, 1.0]))
+    c = oracle.spgemm(a, b)
+    assert list(c.rp) == [0, 2, 4, 6]
+    assert list(c.ci) == [0, 1, 0, 1, 0, 1]
+    assert c.va[0] == 0.0 and c.va[1] == 1.0
+    assert c.va[2] == -1e16 and c.va[3] == 1e16
+    assert c.va[4] == -1e16 and c.va[5] == 1e16
+    one = np.array([0, 1], np.int32)
+    zero = np.array([0], np.int32)
+    z = oracle.spgemm(oracle.Csr(1, 1, one, zero, np.array([-1.0])), oracle.Csr(1, 1, one, zero, np.array([0.0])))
+    assert z.nnz == 1 and z.va[0] == 0.0 and np.signbit(z.va[0])
+
+
 def test_singular_neumann_with_reference(oracle):
     """Amg.SingularNeumannWithReference (tests/test_amg.cpp:254-265): x == 3."""
     mesh = cases.box_mesh(40, 40, 1)
