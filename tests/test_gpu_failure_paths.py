@@ -89,7 +89,13 @@ def test_bicgstab_guards(oracle, block, label):
 @pytest.mark.parametrize("method", [Method.Cg, Method.BiCgStab])
 def test_zero_diagonal_jacobi_scaling_csr(oracle, method):
     """Five-point operator of tests/test_gpu.cpp with three diagonal entries set to zero: the
-    iterates after a fixed number of iterations agree with the oracle and the reference."""
+    iterates after a fixed number of iterations agree with the oracle and the reference.
+
+    The nonsymmetric operator with zero diagonals makes BiCGStab's iterates sensitive to
+    rounding: a 1e-15 relative perturbation of b moves the oracle's iterate by 2e-12 after two
+    iterations, 1.3e-9 after three and 1.7e-4 after five, so BiCGStab is compared after three
+    iterations at 1e-6 (any other reduction order, the GPU's included, differs at that level),
+    while the scaling 0.0 instead of 1.0 at the zero diagonals moves the iterate by 110%."""
     rp, ci, va = poisson_2d_csr(20, 0.3 if method == Method.BiCgStab else 0.0)
     va = va.copy()
     for row in (0, 57, 399):
@@ -97,12 +103,13 @@ def test_zero_diagonal_jacobi_scaling_csr(oracle, method):
             if ci[k] == row:
                 va[k] = 0.0
     b = np.cos(0.05 * np.arange(400))
-    out, _ = _run_all(oracle, rp, ci, va, b, method, 1e-30, 5)
+    its, bar = (5, 1e-12) if method == Method.Cg else (3, 1e-6)
+    out, _ = _run_all(oracle, rp, ci, va, b, method, 1e-30, its)
     x_o = out["oracle"][3]
     for who, (it, conv, res, x) in out.items():
         rel = float(np.linalg.norm(x - x_o) / np.linalg.norm(x_o))
         print(f"{method.name} {who}: it {it} res {res:.6e} rel vs oracle {rel:.2e}")
-        assert it == 5 and not conv and rel <= 1e-12, who
+        assert it == its and not conv and rel <= bar, who
 
 
 def test_zero_diagonal_on_the_ldu_seam(oracle):
@@ -125,7 +132,10 @@ def test_zero_diagonal_on_the_ldu_seam(oracle):
             so = oracle.solve(oracle.Csr(n, n, a.row_ptr, a.col, a.val), b, xo, int(method), 1e-30, 4)
             rel = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
             print(f"{method.name}: gpu it {st.iterations} oracle it {so.iterations} rel {rel:.2e}")
-            assert st.iterations == so.iterations == 4 and rel <= 1e-12
+            # a 1e-15 relative perturbation of b moves the oracle's fourth BiCGStab iterate by
+            # 1e-11 here (CG: 4e-14), so reduction orders differ at that level
+            bar = 1e-12 if method == Method.Cg else 1e-9
+            assert st.iterations == so.iterations == 4 and rel <= bar
 
 
 def _singular_coarse_system():

@@ -98,7 +98,9 @@ def test_persistent_system_repeated_solves(oracle):
             # relative below, since near machine precision BiCGStab's count follows rounding.
             for tol, check_iters in ((1e-12, method == Method.Cg), (1e-8 * np.linalg.norm(src), True)):
                 x = np.zeros(n)
-                st = dev.solve(diag, up, lo, src, x, method, True, SolveControl(tol, 1000))
+                # use_amg as the reference's caller sets it: only for GpuCg
+                # (ldu_matrix.cpp:358); with use_amg the device runs AMG-CG whatever the method
+                st = dev.solve(diag, up, lo, src, x, method, method == Method.Cg, SolveControl(tol, 1000))
                 ref = np.zeros(n)
                 if method == Method.Cg:
                     so = oh.amg_cg(src, ref, tol, 1000, level0=oa)

@@ -43,12 +43,15 @@ def _solve_all(systems, prec):
                 st = d.solve_resident(Method.Cg, True, prec, c)
                 assert st.converged
                 out.append((st.iterations, st.residual, d.get_x()))
+        reps = []
         with cfd_amd.Batch(devs, Method.Cg, True, prec) as batch:
             for _ in range(2):
                 for d in devs:
                     d.set_x(None)
                 stats = batch.solve(ctls)
-                out += [(st.iterations, st.residual, d.get_x()) for st, d in zip(stats, devs)]
+                reps.append([(st.iterations, st.residual, d.get_x()) for st, d in zip(stats, devs)])
+        for first, second in zip(*reps):  # per member: first batch solve, then the replayed one
+            out += [first, second]
         return out
     finally:
         for d in devs:
