@@ -234,6 +234,8 @@ _SIGNATURES = {
     "cfdg_last_error": (ST, [C.c_char_p, C.c_size_t]),
     "cfdg_solve_csr": (ST, [C.POINTER(Csr), P_DBL, P_DBL, C.c_int, C.POINTER(SolveControl),
                             C.POINTER(SolveStats)]),
+    "cfdg_solve_csr_noalloc": (ST, [C.POINTER(Csr), VP, VP, C.c_int, C.POINTER(SolveControl),
+                                    C.POINTER(SolveStats)]),
     "cfdg_solve_amg_cg": (ST, [C.POINTER(AmgView), P_DBL, P_DBL, C.POINTER(SolveControl),
                                C.POINTER(SolveStats)]),
     "cfdg_solve_amg_cg_prec": (ST, [C.POINTER(AmgView), P_DBL, P_DBL, C.c_int,

@@ -498,6 +498,35 @@ __device__ __forceinline__ double* outer_vec(const SysDev& S, int id) {
     }
 }
 
+// Level-0 FP64 row product of the calling lane's row. Csr = false: the library's sliced-ELL
+// layout (every lane of a live slice takes part). Csr = true: a caller's CSR arrays read in
+// place (a64.sp = row offsets, a64.col, a64.val; one lane per row, entries in CSR order), which
+// gives the same sums: the sliced-ELL padding and stencil gaps add exact zeros after or between
+// the same products in the same order. Rows past the end give 0.
+template <bool Csr>
+__device__ __forceinline__ bool a0_live(const EllView& m, int slice) {
+    return Csr || slice < m.n_slices;
+}
+
+template <bool Csr, class F>
+__device__ __forceinline__ double a0_row_f(const EllView& m, int slice, int lane, int row, F load) {
+    if constexpr (Csr) {
+        if (row >= m.n_rows) {
+            return 0.0;
+        }
+        const auto sp = gm(m.sp);
+        return csr_lane_sum<double>(gm(m.col), gm(static_cast<const double*>(m.val)), sp[row], sp[row + 1], 1, load);
+    } else {
+        return ell_row_f<double>(m, slice, lane, load);
+    }
+}
+
+template <bool Csr>
+__device__ __forceinline__ double a0_row(const EllView& m, int slice, int lane, int row, const double* __restrict__ x) {
+    const auto gx = gm(x);
+    return a0_row_f<Csr>(m, slice, lane, row, [&](int c) { return gx[c]; });
+}
+
 #define CFDG_BLOCK_PROLOGUE(sysarr, blocks)                                                     \
     const int2 bk_ = (blocks)[xcd_block()];                                                       \
     const SysDev& S = (sysarr)[bk_.x];                                                            \
@@ -545,6 +574,7 @@ __global__ void __launch_bounds__(k_block) k_residual0(const SysDev* __restrict_
 }
 
 // Jacobi CG start: r = b - A x, z = dinv r, p = 0; partials r.r, r.z, b.b.
+template <bool Csr>
 __global__ void __launch_bounds__(k_block) k_cg_init(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
     if (S.st->status != ST_RUNNING) {
@@ -553,8 +583,8 @@ __global__ void __launch_bounds__(k_block) k_cg_init(const SysDev* __restrict__ 
     double rr = 0.0;
     double rz = 0.0;
     double bb = 0.0;
-    if (slice < S.a64.n_slices) {
-        const double ax = ell_row<double>(S.a64, slice, lane, S.x);
+    if (a0_live<Csr>(S.a64, slice)) {
+        const double ax = a0_row<Csr>(S.a64, slice, lane, row, S.x);
         if (row < S.n) {
             const double bi = gm(S.b)[row];
             const double ri = bi - ax;
@@ -579,7 +609,7 @@ __global__ void __launch_bounds__(k_block) k_cg_init(const SysDev* __restrict__ 
 
 // y = A x (level-0 FP64 operator) with the partial sum of w.y into `slot`; with Two, also the
 // partial of y.w2 into slot2. Used for q = A p (p.q), v = A ph (r0.v), t = A sh (t.t, t.s).
-template <bool Two>
+template <bool Two, bool Csr>
 __global__ void __launch_bounds__(k_block) k_spmv_dot(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int which_x,
                            int which_y, int which_w, int slot, int which_w2, int slot2) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
@@ -591,8 +621,8 @@ __global__ void __launch_bounds__(k_block) k_spmv_dot(const SysDev* __restrict__
     const double* w = outer_vec(S, which_w);
     double d1 = 0.0;
     double d2 = 0.0;
-    if (slice < S.a64.n_slices) {
-        const double yi = ell_row<double>(S.a64, slice, lane, x);
+    if (a0_live<Csr>(S.a64, slice)) {
+        const double yi = a0_row<Csr>(S.a64, slice, lane, row, x);
         if (row < S.n) {
             y[row] = yi;
             d1 = w[row] * yi;
@@ -630,7 +660,7 @@ __device__ __forceinline__ double* p_new_of(const SysDev& S, int parity) { retur
 // p update, krylov_kernels.inc:293-296, folded into the next product; identical arithmetic),
 // p_new written for the own rows, and the partial p_new.q. z is the level-0 V-cycle output
 // v[z_buf] (which = 1, FP64 or FP32) or the Jacobi-preconditioned residual S.z (which = 0).
-template <class TZ>
+template <class TZ, bool Csr>
 __global__ void __launch_bounds__(k_block) k_spmv_pz(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int parity,
                           int which, int z_buf) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
@@ -642,8 +672,8 @@ __global__ void __launch_bounds__(k_block) k_spmv_pz(const SysDev* __restrict__ 
     const auto pn = gm(p_new_of(S, parity));
     const double beta = S.st->beta;
     double d = 0.0;
-    if (slice < S.a64.n_slices) {
-        const double qi = ell_row_f<double>(S.a64, slice, lane, [&](int c) {
+    if (a0_live<Csr>(S.a64, slice)) {
+        const double qi = a0_row_f<Csr>(S.a64, slice, lane, row, [&](int c) {
             return static_cast<double>(z[c]) + beta * po[c];
         });
         if (row < S.n) {
@@ -720,11 +750,12 @@ __global__ void __launch_bounds__(k_block) k_p_update(const SysDev* __restrict__
 }
 
 // True residual |b - A x|^2 partials (always runs, after the solve).
+template <bool Csr>
 __global__ void __launch_bounds__(k_block) k_true_residual(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
     double t = 0.0;
-    if (slice < S.a64.n_slices) {
-        const double ax = ell_row<double>(S.a64, slice, lane, S.x);
+    if (a0_live<Csr>(S.a64, slice)) {
+        const double ax = a0_row<Csr>(S.a64, slice, lane, row, S.x);
         if (row < S.n) {
             const double ri = gm(S.b)[row] - ax;
             t = ri * ri;
@@ -743,6 +774,7 @@ __global__ void __launch_bounds__(k_block) k_true_residual(const SysDev* __restr
 // ------------------------------------------------------------------------------------------
 
 // r = b - A x; r0 = r; p = v = 0; partials r.r, b.b and r0.r.
+template <bool Csr>
 __global__ void __launch_bounds__(k_block) k_bicg_init(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
     if (S.st->status != ST_RUNNING) {
@@ -750,8 +782,8 @@ __global__ void __launch_bounds__(k_block) k_bicg_init(const SysDev* __restrict_
     }
     double rr = 0.0;
     double bb = 0.0;
-    if (slice < S.a64.n_slices) {
-        const double ax = ell_row<double>(S.a64, slice, lane, S.x);
+    if (a0_live<Csr>(S.a64, slice)) {
+        const double ax = a0_row<Csr>(S.a64, slice, lane, row, S.x);
         if (row < S.n) {
             const double bi = gm(S.b)[row];
             const double ri = bi - ax;

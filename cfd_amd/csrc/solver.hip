@@ -16,7 +16,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
+#include <string>
 
 namespace cfdg {
 
@@ -1472,14 +1474,14 @@ void enqueue_init(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t
     const int g0 = tb.tn[0];
     const int2* b0 = tb.t[0].get();
     if (pl.method == CFDG_METHOD_BICGSTAB) {
-        CFDG_LAUNCH(k_bicg_init, g0, s, d, b0);
+        CFDG_LAUNCH(k_bicg_init<false>, g0, s, d, b0);
         fin<FIN_BICG_INIT>(d, tb, s);
         return;
     }
     // CG start: r, z and beta = 0; p = z + 0 p_old (p_old = 0) is formed by the first
     // iteration's product (k_spmv_pz).
     if (!pl.amg) {
-        CFDG_LAUNCH(k_cg_init, g0, s, d, b0);
+        CFDG_LAUNCH(k_cg_init<false>, g0, s, d, b0);
         fin<FIN_CG_INIT>(d, tb, s);
         return;
     }
@@ -1506,12 +1508,12 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
     if (pl.method == CFDG_METHOD_BICGSTAB) {
         fin<FIN_BICG_RHO>(d, tb, s);
         CFDG_LAUNCH(k_bicg_p, g0, s, d, b0);
-        CFDG_LAUNCH(k_spmv_dot<false>, g0, s, d, b0, V_PH, V_V, V_R0, SLOT_R0V, 0, 0);
+        CFDG_LAUNCH((k_spmv_dot<false, false>), g0, s, d, b0, V_PH, V_V, V_R0, SLOT_R0V, 0, 0);
         fin<FIN_BICG_ALPHA>(d, tb, s);
         CFDG_LAUNCH(k_bicg_s, g0, s, d, b0);
         fin<FIN_BICG_S>(d, tb, s);
         CFDG_LAUNCH(k_bicg_early_x, g0, s, d, b0);
-        CFDG_LAUNCH(k_spmv_dot<true>, g0, s, d, b0, V_SH, V_T, V_T, SLOT_TT, V_S, SLOT_TS);
+        CFDG_LAUNCH((k_spmv_dot<true, false>), g0, s, d, b0, V_SH, V_T, V_T, SLOT_TT, V_S, SLOT_TS);
         fin<FIN_BICG_OMEGA>(d, tb, s);
         CFDG_LAUNCH(k_bicg_xr, g0, s, d, b0);
         fin<FIN_BICG_RES>(d, tb, s, status_out);
@@ -1520,11 +1522,11 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
     const int fz = final_index(pl.sweeps);
     probe_begin(PROBE_OUTER_SPMV, s);
     if (!pl.amg) {
-        CFDG_LAUNCH(k_spmv_pz<double>, g0, s, d, b0, parity, 0, 0);
+        CFDG_LAUNCH((k_spmv_pz<double, false>), g0, s, d, b0, parity, 0, 0);
     } else if (pl.fp32) {
-        CFDG_LAUNCH(k_spmv_pz<float>, g0, s, d, b0, parity, 1, fz);
+        CFDG_LAUNCH((k_spmv_pz<float, false>), g0, s, d, b0, parity, 1, fz);
     } else {
-        CFDG_LAUNCH(k_spmv_pz<double>, g0, s, d, b0, parity, 1, fz);
+        CFDG_LAUNCH((k_spmv_pz<double, false>), g0, s, d, b0, parity, 1, fz);
     }
     probe_end(PROBE_OUTER_SPMV, s);
     fin<FIN_ALPHA>(d, tb, s);
@@ -1680,7 +1682,7 @@ public:
                 }
             }
         }
-        CFDG_LAUNCH(k_true_residual, tb_.tn[0], s_, d, tb_.t[0].get());
+        CFDG_LAUNCH(k_true_residual<false>, tb_.tn[0], s_, d, tb_.t[0].get());
         fin<FIN_TRUE>(d, tb_, s_);
         k_gather_states<<<ceil_div(n_sys, 64), 64, 0, s_>>>(d, n_sys, d_out_.get());
         hip_check(hipEventRecord(t1_.get(), s_), "event record");
@@ -1966,7 +1968,7 @@ void SlotRunner::step(std::vector<Done>& done) {
                 continue;
             }
             const Tables& one = m.single[ju];
-            CFDG_LAUNCH(k_true_residual, one.tn[0], m.s, d, one.t[0].get());
+            CFDG_LAUNCH(k_true_residual<false>, one.tn[0], m.s, d, one.t[0].get());
             fin<FIN_TRUE>(d, one, m.s);
             hip_check(hipMemcpyAsync(m.out_host.get() + j, m.sys[ju]->state(), sizeof(SolveState),
                                      hipMemcpyDeviceToHost, m.s),
@@ -2403,6 +2405,253 @@ extern "C" cfdg_status cfdg_solve_csr(const cfdg_csr* a, const double* b, double
         s.set_x(x);
         *stats = solve_one(s, method, false, CFDG_PREC_FP64, *control);
         s.get_x(x);
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// Allocation-free CSR solve (no counterpart in the reference). The caller's device arrays are
+// read in place; work vectors, block partials, the solve state, the descriptor and the launch
+// table live in `scratch`, a static device array of fixed size, so the call allocates nothing.
+// ------------------------------------------------------------------------------------------
+namespace cfdg {
+namespace {
+
+constexpr std::size_t round_up(std::size_t a, std::size_t b) { return (a + b - 1) / b * b; }
+
+constexpr int k_na_max_rows = CFDG_NOALLOC_MAX_ROWS;
+constexpr int k_na_vectors = 12; // r p p2 q z dinv r0 v s t ph sh
+constexpr std::size_t k_na_max_blocks = static_cast<std::size_t>(k_na_max_rows) / k_block;
+static_assert(k_na_max_rows % k_block == 0, "whole blocks");
+// Byte offsets of the scratch regions.
+constexpr std::size_t k_na_state = 0;   // SolveState
+constexpr std::size_t k_na_flag = 256;  // int: 1 when the CSR check failed
+constexpr std::size_t k_na_ids = 512;   // int: system id 0 (finalize launches)
+constexpr std::size_t k_na_desc = 1024; // SysDev
+constexpr std::size_t k_na_blocks = round_up(k_na_desc + sizeof(SysDev), 256);
+constexpr std::size_t k_na_partials = round_up(k_na_blocks + k_na_max_blocks * sizeof(int2), 256);
+constexpr std::size_t k_na_vecs =
+    round_up(k_na_partials + static_cast<std::size_t>(k_slots) * k_na_max_blocks * sizeof(double), 256);
+constexpr std::size_t k_na_bytes =
+    k_na_vecs + static_cast<std::size_t>(k_na_vectors) * static_cast<std::size_t>(k_na_max_rows) * sizeof(double);
+static_assert(k_na_bytes <= CFDG_NOALLOC_SCRATCH_BYTES, "scratch size stated in cfdgpu.h");
+static_assert(sizeof(SolveState) <= k_na_flag - k_na_state, "state region");
+
+// Checks a caller's CSR matrix in place: row offsets from 0 to nnz and non-decreasing, columns
+// in [0, n). Any violation stores 1 at *bad (every writer stores the same value). Thread i
+// checks row offset i (i <= n) and column i (i < nnz).
+__global__ void __launch_bounds__(k_block) k_na_check(int n, int nnz, const int* __restrict__ rp,
+                                                      const int* __restrict__ col, int* __restrict__ bad) {
+    const long long i = static_cast<long long>(blockIdx.x) * k_block + threadIdx.x;
+    bool ok = true;
+    if (i <= n) {
+        const int a = rp[i];
+        ok = a >= 0 && a <= nnz && (i != 0 || a == 0) && (i != n || a == nnz) && (i == n || rp[i + 1] >= a);
+    }
+    if (i < nnz) {
+        const int c = col[i];
+        ok = ok && c >= 0 && c < n;
+    }
+    if (!ok) {
+        *bad = 1;
+    }
+}
+
+// Launch table (system 0, block b), system id 0, and the inverse diagonal of the checked matrix
+// (1 / a_ii, 1.0 for a zero or missing diagonal, the row's last nonzero diagonal entry winning,
+// as the host conversion of the one-shot solve; krylov_openmp.cpp:147-157).
+__global__ void __launch_bounds__(k_block) k_na_setup(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                                      const double* __restrict__ val, int nblk, int2* __restrict__ blocks,
+                                                      int* __restrict__ ids, double* __restrict__ dinv) {
+    const int i = blockIdx.x * k_block + threadIdx.x;
+    if (i < nblk) {
+        blocks[i] = make_int2(0, i);
+    }
+    if (i == 0) {
+        ids[0] = 0;
+    }
+    if (i < n) {
+        double d = 1.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            if (col[k] == i && val[k] != 0.0) {
+                d = 1.0 / val[k];
+            }
+        }
+        dinv[i] = d;
+    }
+}
+
+constexpr int k_na_max_devices = 64;
+std::mutex g_na_mutex;                          // one allocation-free solve at a time per process
+void* g_na_scratch[k_na_max_devices] = {};      // device address of `scratch` per device
+
+} // namespace
+} // namespace cfdg
+
+// The scratch area of cfdg_solve_csr_noalloc: one copy per device, held until the process ends.
+// Host code never passes this name to a kernel (on the host it is only a stand-in); it uses the
+// device address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[CFDG_NOALLOC_SCRATCH_BYTES];
+
+namespace cfdg {
+namespace {
+
+// Device address of `scratch` on device `dev` (current), looked up once per device.
+unsigned char* na_scratch(int dev) {
+    require(dev >= 0 && dev < k_na_max_devices, "solve_csr_noalloc: device index out of range");
+    if (g_na_scratch[dev] == nullptr) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        require(p != nullptr, "solve_csr_noalloc: no device address for the scratch area");
+        g_na_scratch[dev] = p;
+    }
+    return static_cast<unsigned char*>(g_na_scratch[dev]);
+}
+
+template <int Op>
+void na_fin(const SysDev* d, const int* ids) {
+    k_finalize<Op><<<1, k_fin_block, 0, nullptr>>>(d, ids, nullptr);
+}
+
+int na_status(const SolveState* st) {
+    int status = 0;
+    hip_check(hipMemcpy(&status, &st->status, sizeof(int), hipMemcpyDeviceToHost), "D2H status");
+    return status;
+}
+
+} // namespace
+} // namespace cfdg
+
+extern "C" cfdg_status cfdg_solve_csr_noalloc(const cfdg_csr* a, const double* b, double* x,
+                                              cfdg_method method, const cfdg_solve_control* control,
+                                              cfdg_solve_stats* stats) {
+    return guarded([&] {
+        require(a != nullptr && b != nullptr && x != nullptr && stats != nullptr, "null argument");
+        check_control(control);
+        require(method == CFDG_METHOD_CG || method == CFDG_METHOD_BICGSTAB, "unknown method");
+        require(a->n_rows >= 1 && (a->n_cols == 0 || a->n_cols == a->n_rows),
+                "CsrView: invalid square CSR matrix");
+        require(a->nnz >= 0 && a->row_ptr != nullptr && (a->nnz == 0 || (a->col != nullptr && a->val != nullptr)),
+                "CsrView: invalid square CSR matrix");
+        if (a->n_rows > k_na_max_rows) {
+            throw CapacityError("solve_csr_noalloc: " + std::to_string(a->n_rows) + " rows exceed the maximum of " +
+                                std::to_string(k_na_max_rows) + " (fixed scratch area)");
+        }
+        require_device();
+        const auto t_start = std::chrono::steady_clock::now();
+        std::lock_guard<std::mutex> lock(g_na_mutex);
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        unsigned char* base = na_scratch(dev);
+        const int n = a->n_rows;
+        const int nnz = a->nnz;
+        const int nblk = ceil_div(n, k_block);
+        const std::size_t stride = round_up(static_cast<std::size_t>(n), k_wave) * sizeof(double);
+        auto vec = [&](int i) { return reinterpret_cast<double*>(base + k_na_vecs + static_cast<std::size_t>(i) * stride); };
+        auto* st = reinterpret_cast<SolveState*>(base + k_na_state);
+        auto* flag = reinterpret_cast<int*>(base + k_na_flag);
+        auto* ids = reinterpret_cast<int*>(base + k_na_ids);
+        auto* desc = reinterpret_cast<SysDev*>(base + k_na_desc);
+        auto* blocks = reinterpret_cast<int2*>(base + k_na_blocks);
+        auto* partials = reinterpret_cast<double*>(base + k_na_partials);
+
+        // The matrix is checked on the device before any kernel gathers through it.
+        const int zero = 0;
+        hip_check(hipMemcpy(flag, &zero, sizeof(int), hipMemcpyHostToDevice), "H2D flag");
+        const long long checks = std::max<long long>(static_cast<long long>(n) + 1, nnz);
+        k_na_check<<<ceil_div(checks, k_block), k_block, 0, nullptr>>>(n, nnz, a->row_ptr, a->col, flag);
+        hip_check(hipGetLastError(), "k_na_check launch");
+        int bad = 0;
+        hip_check(hipMemcpy(&bad, flag, sizeof(int), hipMemcpyDeviceToHost), "D2H flag");
+        require(bad == 0, "CsrView: invalid square CSR matrix (device check)");
+
+        SysDev d{};
+        d.st = st;
+        d.partials = partials;
+        d.nblk0 = nblk;
+        d.n = n;
+        d.n_levels = 0;
+        d.flip = 0;
+        d.a64.n_rows = n;
+        d.a64.tpr = 1;
+        d.a64.csr = 1;
+        d.a64.sp = a->row_ptr;
+        d.a64.col = a->col;
+        d.a64.val = a->val;
+        d.b = b;
+        d.x = x;
+        d.r = vec(0);
+        d.p = vec(1);
+        d.p2 = vec(2);
+        d.q = vec(3);
+        d.z = vec(4);
+        double* dinv = vec(5);
+        d.dinv = dinv;
+        d.r0 = vec(6);
+        d.v = vec(7);
+        d.s = vec(8);
+        d.t = vec(9);
+        d.ph = vec(10);
+        d.sh = vec(11);
+        SolveState s0{};
+        s0.tol = control->abs_tolerance;
+        s0.max_it = control->max_iterations;
+        s0.status = ST_RUNNING;
+        hip_check(hipMemcpy(desc, &d, sizeof(SysDev), hipMemcpyHostToDevice), "H2D descriptor");
+        hip_check(hipMemcpy(st, &s0, sizeof(SolveState), hipMemcpyHostToDevice), "H2D state");
+        k_na_setup<<<nblk, k_block, 0, nullptr>>>(n, a->row_ptr, a->col, a->val, nblk, blocks, ids, dinv);
+        hip_check(hipGetLastError(), "k_na_setup launch");
+
+        // The solver's own kernels over the CSR arrays, one iteration at a time; the host reads
+        // the status after each iteration (surplus iterations would be no-ops on the device).
+        const int max_it = control->max_iterations;
+        if (method == CFDG_METHOD_CG) {
+            k_cg_init<true><<<nblk, k_block, 0, nullptr>>>(desc, blocks);
+            na_fin<FIN_CG_INIT>(desc, ids);
+        } else {
+            k_bicg_init<true><<<nblk, k_block, 0, nullptr>>>(desc, blocks);
+            na_fin<FIN_BICG_INIT>(desc, ids);
+        }
+        hip_check(hipGetLastError(), "init launch");
+        for (int k = 1; k <= max_it + 2; ++k) {
+            const int parity = k & 1;
+            if (method == CFDG_METHOD_CG) {
+                k_spmv_pz<double, true><<<nblk, k_block, 0, nullptr>>>(desc, blocks, parity, 0, 0);
+                na_fin<FIN_ALPHA>(desc, ids);
+                k_update_xr<1><<<nblk, k_block, 0, nullptr>>>(desc, blocks, parity);
+                na_fin<FIN_CG_RES_BETA>(desc, ids);
+            } else {
+                na_fin<FIN_BICG_RHO>(desc, ids);
+                k_bicg_p<<<nblk, k_block, 0, nullptr>>>(desc, blocks);
+                k_spmv_dot<false, true><<<nblk, k_block, 0, nullptr>>>(desc, blocks, V_PH, V_V, V_R0, SLOT_R0V, 0, 0);
+                na_fin<FIN_BICG_ALPHA>(desc, ids);
+                k_bicg_s<<<nblk, k_block, 0, nullptr>>>(desc, blocks);
+                na_fin<FIN_BICG_S>(desc, ids);
+                k_bicg_early_x<<<nblk, k_block, 0, nullptr>>>(desc, blocks);
+                k_spmv_dot<true, true><<<nblk, k_block, 0, nullptr>>>(desc, blocks, V_SH, V_T, V_T, SLOT_TT, V_S, SLOT_TS);
+                na_fin<FIN_BICG_OMEGA>(desc, ids);
+                k_bicg_xr<<<nblk, k_block, 0, nullptr>>>(desc, blocks);
+                na_fin<FIN_BICG_RES>(desc, ids);
+            }
+            hip_check(hipGetLastError(), "iteration launch");
+            const int status = na_status(st);
+            if (status != ST_RUNNING && status != ST_CONVERGED_PENDING) {
+                break;
+            }
+        }
+        k_true_residual<true><<<nblk, k_block, 0, nullptr>>>(desc, blocks);
+        na_fin<FIN_TRUE>(desc, ids);
+        hip_check(hipGetLastError(), "true residual launch");
+        SolveState out{};
+        hip_check(hipMemcpy(&out, st, sizeof(SolveState), hipMemcpyDeviceToHost), "D2H state");
+        *stats = cfdg_solve_stats{};
+        stats->iterations = std::min(out.it, out.max_it);
+        stats->converged = out.status == ST_CONVERGED ? 1 : 0;
+        stats->on_device = 1;
+        stats->exit_reason = out.status;
+        stats->residual = out.res;
+        stats->true_residual = out.true_res;
+        stats->rhs_norm = out.bnorm;
+        stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     });
 }
 

@@ -200,6 +200,29 @@ def solve(a: CsrView, b: np.ndarray, x: np.ndarray, method: Method,
     return SolveStats.from_c(st)
 
 
+# Limits of the allocation-free solve (cfdgpu.h).
+NOALLOC_MAX_ROWS = 524288
+NOALLOC_SCRATCH_BYTES = 51380224
+
+
+def solve_device_noalloc(n: int, nnz: int, row_ptr: int, col: int, val: int, b: int, x: int,
+                         method: Method, control: SolveControl) -> SolveStats:
+    """cfdg_solve_csr_noalloc (no counterpart in the reference): solve() on arrays that already
+    live in device memory, given as integer device addresses (row_ptr int32[n + 1], col
+    int32[nnz], val float64[nnz], b and x float64[n], x in/out), with no device allocation: the
+    work space is the library's fixed scratch area (at most NOALLOC_MAX_ROWS rows)."""
+    ca = L.Csr()
+    ca.n_rows, ca.n_cols, ca.nnz = int(n), 0, int(nnz)
+    ca.row_ptr = C.cast(C.c_void_p(row_ptr), C.POINTER(C.c_int))
+    ca.col = C.cast(C.c_void_p(col), C.POINTER(C.c_int))
+    ca.val = C.cast(C.c_void_p(val), C.POINTER(C.c_double))
+    st = L.SolveStats()
+    cc = control.c()
+    L.call("cfdg_solve_csr_noalloc", C.byref(ca), C.c_void_p(b), C.c_void_p(x), int(method),
+           C.byref(cc), C.byref(st))
+    return SolveStats.from_c(st)
+
+
 def solve_amg_cg(amg: AmgView, b: np.ndarray, x: np.ndarray, control: SolveControl,
                  precision: Precision = Precision.FP64) -> SolveStats:
     """solve_amg_cg() (krylov.hpp:61-62): CG preconditioned by one V-cycle per iteration."""

@@ -146,6 +146,24 @@ cfdg_status cfdg_solve_amg_cg_prec(const cfdg_amg_view* amg, const double* b, do
                                    cfdg_precision precision, const cfdg_solve_control* control,
                                    cfdg_solve_stats* stats);
 
+/* Allocation-free variant of cfdg_solve_csr. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Every array is a DEVICE pointer of the current device: the
+ * matrix (a->row_ptr[n_rows + 1], a->col[nnz], a->val[nnz]), b[n_rows] and x[n_rows] (initial
+ * guess in, solution out); the structs themselves are host memory. The matrix is checked on the
+ * device before use (CFDG_ERR_INVALID_ARGUMENT). Work vectors, partial sums, the solve state and
+ * the launch table live in a static device array of the library, CFDG_NOALLOC_SCRATCH_BYTES on
+ * every device the library runs on, held until the process ends; more than CFDG_NOALLOC_MAX_ROWS
+ * rows are rejected with CFDG_ERR_CAPACITY. Same arithmetic as cfdg_solve_csr (bit-identical
+ * results). Runs on the null stream of the current device and returns when it has finished;
+ * stats->solve_ms is host wall time. Every call on a device shares that device's array, which
+ * keeps the last call's contents: calls that overlapped on one device would race on it, so the
+ * library serialises these calls within a process (one lock); each process has its own array. */
+#define CFDG_NOALLOC_MAX_ROWS 524288
+#define CFDG_NOALLOC_SCRATCH_BYTES 51380224
+cfdg_status cfdg_solve_csr_noalloc(const cfdg_csr* a, const double* b, double* x, cfdg_method method,
+                                   const cfdg_solve_control* control, cfdg_solve_stats* stats);
+
 /* Replaces assemble_convection_diffusion() (krylov.hpp:105-116; host loop
  * krylov_openmp.cpp:289-300). upper/lower are written, diag/divergence are accumulated.
  * The device kernel gathers per cell in ascending face order, so every cell sees its faces in
