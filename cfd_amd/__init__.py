@@ -14,6 +14,7 @@ from .server import ServedRequest, Server, ServerCounters, estimate_footprint  #
 from .krylov import (AmgLevelView, AmgView, Batch, CsrView, DeviceSolver, LduPattern, Method,  # noqa: E402
                      Precision, SolveControl, SolveStats, assemble_convection_diffusion,
                      backend_name, device_available, device_description, solve, solve_amg_cg)
+from .ldu import AmgCache, LinearSolveResult, LinearSolverSettings  # noqa: E402
 
 
 def _apply_tuning() -> None:
@@ -40,4 +41,5 @@ __all__ = [
     "assemble_convection_diffusion", "backend_name", "device_available", "device_description",
     "solve", "solve_amg_cg", "Server", "ServedRequest", "ServerCounters", "estimate_footprint",
     "DeviceFv", "FvMesh", "fv_box", "PressureInputs", "PressureEquation", "synthetic_pressure_inputs",
+    "AmgCache", "LinearSolverSettings", "LinearSolveResult",
 ]

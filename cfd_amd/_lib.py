@@ -207,6 +207,33 @@ class AmgSettings(C.Structure):
     ]
 
 
+class AmgCacheOptions(C.Structure):
+    _fields_ = [("max_age", C.c_int), ("stale_rebuild", C.c_int), ("amg", AmgSettings)]
+
+
+class LinearSettings(C.Structure):
+    _fields_ = [
+        ("method", C.c_int),
+        ("tolerance", C.c_double),
+        ("rel_tolerance", C.c_double),
+        ("max_iterations", C.c_int),
+        ("precision", C.c_int),
+    ]
+
+
+class LinearResult(C.Structure):
+    _fields_ = [
+        ("initial_residual", C.c_double),
+        ("final_residual", C.c_double),
+        ("iterations", C.c_int),
+        ("solved", C.c_int),
+        ("used_amg", C.c_int),
+        ("setups", C.c_int),
+        ("target", C.c_double),
+        ("stats", SolveStats),
+    ]
+
+
 class BoxSpec(C.Structure):
     _fields_ = [
         ("nx", C.c_int),
@@ -284,6 +311,13 @@ _SIGNATURES = {
     "cfdg_amg_coarse_inverse": (ST, [VP, P_DBL]),
     "cfdg_amg_view_of": (ST, [VP, C.POINTER(AmgLevelView), C.POINTER(AmgView)]),
     "cfdg_amg_setup_ms": (ST, [VP, P_DBL]),
+    "cfdg_amg_cache_default_options": (ST, [C.POINTER(AmgCacheOptions)]),
+    "cfdg_amg_cache_create": (ST, [C.POINTER(LduPattern), C.POINTER(AmgCacheOptions), C.POINTER(VP)]),
+    "cfdg_amg_cache_destroy": (ST, [VP]),
+    "cfdg_amg_cache_state": (ST, [VP, P_INT, P_INT, P_INT]),
+    "cfdg_ldu_solve": (ST, [VP, P_DBL, P_DBL, P_DBL, P_DBL, P_DBL, C.POINTER(LinearSettings),
+                            C.POINTER(LinearResult)]),
+    "cfdg_system_normalised_residual": (ST, [VP, P_DBL, P_DBL, P_DBL]),
     "cfdg_fv_create": (ST, [C.POINTER(FvMesh), C.POINTER(VP)]),
     "cfdg_fv_destroy": (ST, [VP]),
     "cfdg_fv_gradient": (ST, [VP, P_DBL, P_DBL, C.c_int, P_DBL]),

@@ -1516,6 +1516,79 @@ __global__ void __launch_bounds__(k_fin_block) k_finalize(const SysDev* __restri
 }
 
 // ------------------------------------------------------------------------------------------
+// Normalised residual (ldu_matrix.cpp:65-103): res = sum_c |b_c - (Ax)_c| and
+// norm = sum_c |(Ax)_c - (A xref)_c| + |b_c - (A xref)_c|, with A xref = rowsum(A) * mean(x).
+// Three passes with fixed-order sums: block sums of x, their total, then per row the two terms
+// and (b - Ax)^2 (the level-0 FP64 operator, one lane per row, rows of 256-row blocks).
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(k_block) k_nr_xsum(int n, const double* __restrict__ x, double* __restrict__ part) {
+    const int i = blockIdx.x * k_block + threadIdx.x;
+    double v[1] = {i < n ? x[i] : 0.0};
+    double o[1];
+    block_sums<1>(v, o);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = o[0];
+    }
+}
+
+// sums[0] = sum of x (k_nr_reduce); part receives three arrays of nblk block sums.
+__global__ void __launch_bounds__(k_block) k_nr_rows(EllView a, int n, const double* __restrict__ x,
+                                                     const double* __restrict__ b, const double* __restrict__ sums,
+                                                     double* __restrict__ part, int nblk) {
+    const int wave = threadIdx.x / k_wave;
+    const int lane = threadIdx.x % k_wave;
+    const int slice = blockIdx.x * k_slices_per_block + wave;
+    const int row = slice * k_wave + lane;
+    const double mean = sums[0] / static_cast<double>(n);
+    double t[3] = {0.0, 0.0, 0.0};
+    if (slice < a.n_slices) {
+        const double ax = ell_row<double>(a, slice, lane, x);
+        const double rs = ell_row_f<double>(a, slice, lane, [](int) { return 1.0; });
+        if (row < n) {
+            const double bi = b[row];
+            const double ax_ref = rs * mean;
+            const double r = bi - ax;
+            t[0] = fabs(r);
+            t[1] = fabs(ax - ax_ref) + fabs(bi - ax_ref);
+            t[2] = r * r;
+        }
+    }
+    double o[3];
+    block_sums<3>(t, o);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = o[0];
+        part[nblk + blockIdx.x] = o[1];
+        part[2 * nblk + blockIdx.x] = o[2];
+    }
+}
+
+// out[q] = sum over i < nblk of part[q nblk + i], q < count, in a fixed order (one workgroup).
+__global__ void __launch_bounds__(k_fin_block) k_nr_reduce(const double* __restrict__ part, int nblk, int count,
+                                                           double* __restrict__ out) {
+    __shared__ double red[k_fin_block / k_wave];
+    for (int q = 0; q < count; ++q) {
+        const double* p = part + static_cast<long long>(q) * nblk;
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nblk; i += k_fin_block) {
+            s += p[i];
+        }
+        s = wave_sum(s);
+        if (threadIdx.x % k_wave == 0) {
+            red[threadIdx.x / k_wave] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = red[0];
+            for (int w = 1; w < k_fin_block / k_wave; ++w) {
+                t += red[w];
+            }
+            out[q] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Assembly kernels
 // ------------------------------------------------------------------------------------------
 

@@ -617,6 +617,29 @@ void System::init_vectors() {
     hip_check(hipStreamSynchronize(stream_.get()), "init sync");
 }
 
+void System::residual_norms(double out[3]) {
+    require(assembled_, "normalised residual: the matrix has not been assembled");
+    const EllView a = a0_.view64();
+    require(a.tpr == 1 && a.csr == 0, "normalised residual: unexpected level-0 layout");
+    hipStream_t s = stream_.get();
+    const int nblk = ceil_div(n_, k_block);
+    if (norms_.size() < 4) {
+        norms_.resize(4);
+    }
+    double* part = partials_.get(); // k_slots * nblk entries; no solve is queued meanwhile
+    k_nr_xsum<<<nblk, k_block, 0, s>>>(n_, x_.get(), part);
+    k_nr_reduce<<<1, k_fin_block, 0, s>>>(part, nblk, 1, norms_.get());
+    k_nr_rows<<<nblk, k_block, 0, s>>>(a, n_, x_.get(), b_.get(), norms_.get(), part, nblk);
+    k_nr_reduce<<<1, k_fin_block, 0, s>>>(part, nblk, 3, norms_.get() + 1);
+    hip_check(hipGetLastError(), "normalised residual launch");
+    double h[4];
+    hip_check(hipMemcpyAsync(h, norms_.get(), sizeof h, hipMemcpyDeviceToHost, s), "D2H norms");
+    hip_check(hipStreamSynchronize(s), "norms sync");
+    out[0] = h[1];
+    out[1] = h[2];
+    out[2] = h[3];
+}
+
 void System::assemble(const double* diag, const double* upper, const double* lower) {
     require(nf_ >= 0 && a0_src_.size() > 0, "assemble: system was not created from an LDU pattern");
     require(diag != nullptr && (nf_ == 0 || (upper != nullptr && lower != nullptr)),
@@ -2091,6 +2114,12 @@ cfdg_solve_stats solve_one(System& s, cfdg_method method, bool use_amg, cfdg_pre
 
 } // namespace
 
+cfdg_solve_stats cfdg::solve_system(System& s, cfdg_method method, bool use_amg, cfdg_precision precision,
+                                    const cfdg_solve_control& control) {
+    check_control(&control);
+    return solve_one(s, method, use_amg, precision, control);
+}
+
 extern "C" cfdg_status cfdg_system_create(const cfdg_ldu_pattern* pattern, cfdg_system** out) {
     return guarded([&] {
         require(pattern != nullptr && out != nullptr, "null argument");
@@ -2384,6 +2413,24 @@ extern "C" cfdg_status cfdg_batch_solve(cfdg_batch* batch, const cfdg_solve_cont
         std::vector<cfdg_solve_stats> st;
         batch->runner->solve(c, st, nullptr);
         std::copy(st.begin(), st.end(), stats);
+    });
+}
+
+extern "C" cfdg_status cfdg_system_normalised_residual(cfdg_system* sys, double* normalised, double* res,
+                                                       double* norm) {
+    return guarded([&] {
+        require(sys != nullptr, "null argument");
+        double o[3];
+        sys->sys->residual_norms(o);
+        if (normalised != nullptr) {
+            *normalised = o[0] / (o[1] + 1e-20); // k_small (ldu_matrix.cpp:22, 65-68)
+        }
+        if (res != nullptr) {
+            *res = o[0];
+        }
+        if (norm != nullptr) {
+            *norm = o[1];
+        }
     });
 }
 

@@ -81,6 +81,9 @@ public:
     void copy_rhs_from(const double* dev, hipStream_t s);
     void copy_x_to(double* dev, hipStream_t s) const;
     [[nodiscard]] std::size_t footprint() const;
+    // residual_norms (ldu_matrix.cpp:74-103) of the resident operator, right-hand side and
+    // iterate: out[0] = |b - A x|_1, out[1] = the normalisation, out[2] = |b - A x|_2^2.
+    void residual_norms(double out[3]);
 
     [[nodiscard]] int n() const { return n_; }
     [[nodiscard]] int n_faces() const { return nf_; }
@@ -136,6 +139,7 @@ private:
     DevBuf<float> r32_;
     DevBuf<double> partials_;
     DevBuf<SolveState> state_;
+    DevBuf<double> norms_; // sum of x and the three residual norms
     std::vector<DevLevel> levels_;
     bool has_amg_ = false;
     bool assembled_ = false;
@@ -149,6 +153,11 @@ private:
     bool bf16_ready_ = false;
     void make_bf16();
 };
+
+// One solve of a resident system from its resident right-hand side and iterate (the C-ABI's
+// single-system path: cached launch tables and graphs per method and precision).
+cfdg_solve_stats solve_system(System& s, cfdg_method method, bool use_amg, cfdg_precision precision,
+                              const cfdg_solve_control& control);
 
 // Solve options shared by every system of a batch.
 struct BatchOptions {

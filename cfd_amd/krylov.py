@@ -319,6 +319,14 @@ class DeviceSolver:
         L.call("cfdg_system_get_x", self._h, L.dptr(x))
         return x
 
+    def normalised_residual(self):
+        """LduMatrix::normalised_residual / residual_norms (ldu_matrix.cpp:65-103) of the
+        resident matrix, right-hand side and iterate, on the device: (res / (norm + 1e-20), res,
+        norm) with res = |b - A x|_1."""
+        v, r, m = C.c_double(), C.c_double(), C.c_double()
+        L.call("cfdg_system_normalised_residual", self._h, C.byref(v), C.byref(r), C.byref(m))
+        return v.value, r.value, m.value
+
     def solve_resident(self, method: Method, use_amg: bool, precision: Precision,
                        control: SolveControl) -> SolveStats:
         st = L.SolveStats()

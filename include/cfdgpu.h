@@ -208,6 +208,13 @@ cfdg_status cfdg_system_solve_resident(cfdg_system* sys, cfdg_method method, int
                                        cfdg_precision precision,
                                        const cfdg_solve_control* control,
                                        cfdg_solve_stats* stats);
+/* Replaces LduMatrix::normalised_residual / residual_norms (ldu_matrix.cpp:65-103) for the
+ * resident matrix, right-hand side and iterate, computed on the device: res = |b - A x|_1,
+ * norm = sum_c |(A x)_c - (A xref)_c| + |b_c - (A xref)_c| with A xref = rowsum(A) mean(x),
+ * normalised = res / (norm + 1e-20). Sums in a fixed order (reproducible run to run; they agree
+ * with the reference's serial face-order sums to rounding). Output pointers may be NULL. */
+cfdg_status cfdg_system_normalised_residual(cfdg_system* sys, double* normalised, double* res,
+                                            double* norm);
 /* Kernel timing for benchmarks (no counterpart in the reference). kind 1 brackets every launch
  * of the last level-0 Jacobi sweep of the V-cycle, kind 2 every outer FP64 q = A p, with HIP
  * events on the system's stream during the following single-system solves; 0 turns it off.
@@ -503,6 +510,61 @@ cfdg_status cfdg_amg_view_of(const cfdg_amg* amg, cfdg_amg_level_view* levels,
                              cfdg_amg_view* view);
 /* Setup wall time of the last build in milliseconds. */
 cfdg_status cfdg_amg_setup_ms(const cfdg_amg* amg, double* ms);
+
+/* ------------------------------------------------------------------------------------------
+ * The reference's caller policy for the device solver: LduMatrix::solve with a CSR pattern and
+ * an AmgCache (ldu_matrix.cpp:111-131 and solve_persistent 287-366; AmgCache
+ * ldu_matrix.hpp:61-75), so that a caller without the reference's fv library gets the same
+ * behaviour: the initial normalised residual and the early return below the tolerance, the
+ * tolerance mapping target = max(rel_tolerance, tolerance / initial) * |b - A x0|_2, the
+ * hierarchy rebuilt from the current matrix when missing or max_age solves old, AMG-PCG only
+ * when the hierarchy has more than one level (else Jacobi CG), BiCGStab without AMG, the
+ * hierarchy otherwise stale (built from an older matrix). Optionally the host AmgCg path's
+ * stale-hierarchy rule (ldu_matrix.cpp:225-258): a solve that needs more than twice the
+ * iterations of the first solve after the last setup, plus 2, rebuilds the hierarchy and
+ * continues from the current iterate.
+ * ---------------------------------------------------------------------------------------- */
+
+typedef struct cfdg_amg_cache cfdg_amg_cache;
+
+typedef struct cfdg_amg_cache_options {
+    int max_age;                /* rebuild after this many solves (ldu_matrix.hpp:74: 20) */
+    int stale_rebuild;          /* 0: solve_persistent (the device path); 1: also the rule above */
+    cfdg_amg_settings amg;      /* hierarchy settings (cfdg_amg_default_settings) */
+} cfdg_amg_cache_options;
+
+/* LinearSolverSettings (ldu_matrix.hpp:23-28) for the device solver types. */
+typedef struct cfdg_linear_settings {
+    cfdg_method method;         /* CG: GpuCg, BICGSTAB: GpuBiCgStab */
+    double tolerance;           /* absolute, on the normalised residual (reference default 1e-10) */
+    double rel_tolerance;       /* relative to the initial normalised residual (default 0.01) */
+    int max_iterations;         /* default 1000 */
+    cfdg_precision precision;   /* V-cycle precision of AMG-PCG; FP64 is the reference */
+} cfdg_linear_settings;
+
+/* LinearSolveResult (ldu_matrix.hpp:30-34) plus what the policy did. */
+typedef struct cfdg_linear_result {
+    double initial_residual;    /* normalised, before the solve */
+    double final_residual;      /* normalised, after the solve */
+    int iterations;             /* Krylov iterations (both solves when the stale rule rebuilt) */
+    int solved;                 /* 0: returned before solving (initial residual < tolerance) */
+    int used_amg;               /* AMG-PCG (1) or the method's Jacobi-preconditioned solver (0) */
+    int setups;                 /* hierarchies built during this call */
+    double target;              /* absolute tolerance on |r|_2 handed to the device solver */
+    cfdg_solve_stats stats;     /* of the last device solve of the call */
+} cfdg_linear_result;
+
+cfdg_status cfdg_amg_cache_default_options(cfdg_amg_cache_options* out);
+/* The cache owns a device system on the pattern (created here) and the current hierarchy. */
+cfdg_status cfdg_amg_cache_create(const cfdg_ldu_pattern* pattern, const cfdg_amg_cache_options* options,
+                                  cfdg_amg_cache** out);
+cfdg_status cfdg_amg_cache_destroy(cfdg_amg_cache* cache);
+/* AmgCache::age, the levels of the current hierarchy (0: none yet) and hierarchies built so far. */
+cfdg_status cfdg_amg_cache_state(const cfdg_amg_cache* cache, int* age, int* n_levels, int* setups);
+/* LduMatrix::solve (ldu_matrix.cpp:111-131, 287-366): host LDU coefficients and source, x in/out. */
+cfdg_status cfdg_ldu_solve(cfdg_amg_cache* cache, const double* diag, const double* upper,
+                           const double* lower, const double* source, double* x,
+                           const cfdg_linear_settings* settings, cfdg_linear_result* result);
 
 /* ------------------------------------------------------------------------------------------
  * Case generators (restate the reference's box mesher and FV geometry so that the benchmark

@@ -541,3 +541,83 @@ class FvReference:
                                           int(bool(venkat)), dp(e), float(central), int(bool(transpose_term)),
                                           dp(out))
         return out.reshape(self.nc, 3)
+
+
+# ------------------------------------------------------------------------------------------
+# The reference's caller policy for the device solver, restated over the oracle's primitives:
+# LduMatrix::solve with a CSR pattern and an AmgCache (src/fv/src/ldu_matrix.cpp:111-131,
+# solve_persistent 287-366; AmgCache src/fv/include/cfd/fv/ldu_matrix.hpp:61-75), and the
+# host AmgCg path's stale-hierarchy rule (src/fv/src/ldu_matrix.cpp:225-258) when
+# stale_rebuild is set.
+# ------------------------------------------------------------------------------------------
+class AmgCacheState:
+    """AmgCache (ldu_matrix.hpp:61-75): the hierarchy, its age and the iterations of the first
+    solve after the last setup."""
+
+    def __init__(self, max_age: int = 20, stale_rebuild: bool = False, decay: float = 1.0):
+        self.h: Optional[Hierarchy] = None
+        self.age = 0
+        self.setup_iterations = 0
+        self.max_age = max_age
+        self.stale_rebuild = stale_rebuild
+        self.decay = decay
+        self.setups = 0
+
+
+def ldu_csr(pattern, diag, upper, lower) -> Csr:
+    """CSR values placed through the pattern's positions (ldu_matrix.cpp:315-323)."""
+    nnz, rp, ci, dpos, upos, lpos = pattern
+    va = np.zeros(nnz)
+    va[dpos] = diag
+    va[upos] = upper
+    va[lpos] = lower
+    return Csr(len(diag), len(diag), rp, ci[:nnz], va)
+
+
+def ldu_solve(cache: AmgCacheState, owner, neighbour, pattern, diag, upper, lower, source, x,
+              method: int, tolerance: float, rel_tolerance: float, max_iterations: int) -> dict:
+    """LduMatrix::solve (ldu_matrix.cpp:111-131) for GpuCg (method 0) / GpuBiCgStab (method 1)
+    with a pattern and a cache: solve_persistent (:287-366), x in/out."""
+    n = len(diag)
+    initial = ldu_normalised_residual(owner, neighbour, diag, upper, lower, source, x)
+    out = dict(initial=initial, final=initial, iterations=0, solved=False, used_amg=False, setups=0,
+               target=None)
+    if initial < tolerance or n == 0:  # :121-124
+        return out
+    r = _d(source) - ldu_multiply(owner, neighbour, diag, upper, lower, x)
+    r0 = float(np.cumsum(r * r)[-1])  # the serial sum of :293-297
+    target = max(rel_tolerance, tolerance / max(initial, 1e-20)) * float(np.sqrt(r0))
+    a = ldu_csr(pattern, diag, upper, lower)
+    cg = method == 0
+    fresh = False
+    if cg and (cache.h is None or cache.age >= cache.max_age):  # :314-326
+        cache.h = Hierarchy(a, decay=cache.decay)
+        cache.age = 0
+        cache.setups += 1
+        out["setups"] += 1
+        fresh = True
+    if cg:  # :355-357
+        cache.age += 1
+    use_amg = cg and cache.h is not None and cache.h.n_levels > 1  # :358
+    tol = max(target, 1e-300)
+    if use_amg:
+        st = cache.h.amg_cg(source, x, tol, max_iterations, level0=a)
+    else:
+        st = solve(a, source, x, 0 if cg else 1, tol, max_iterations)
+    its = st.iterations
+    if cache.stale_rebuild and cg:  # :242-256
+        if fresh:
+            cache.setup_iterations = max(its, 1)
+        elif its > 2 * cache.setup_iterations + 2:
+            cache.h = Hierarchy(a, decay=cache.decay)
+            cache.setups += 1
+            out["setups"] += 1
+            use_amg = cache.h.n_levels > 1
+            more = (cache.h.amg_cg(source, x, tol, max_iterations, level0=a) if use_amg
+                    else solve(a, source, x, 0, tol, max_iterations))
+            its += more.iterations
+            cache.age = 1
+            cache.setup_iterations = max(more.iterations, 1)
+    out.update(final=ldu_normalised_residual(owner, neighbour, diag, upper, lower, source, x),
+               iterations=its, solved=True, used_amg=use_amg, target=tol)
+    return out
