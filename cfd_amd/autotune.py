@@ -9,7 +9,9 @@
    and writes a rule for a matrix class (rows within x1.5, mean row length within x1.25) where
    the best layout beats the library's built-in choice by more than 10%.
 2. Mixed-precision smoother. It times resident mixed-precision solves of a jittered pressure
-   system for 1 and 2 damped-Jacobi sweeps per side and keeps the faster.
+   system and systems of the batch workload's shapes for Chebyshev smoothers of degree 1 to 3
+   per side (degree k: k damped-Jacobi sweeps weighted by the reciprocals of the Chebyshev
+   polynomial's roots over [h / ratio, h]) and several interval ratios, and keeps the fastest.
 
 The result is a JSON table; `cfd_amd` applies it at import (cfdg_set_layout_rules,
 cfdg_set_mixed_smoother) unless CFDG_NO_TUNING is set. A table changes which layouts and which
@@ -100,9 +102,9 @@ def tune_layouts():
 
 
 def tune_smoother(n_side: int = 96, reps: int = 3, sample: int = 12):
-    """Mixed-precision smoother: damped-Jacobi sweeps per side and the ratio of the damping
-    interval, timed as the total mixed solve time of one n_side^3 jittered box and `sample`
-    systems of the batch workload's shapes (cases.batch_shapes)."""
+    """Mixed-precision smoother: Chebyshev degree per side (the sweeps of cfdg_set_mixed_smoother)
+    and the ratio of the damping interval, timed as the total mixed solve time of one n_side^3
+    jittered box and `sample` systems of the batch workload's shapes (cases.batch_shapes)."""
     from . import AmgHierarchy, AmgSettings, DeviceSolver, Method, Precision, SolveControl, cases
 
     systems = []
@@ -116,7 +118,8 @@ def tune_smoother(n_side: int = 96, reps: int = 3, sample: int = 12):
         a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
         systems.append((s, pat, AmgHierarchy(a, AmgSettings(strength_decay=0.5)).view()))
     result = {}
-    for sweeps, ratio in ((1, 10.0), (1, 8.0), (1, 6.0), (1, 5.0), (2, 10.0), (2, 6.0)):
+    for sweeps, ratio in ((1, 8.0), (1, 6.0), (1, 5.0), (2, 12.0), (2, 8.0), (2, 5.0), (3, 20.0), (3, 12.0),
+                          (3, 8.0)):
         L.call("cfdg_set_mixed_smoother", sweeps, ratio)
         total = 0.0
         its = 0
@@ -135,7 +138,7 @@ def tune_smoother(n_side: int = 96, reps: int = 3, sample: int = 12):
             its += st.iterations
             dev.close()
         result[(sweeps, ratio)] = (total, its)
-        print(f"mixed smoother, {sweeps} sweep(s), ratio {ratio:4.1f}: {total:.3f} ms, {its} iterations in all",
+        print(f"mixed smoother, Chebyshev degree {sweeps}, ratio {ratio:4.1f}: {total:.3f} ms, {its} iterations in all",
               flush=True)
     best = min(result, key=lambda k: result[k][0])
     return {"sweeps": best[0], "ratio": best[1],

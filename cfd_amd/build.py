@@ -32,6 +32,8 @@ def build_library(force: bool = True) -> Path:
 
 def build_oracle(force: bool = True) -> None:
     _make(ROOT / "oracle", force=force)
+    # the integration guide's binding over libcfdgpu, for its tests (needs the reference header)
+    subprocess.run([sys.executable, str(ROOT / "oracle" / "build_shim.py")], check=True)
 
 
 def main(argv=None) -> int:

@@ -281,8 +281,8 @@ __device__ __forceinline__ T ell_row(const EllView& m, int slice, int lane, cons
 // first sweep multiplies A by x = 0, so x1 = (w dinv) b exactly).
 template <class T>
 __device__ __forceinline__ T ell_row_x1(const EllView& m, int slice, int lane, const gmem_t<const T>* __restrict__ wd,
-                                        const gmem_t<const T>* __restrict__ b) {
-    return ell_row_f<T>(m, slice, lane, [&](int c) { return wd[c] * b[c]; });
+                                        const gmem_t<const T>* __restrict__ b, T f) {
+    return ell_row_f<T>(m, slice, lane, [&](int c) { return (f * wd[c]) * b[c]; });
 }
 
 // Row sums of two slices with equal widths and one column format, their loads issued together
@@ -449,19 +449,19 @@ __device__ __forceinline__ T mat_row(const EllView& m, int row, int sub, const T
 
 template <class T>
 __device__ __forceinline__ T mat_row_x1(const EllView& m, int row, int sub, const gmem_t<const T>* __restrict__ wd,
-                                        const gmem_t<const T>* __restrict__ b) {
+                                        const gmem_t<const T>* __restrict__ b, T f) {
     if (m.csr == 0) {
         const int lg = __builtin_ctz(m.tpr);
         const int slice = row >> (6 - lg);
         const T s = slice < m.n_slices
-                        ? ell_row_x1<T>(m, slice, static_cast<int>(threadIdx.x) & (k_wave - 1), wd, b)
+                        ? ell_row_x1<T>(m, slice, static_cast<int>(threadIdx.x) & (k_wave - 1), wd, b, f)
                         : T(0);
         return m.tpr > 1 ? group_sum(s, m.tpr) : s;
     }
     T s = T(0);
     if (row < m.n_rows) {
         s = csr_lane_sum<T>(gm(m.col), gm(static_cast<const T*>(m.val)), m.sp[row] + sub, m.sp[row + 1],
-                            m.tpr, [&](int c) { return wd[c] * b[c]; });
+                            m.tpr, [&](int c) { return (f * wd[c]) * b[c]; });
     }
     return group_sum(s, m.tpr);
 }
@@ -901,17 +901,20 @@ __global__ void __launch_bounds__(k_block) k_bicg_xr(const SysDev* __restrict__ 
     const int sub = static_cast<int>(threadIdx.x) & (tpr - 1);                                    \
     (void)sub;
 
-// First two Jacobi sweeps from x = 0: v[out] = x1 + wd (b - A x1), x1 = wd b (sliced-ELL
-// operators; the gathers of wd and b stay cache-resident for short rows).
+// First two Jacobi sweeps from x = 0: v[out] = x1 + (f2 wd) (b - A x1), x1 = (f1 wd) b
+// (sliced-ELL operators; the gathers of wd and b stay cache-resident for short rows). f1, f2:
+// the sweeps' factors on the level's damping (Chebyshev weights; 1 for plain damped Jacobi,
+// which leaves the arithmetic unchanged).
 template <class T>
-__global__ void __launch_bounds__(k_block) k_presmooth2(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
+__global__ void __launch_bounds__(k_block) k_presmooth2(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out,
+                                                        T f1, T f2) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
     const auto b = gm(static_cast<const T*>(L.b));
     const auto wd = gm(static_cast<const T*>(L.wd));
-    const T ax = mat_row_x1<T>(M, row, sub, wd, b);
+    const T ax = mat_row_x1<T>(M, row, sub, wd, b, f1);
     if (sub == 0 && row < L.n) {
-        const T x1 = wd[row] * b[row];
-        gm(static_cast<T*>(L.v[out]))[row] = x1 + wd[row] * (b[row] - ax);
+        const T x1 = (f1 * wd[row]) * b[row];
+        gm(static_cast<T*>(L.v[out]))[row] = x1 + (f2 * wd[row]) * (b[row] - ax);
     }
 }
 
@@ -919,22 +922,23 @@ __global__ void __launch_bounds__(k_block) k_presmooth2(const SysDev* __restrict
 // V-cycle operations on one row (or one lane of a row's group), used by the level kernels below.
 // ------------------------------------------------------------------------------------------
 
-// v[out] = wd b.
+// v[out] = (f wd) b.
 template <class T>
-__device__ __forceinline__ void op_jacobi_zero(const LevelDev& L, int row, int out) {
-    gm(static_cast<T*>(L.v[out]))[row] = gm(static_cast<const T*>(L.wd))[row] * gm(static_cast<const T*>(L.b))[row];
+__device__ __forceinline__ void op_jacobi_zero(const LevelDev& L, int row, int out, T f) {
+    gm(static_cast<T*>(L.v[out]))[row] = (f * gm(static_cast<const T*>(L.wd))[row]) * gm(static_cast<const T*>(L.b))[row];
 }
 
-// v[out] = v[in] + wd (b - A v[in]); returns the new entry (0 for rows past the end / sub > 0).
+// v[out] = v[in] + (f wd) (b - A v[in]); returns the new entry (0 for rows past the end /
+// sub > 0).
 template <class T>
-__device__ __forceinline__ T op_jacobi(const LevelDev& L, const EllView& M, int row, int sub, int in, int out) {
+__device__ __forceinline__ T op_jacobi(const LevelDev& L, const EllView& M, int row, int sub, int in, int out, T f) {
     const T* x = static_cast<const T*>(L.v[in]);
     const auto b = gm(static_cast<const T*>(L.b));
     const auto wd = gm(static_cast<const T*>(L.wd));
     const T ax = mat_row<T>(M, row, sub, x);
     T z = T(0);
     if (sub == 0 && row < L.n) {
-        z = x[row] + wd[row] * (b[row] - ax);
+        z = x[row] + (f * wd[row]) * (b[row] - ax);
         gm(static_cast<T*>(L.v[out]))[row] = z;
     }
     return z;
@@ -949,41 +953,41 @@ __device__ __forceinline__ void op_residual(const LevelDev& L, const EllView& M,
     }
 }
 
-// v[out] = b - A x1 with x1 = wd b formed per gathered column.
+// v[out] = b - A x1 with x1 = (f wd) b formed per gathered column.
 template <class T>
-__device__ __forceinline__ void op_residual_x1(const LevelDev& L, const EllView& M, int row, int sub, int out) {
+__device__ __forceinline__ void op_residual_x1(const LevelDev& L, const EllView& M, int row, int sub, int out, T f) {
     const auto b = gm(static_cast<const T*>(L.b));
     const auto wd = gm(static_cast<const T*>(L.wd));
-    const T ax = mat_row_x1<T>(M, row, sub, wd, b);
+    const T ax = mat_row_x1<T>(M, row, sub, wd, b, f);
     if (sub == 0 && row < L.n) {
         gm(static_cast<T*>(L.v[out]))[row] = b[row] - ax;
     }
 }
 
-// b_{l+1} = R_l v_l[in] for a row of level l + 1; with x1_out >= 0 also v_{l+1}[x1_out] = wd b
-// unless level l + 1 is the coarsest.
+// b_{l+1} = R_l v_l[in] for a row of level l + 1; with x1_out >= 0 also
+// v_{l+1}[x1_out] = (f wd) b unless level l + 1 is the coarsest.
 template <class T>
 __device__ __forceinline__ void op_restrict(const LevelDev& L, const LevelDev& N, const EllView& M, int row, int sub,
-                                            int in, int x1_out) {
+                                            int in, int x1_out, T f) {
     const T y = mat_row<T>(M, row, sub, static_cast<const T*>(L.v[in]));
     if (sub == 0 && row < L.n_next) {
         gm(static_cast<T*>(N.b))[row] = y;
         if (x1_out >= 0 && !N.coarsest) {
-            gm(static_cast<T*>(N.v[x1_out]))[row] = gm(static_cast<const T*>(N.wd))[row] * y;
+            gm(static_cast<T*>(N.v[x1_out]))[row] = (f * gm(static_cast<const T*>(N.wd))[row]) * y;
         }
     }
 }
 
-// v_l[io] += P_l v_{l+1}[coarse_buf]; X1: v_l[io] = wd b + P_l v_{l+1}[coarse_buf] (the level's
-// single pre-sweep from zero was not stored).
+// v_l[io] += P_l v_{l+1}[coarse_buf]; X1: v_l[io] = (f wd) b + P_l v_{l+1}[coarse_buf] (the
+// level's single pre-sweep from zero was not stored).
 template <class T, bool X1>
 __device__ __forceinline__ void op_prolong(const LevelDev& L, const LevelDev& N, const EllView& M, int row, int sub,
-                                           int io, int coarse_buf) {
+                                           int io, int coarse_buf, T f) {
     const T t = mat_row<T>(M, row, sub, static_cast<const T*>(N.v[coarse_buf]));
     if (sub == 0 && row < L.n) {
         const auto x = gm(static_cast<T*>(L.v[io]));
         if constexpr (X1) {
-            x[row] = gm(static_cast<const T*>(L.wd))[row] * gm(static_cast<const T*>(L.b))[row] + t;
+            x[row] = (f * gm(static_cast<const T*>(L.wd))[row]) * gm(static_cast<const T*>(L.b))[row] + t;
         } else {
             x[row] = x[row] + t;
         }
@@ -1007,9 +1011,10 @@ __device__ __forceinline__ void op_coarse_row(const LevelDev& L, int i, int lane
     }
 }
 
-// One Jacobi sweep from x = 0: v[out] = wd b (row-wise, 256 rows per workgroup).
+// One Jacobi sweep from x = 0: v[out] = (f wd) b (row-wise, 256 rows per workgroup).
 template <class T>
-__global__ void __launch_bounds__(k_block) k_jacobi_zero(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
+__global__ void __launch_bounds__(k_block) k_jacobi_zero(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out,
+                                                         T f) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
     if (S.st->status != ST_RUNNING) {
         return;
@@ -1018,7 +1023,7 @@ __global__ void __launch_bounds__(k_block) k_jacobi_zero(const SysDev* __restric
     if (L.coarsest || row >= L.n) {
         return;
     }
-    op_jacobi_zero<T>(L, row, out);
+    op_jacobi_zero<T>(L, row, out, f);
 }
 
 // v[out] = 0 (smoothing-free V-cycles; row-wise).
@@ -1035,13 +1040,14 @@ __global__ void __launch_bounds__(k_block) k_zero_level(const SysDev* __restrict
     gm(static_cast<T*>(L.v[out]))[row] = T(0);
 }
 
-// Jacobi sweep v[out] = v[in] + wd (b - A v[in]). Dots (level 0 only, sliced ELL): 0 none;
+// Jacobi sweep v[out] = v[in] + (f wd) (b - A v[in]). Dots (level 0 only, sliced ELL): 0 none;
 // 1 partial r.z with r = b (FP64 level 0, where b aliases r); 2 partials r.z and z.q with r, q
 // the FP64 outer vectors (mixed level 0).
 template <class T, int Dots>
-__global__ void __launch_bounds__(k_block) k_jacobi(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in, int out) {
+__global__ void __launch_bounds__(k_block) k_jacobi(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in, int out,
+                                                    T f) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
-    const T z = op_jacobi<T>(L, M, row, sub, in, out);
+    const T z = op_jacobi<T>(L, M, row, sub, in, out, f);
     double d1 = 0.0;
     double d2 = 0.0;
     if (Dots != 0 && sub == 0 && row < L.n) {
@@ -1097,10 +1103,10 @@ constexpr int k_block_pair = k_block / 2;
     const int row0 = s0 * k_wave + lane;                                                          \
     const int row1 = row0 + k_wave;
 
-// Jacobi sweep v[out] = v[in] + wd (b - A v[in]) of level 0 with the partial dots of k_jacobi.
+// Jacobi sweep v[out] = v[in] + (f wd) (b - A v[in]) of level 0 with the partial dots of k_jacobi.
 template <class T, int Dots>
 __global__ void __launch_bounds__(k_block_pair) k_jacobi_pair(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in,
-                                                              int out) {
+                                                              int out, T f) {
     CFDG_PAIR_PROLOGUE(sys, blocks, level)
     const T* x = static_cast<const T*>(L.v[in]);
     const auto gx = gm(x);
@@ -1118,7 +1124,7 @@ __global__ void __launch_bounds__(k_block_pair) k_jacobi_pair(const SysDev* __re
     for (int h = 0; h < 2; ++h) {
         const int row = rows[h];
         if (row < L.n) {
-            const T z = gx[row] + wd[row] * (b[row] - ys[h]);
+            const T z = gx[row] + (f * wd[row]) * (b[row] - ys[h]);
             o[row] = z;
             if (Dots == 1) {
                 d1[h] = static_cast<double>(b[row]) * static_cast<double>(z);
@@ -1206,10 +1212,11 @@ __device__ __forceinline__ void pair_rows(const EllView& m, int y, F load, E epi
         return;                                                                                   \
     }
 
-// k_prolong with pair_rows: v_l[io] += P_l v_{l+1}[coarse_buf]; X1: v_l[io] = wd b + P_l v_{l+1}.
+// k_prolong with pair_rows: v_l[io] += P_l v_{l+1}[coarse_buf]; X1: v_l[io] = (f wd) b +
+// P_l v_{l+1}.
 template <class T, bool X1>
 __global__ void __launch_bounds__(k_block_pair) k_prolong_pair(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int io,
-                                                               int coarse_buf) {
+                                                               int coarse_buf, T f) {
     CFDG_PAIR_LEVEL_PROLOGUE(sys, blocks, level)
     const LevelDev& N = S.lv[level + 1];
     const auto e = gm(static_cast<const T*>(N.v[coarse_buf]));
@@ -1217,7 +1224,7 @@ __global__ void __launch_bounds__(k_block_pair) k_prolong_pair(const SysDev* __r
     pair_rows<T>(L.p, bk_.y, [&](int c) { return e[c]; }, [&](int row, int sub, T t) {
         if (sub == 0 && row < L.n) {
             if constexpr (X1) {
-                x[row] = gm(static_cast<const T*>(L.wd))[row] * gm(static_cast<const T*>(L.b))[row] + t;
+                x[row] = (f * gm(static_cast<const T*>(L.wd))[row]) * gm(static_cast<const T*>(L.b))[row] + t;
             } else {
                 x[row] = x[row] + t;
             }
@@ -1233,30 +1240,31 @@ __global__ void __launch_bounds__(k_block) k_residual(const SysDev* __restrict__
 }
 
 // b_{l+1} = R_l v_l[in]; rows of level l + 1. With x1_out >= 0 the next level's first
-// Jacobi sweep from zero is stored too, v_{l+1}[x1_out] = wd b_{l+1} (what k_jacobi_zero
+// Jacobi sweep from zero is stored too, v_{l+1}[x1_out] = (f wd) b_{l+1} (what k_jacobi_zero
 // would compute from the stored b), unless level l + 1 is the coarsest.
 template <class T>
 __global__ void __launch_bounds__(k_block) k_restrict(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in,
-                                                      int x1_out) {
+                                                      int x1_out, T f) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, r)
-    op_restrict<T>(L, S.lv[level + 1], M, row, sub, in, x1_out);
+    op_restrict<T>(L, S.lv[level + 1], M, row, sub, in, x1_out, f);
 }
 
 // v_l[io] += P_l v_{l+1}[coarse_buf]. X1: the level's single pre-sweep from zero was not
-// stored; its result wd b is formed here (v[io] = wd b + P v_{l+1}, the same sums).
+// stored; its result (f wd) b is formed here (v[io] = (f wd) b + P v_{l+1}, the same sums).
 template <class T, bool X1 = false>
 __global__ void __launch_bounds__(k_block) k_prolong(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int io,
-                          int coarse_buf) {
+                          int coarse_buf, T f) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, p)
-    op_prolong<T, X1>(L, S.lv[level + 1], M, row, sub, io, coarse_buf);
+    op_prolong<T, X1>(L, S.lv[level + 1], M, row, sub, io, coarse_buf, f);
 }
 
-// v[out] = b - A x1 with x1 = wd b, the single Jacobi sweep from zero, formed per gathered
-// column instead of stored (one pre-sweep per side).
+// v[out] = b - A x1 with x1 = (f wd) b, the single Jacobi sweep from zero, formed per
+// gathered column instead of stored (one pre-sweep per side).
 template <class T>
-__global__ void __launch_bounds__(k_block) k_residual_x1(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
+__global__ void __launch_bounds__(k_block) k_residual_x1(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out,
+                                                         T f) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, a)
-    op_residual_x1<T>(L, M, row, sub, out);
+    op_residual_x1<T>(L, M, row, sub, out, f);
 }
 
 // Coarsest level: v[out] = C^-1 b with the dense FP64 inverse, one wavefront per row; blocks

@@ -325,9 +325,11 @@ double gershgorin(const cfdg_csr& a) {
 // centre of [h / ratio, h] (the first-degree Chebyshev smoother, which damps the upper part of
 // the spectrum uniformly). A sweep diverges on the top mode only if h (1 + 1 / ratio) is below
 // the largest eigenvalue.
-double mixed_weight(double h) {
+// Damping of the mixed smoother on a level whose D^-1 A spectrum ends at h: the centre of
+// [h / ratio, h]; ratio 0 selects the hierarchy's own (reference) weight w_ref instead.
+double mixed_weight(double h, double w_ref) {
     const double ratio = mixed_ratio();
-    return 2.0 / (h + h / ratio);
+    return ratio == 0.0 ? w_ref : 2.0 / (h + h / ratio);
 }
 
 
@@ -433,7 +435,7 @@ double mixed_ratio() {
     if (v < 0.0) {
         const char* e = std::getenv("CFDG_MIXED_RATIO");
         const double r = e != nullptr ? std::atof(e) : 6.0;
-        v = r > 1.0 ? r : 6.0;
+        v = (r > 1.0 || r == 0.0) ? r : 6.0;
         double expected = -1.0;
         g_mixed_ratio.compare_exchange_strong(expected, v);
         v = g_mixed_ratio.load();
@@ -989,7 +991,7 @@ void System::set_amg(const cfdg_amg_view& v) {
             d.wd64.upload(wd.data(), wd.size(), s);
             // Upper end of the spectrum of D^-1 A: the power-iteration estimate (4-8% low on the
             // box hierarchies) with a 15% margin.
-            const double w32 = mixed_weight(1.15 * device_lambda(d.a, d.wd64, w, s));
+            const double w32 = mixed_ratio() == 0.0 ? w : mixed_weight(1.15 * device_lambda(d.a, d.wd64, w, s), w);
             std::vector<float> wd32(dinv.size());
             for (std::size_t i = 0; i < dinv.size(); ++i) {
                 wd32[i] = static_cast<float>(w32 * dinv[i]);
@@ -1012,7 +1014,8 @@ void System::set_amg(const cfdg_amg_view& v) {
     levels_ = std::move(lv);
     sweeps_ = v.smooth_sweeps;
     weight_ = w;
-    weight0_32_ = mixed_weight(lambda0);
+    weight0_32_ = mixed_weight(lambda0, w);
+    mixed_ratio_ = mixed_ratio();
     has_amg_ = true;
     ++generation_;
     // Level-0 smoother weights follow the (possibly new) weights.
@@ -1377,9 +1380,37 @@ int final_index(int sweeps) {
 // only). Measured slower elsewhere: on BF16 rows, on the x1-forming residual (two gathers per
 // entry), and on every coarser level or multi-lane operator (the halved workgroups cost more
 // than the extra loads in flight gain there).
+// Factors of the sweeps' damping on every level: the degree-k Chebyshev smoother over the
+// damping interval [h / ratio, h] is k damped-Jacobi sweeps whose weights are the reciprocals of
+// the polynomial's roots, tau_j = c + d cos(pi (2j - 1) / (2k)) with c = (h + h / ratio) / 2 and
+// d = (h - h / ratio) / 2. Relative to the level's stored weight 1 / c (degree 1), sweep j takes
+// c / tau_j = 1 / (1 + q cos(pi (2j - 1) / (2k))), q = (ratio - 1) / (ratio + 1), independent of
+// h. Pre-smoothing runs j = 1..k, post-smoothing k..1 (a symmetric cycle). ratio 0 (the
+// reference weight) and k = 1 give factors of exactly 1, which leave the arithmetic unchanged.
+struct SweepFactors {
+    double pre[64];
+    [[nodiscard]] double post(int j, int k) const { return pre[k - 1 - j]; }
+};
+
+SweepFactors sweep_factors(int k, double ratio) {
+    SweepFactors f{};
+    const double q = ratio > 1.0 ? (ratio - 1.0) / (ratio + 1.0) : 0.0;
+    for (int j = 0; j < 64; ++j) {
+        f.pre[j] = 1.0;
+    }
+    if (q > 0.0 && k > 1) {
+        const double pi = 3.14159265358979323846;
+        for (int j = 0; j < k && j < 64; ++j) {
+            f.pre[j] = 1.0 / (1.0 + q * std::cos(pi * (2.0 * j + 1.0) / (2.0 * k)));
+        }
+    }
+    return f;
+}
+
 template <class T, int Dots>
-void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s, bool pair) {
+void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s, bool pair, const SweepFactors& sf) {
     const int fin = final_index(sweeps);
+    const T f1 = static_cast<T>(sf.pre[0]);
     const int L = tb.l_max;
     CFDG_LAUNCH(k_coarse<T>, tb.cn[0], s, d, tb.c[0].get(), 0, fin);
     std::vector<int> cur(static_cast<std::size_t>(L), 0);
@@ -1411,34 +1442,35 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
             // inside the prolongation; bitwise the same as a stored sweep
             cur[li] = 0;
             if (tb.x1[li] != 0) {
-                CFDG_LAUNCH(k_residual_x1<T>, ga, s, d, ba, l, 1);
+                CFDG_LAUNCH(k_residual_x1<T>, ga, s, d, ba, l, 1, f1);
             } else {
                 if (l == 0) {
-                    CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 0);
+                    CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 0, f1);
                 }
                 CFDG_LAUNCH(k_residual<T>, ga, s, d, ba, l, 0, 1);
             }
-            CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1, x1_next);
+            CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1, x1_next, f1);
             CFDG_LAUNCH(k_coarse<T>, tb.cn[li + 1], s, d, tb.c[li + 1].get(), l + 1, fin);
             continue;
         } else {
+            const T f2 = static_cast<T>(sf.pre[1]);
             if (tb.a_ell[li] != 0) {
-                CFDG_LAUNCH(k_presmooth2<T>, ga, s, d, ba, l, 0);
+                CFDG_LAUNCH(k_presmooth2<T>, ga, s, d, ba, l, 0, f1, f2);
             } else { // long rows: form x1 = wd b once instead of gathering it per entry
                 if (l == 0) {
-                    CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 2);
+                    CFDG_LAUNCH(k_jacobi_zero<T>, g, s, d, blk, l, 2, f1);
                 }
-                CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, 2, 0);
+                CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, 2, 0, f2);
             }
             for (int k = 2; k < sweeps; ++k) {
                 const int o = c == 0 ? 2 : 0;
-                CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, c, o);
+                CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, c, o, static_cast<T>(sf.pre[k]));
                 c = o;
             }
         }
         cur[li] = c;
         CFDG_LAUNCH(k_residual<T>, ga, s, d, ba, l, c, 1);
-        CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1, x1_next);
+        CFDG_LAUNCH(k_restrict<T>, tb.tmn[2][li], s, d, tb.tm[2][li].get(), l, 1, x1_next, f1);
         CFDG_LAUNCH(k_coarse<T>, tb.cn[li + 1], s, d, tb.c[li + 1].get(), l + 1, fin);
     }
     for (int l = L - 2; l >= 0; --l) {
@@ -1448,25 +1480,26 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
         const int c0 = cur[li];
         if (sweeps == 1) {
             CFDG_LAUNCH_EITHER(pair && l == 0, (k_prolong_pair<T, true>), (k_prolong<T, true>), tb.tmn[1][li], s, d,
-                               tb.tm[1][li].get(), l, c0, fin);
+                               tb.tm[1][li].get(), l, c0, fin, f1);
         } else {
             CFDG_LAUNCH_EITHER(pair && l == 0, (k_prolong_pair<T, false>), (k_prolong<T, false>), tb.tmn[1][li], s,
-                               d, tb.tm[1][li].get(), l, c0, fin);
+                               d, tb.tm[1][li].get(), l, c0, fin, T(1));
         }
         int c = c0;
         for (int k = 0; k < sweeps; ++k) {
             const int o = c == 1 ? c0 : 1;
             const bool last = k == sweeps - 1;
+            const T fk = static_cast<T>(sf.post(k, sweeps));
             if (l == 0 && last) {
                 probe_begin(PROBE_L0_SWEEP, s);
                 if (pair) {
-                    CFDG_LAUNCH_PAIR((k_jacobi_pair<T, Dots>), ga, s, d, ba, l, c, o);
+                    CFDG_LAUNCH_PAIR((k_jacobi_pair<T, Dots>), ga, s, d, ba, l, c, o, fk);
                 } else {
-                    CFDG_LAUNCH((k_jacobi<T, Dots>), ga, s, d, ba, l, c, o);
+                    CFDG_LAUNCH((k_jacobi<T, Dots>), ga, s, d, ba, l, c, o, fk);
                 }
                 probe_end(PROBE_L0_SWEEP, s);
             } else {
-                CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, c, o);
+                CFDG_LAUNCH((k_jacobi<T, 0>), ga, s, d, ba, l, c, o, fk);
             }
             c = o;
         }
@@ -1491,6 +1524,7 @@ struct Plan {
     bool fp32;
     int sweeps;
     bool bf16 = false;
+    SweepFactors sf = sweep_factors(1, 0.0); // all 1: plain damped Jacobi
 };
 
 void enqueue_init(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t s) {
@@ -1511,12 +1545,12 @@ void enqueue_init(const Plan& pl, const SysDev* d, const Tables& tb, hipStream_t
     if (pl.fp32) {
         CFDG_LAUNCH(k_residual0<true>, g0, s, d, b0);
         fin<FIN_AMG_INIT>(d, tb, s);
-        enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s, !pl.bf16 && pair_sweeps());
+        enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s, !pl.bf16 && pair_sweeps(), pl.sf);
         fin<FIN_BETA_FLEX>(d, tb, s);
     } else {
         CFDG_LAUNCH(k_residual0<false>, g0, s, d, b0);
         fin<FIN_AMG_INIT>(d, tb, s);
-        enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s, false);
+        enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s, false, pl.sf);
         fin<FIN_BETA>(d, tb, s);
     }
 }
@@ -1561,12 +1595,12 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
     if (pl.fp32) {
         CFDG_LAUNCH(k_update_xr<2>, g0, s, d, b0, parity);
         fin<FIN_RES>(d, tb, s);
-        enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s, !pl.bf16 && pair_sweeps());
+        enqueue_vcycle<float, 2>(d, tb, pl.sweeps, s, !pl.bf16 && pair_sweeps(), pl.sf);
         fin<FIN_BETA_FLEX>(d, tb, s, status_out);
     } else {
         CFDG_LAUNCH(k_update_xr<0>, g0, s, d, b0, parity);
         fin<FIN_RES>(d, tb, s);
-        enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s, false);
+        enqueue_vcycle<double, 1>(d, tb, pl.sweeps, s, false, pl.sf);
         fin<FIN_BETA>(d, tb, s, status_out);
     }
 }
@@ -1600,6 +1634,13 @@ public:
         }
         pl_ = Plan{opt.method, amg, fp32, sweeps};
         pl_.bf16 = amg && opt.precision == CFDG_PREC_MIXED_BF16;
+        if (fp32) {
+            const double ratio = sys_[0]->smoother_ratio();
+            for (const System* x : sys_) {
+                require(x->smoother_ratio() == ratio, "solve_batch: systems differ in the mixed smoother");
+            }
+            pl_.sf = sweep_factors(sweeps, ratio);
+        }
         precision_ = amg ? opt.precision : CFDG_PREC_FP64;
         const bool bf16 = amg && opt.precision == CFDG_PREC_MIXED_BF16;
         build_tables(tb_, sys_, amg, s_);
@@ -1823,6 +1864,7 @@ enum SlotPhase : int { SLOT_FREE = 0, SLOT_RUNNING = 1, SLOT_FINISHING = 2 };
 struct SlotMachinery {
     int k = 0;
     Plan pl{};
+    double ratio = 0.0; // mixed smoother interval ratio of the plan's sweep factors
     cfdg_precision precision = CFDG_PREC_FP64;
     hipStream_t s = nullptr;
     DevBuf<SysDev> desc;
@@ -1860,6 +1902,10 @@ SlotRunner::SlotRunner(int slots, const BatchOptions& opt, int sweeps, hipStream
     const bool fp32 = amg && opt.precision != CFDG_PREC_FP64;
     m.pl = Plan{opt.method, amg, fp32, fp32 ? mixed_sweeps() : sweeps};
     m.pl.bf16 = amg && opt.precision == CFDG_PREC_MIXED_BF16;
+    if (fp32) {
+        m.pl.sf = sweep_factors(m.pl.sweeps, mixed_ratio());
+        m.ratio = mixed_ratio();
+    }
     m.precision = amg ? opt.precision : CFDG_PREC_FP64;
     m.s = s;
     const std::size_t k = static_cast<std::size_t>(slots);
@@ -1908,6 +1954,8 @@ int SlotRunner::start(System* x, const cfdg_solve_control& c) {
     if (m.pl.amg) {
         require(x->has_amg(), "solve: use_amg requested but no hierarchy was set");
         require(x->sweeps(m.pl.fp32) == m.pl.sweeps, "slot runner: system's smooth_sweeps differ from the runner's");
+        require(!m.pl.fp32 || x->smoother_ratio() == m.ratio,
+                "slot runner: system's mixed smoother differs from the runner's");
     }
     const int j = static_cast<int>(it - m.phase.begin());
     const std::size_t ju = static_cast<std::size_t>(j);
@@ -2288,7 +2336,7 @@ __global__ void __launch_bounds__(k_block) k_spmv_view(EllView m, const T* __res
 extern "C" cfdg_status cfdg_set_mixed_smoother(int sweeps, double ratio) {
     return guarded([&] {
         require(sweeps >= 0 && sweeps <= 64, "mixed smoother: sweeps must be in [0, 64]");
-        require(ratio > 1.0, "mixed smoother: ratio must exceed 1");
+        require(ratio > 1.0 || ratio == 0.0, "mixed smoother: ratio must exceed 1 (or be 0: reference weight)");
         g_mixed_sweeps.store(sweeps);
         g_mixed_ratio.store(ratio);
     });

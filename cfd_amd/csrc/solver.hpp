@@ -93,6 +93,9 @@ public:
     // smoother's (FP32 V-cycle).
     [[nodiscard]] int sweeps(bool fp32 = false) const { return fp32 ? mixed_sweeps() : sweeps_; }
     [[nodiscard]] double weight() const { return weight_; }
+    // Ratio of the mixed smoother's damping interval the hierarchy's FP32 weights were set for
+    // (0: the hierarchy's own weight).
+    [[nodiscard]] double smoother_ratio() const { return mixed_ratio_; }
     [[nodiscard]] hipStream_t stream() const { return stream_.get(); }
     [[nodiscard]] SolveState* state() const { return state_.get(); }
     [[nodiscard]] int level_rows(int l) const { return levels_[static_cast<std::size_t>(l)].n; }
@@ -146,6 +149,7 @@ private:
     int sweeps_ = 2;
     double weight_ = 0.7;
     double weight0_32_ = 0.7; // level-0 damping of the mixed smoother
+    double mixed_ratio_ = 0.0; // interval ratio of the FP32 weights (set with the hierarchy)
     unsigned generation_ = 0;
     unsigned long long uid_ = next_uid();
     static unsigned long long next_uid();

@@ -442,7 +442,9 @@ cfdg_status cfdg_set_layout_rules(const cfdg_layout_rule* rules, int count);
 cfdg_status cfdg_layout_of(int n_rows, const int* row_ptr, int* lanes, int* csr);
 /* Smoother of the mixed-precision V-cycle: damped-Jacobi sweeps per side (default 1) and the
  * ratio of the damping interval [h / ratio, h] (default 6), h from a per-level estimate of the
- * largest eigenvalue of D^-1 A. The ratio applies to hierarchies set afterwards. */
+ * largest eigenvalue of D^-1 A. Ratio 0 selects the hierarchy's own damping weight instead
+ * (jacobi_weight, the reference smoother), so that with 2 sweeps the mixed V-cycle is the FP64
+ * path's V-cycle in FP32. The ratio applies to hierarchies set afterwards. */
 cfdg_status cfdg_set_mixed_smoother(int sweeps, double ratio);
 /* Average device time of y = A x (fp32 != 0: FP32 values, else FP64) over reps launches with
  * the given layout. */
