@@ -311,6 +311,8 @@ _SIGNATURES = {
     "cfdg_amg_coarse_inverse": (ST, [VP, P_DBL]),
     "cfdg_amg_view_of": (ST, [VP, C.POINTER(AmgLevelView), C.POINTER(AmgView)]),
     "cfdg_amg_setup_ms": (ST, [VP, P_DBL]),
+    "cfdg_amg_coarse_on_device": (ST, [VP, P_INT]),
+    "cfdg_dense_inverse_batch": (ST, [C.c_int, P_INT, C.POINTER(VP), C.POINTER(VP), P_INT]),
     "cfdg_amg_cache_default_options": (ST, [C.POINTER(AmgCacheOptions)]),
     "cfdg_amg_cache_create": (ST, [C.POINTER(LduPattern), C.POINTER(AmgCacheOptions), C.POINTER(VP)]),
     "cfdg_amg_cache_destroy": (ST, [VP]),

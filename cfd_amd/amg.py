@@ -86,6 +86,13 @@ class AmgHierarchy:
         return out.reshape(nc, nc)
 
     @property
+    def coarse_on_device(self) -> bool:
+        """True when the coarsest inverse was computed on the matrix cores (device setup)."""
+        v = C.c_int()
+        L.call("cfdg_amg_coarse_on_device", self._h, C.byref(v))
+        return bool(v.value)
+
+    @property
     def setup_ms(self) -> float:
         v = C.c_double()
         L.call("cfdg_amg_setup_ms", self._h, C.byref(v))
@@ -111,3 +118,21 @@ class AmgHierarchy:
             levels.append(lv)
         return AmgView(levels, self.coarse_inverse(), self.settings.smooth_sweeps,
                        self.settings.jacobi_weight)
+
+
+def dense_inverse_batch(mats):
+    """cfdg_dense_inverse_batch: inverts square float64 matrices on the device (block
+    Gauss-Jordan on the FP64 matrix cores, orders up to 512), the host path for the rest. Returns
+    (inverses, on_device flags)."""
+    mats = [np.ascontiguousarray(m, np.float64) for m in mats]
+    for m in mats:
+        if m.ndim != 2 or m.shape[0] != m.shape[1]:
+            raise L.InvalidArgument(1, "dense_inverse_batch: square matrices required")
+    count = len(mats)
+    outs = [np.zeros_like(m) for m in mats]
+    order = (C.c_int * max(count, 1))(*[m.shape[0] for m in mats])
+    a = (C.c_void_p * max(count, 1))(*[m.ctypes.data for m in mats])
+    inv = (C.c_void_p * max(count, 1))(*[o.ctypes.data for o in outs])
+    flags = (C.c_int * max(count, 1))()
+    L.call("cfdg_dense_inverse_batch", count, order, a, inv, flags)
+    return outs, [bool(flags[i]) for i in range(count)]

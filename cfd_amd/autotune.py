@@ -9,7 +9,8 @@
    and writes a rule for a matrix class (rows within x1.5, mean row length within x1.25) where
    the best layout beats the library's built-in choice by more than 10%.
 2. Mixed-precision smoother. It times resident mixed-precision solves of a jittered pressure
-   system and systems of the batch workload's shapes for Chebyshev smoothers of degree 1 to 3
+   system (the 128^3 benchmark box) and systems of the batch workload's shapes for Chebyshev
+   smoothers of degree 1 to 3
    per side (degree k: k damped-Jacobi sweeps weighted by the reciprocals of the Chebyshev
    polynomial's roots over [h / ratio, h]) and several interval ratios, and keeps the fastest.
 
@@ -101,7 +102,7 @@ def tune_layouts():
     return rules, log
 
 
-def tune_smoother(n_side: int = 96, reps: int = 3, sample: int = 12):
+def tune_smoother(n_side: int = 128, reps: int = 3, sample: int = 12):
     """Mixed-precision smoother: Chebyshev degree per side (the sweeps of cfdg_set_mixed_smoother)
     and the ratio of the damping interval, timed as the total mixed solve time of one n_side^3
     jittered box and `sample` systems of the batch workload's shapes (cases.batch_shapes)."""
@@ -118,7 +119,7 @@ def tune_smoother(n_side: int = 96, reps: int = 3, sample: int = 12):
         a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
         systems.append((s, pat, AmgHierarchy(a, AmgSettings(strength_decay=0.5)).view()))
     result = {}
-    for sweeps, ratio in ((1, 8.0), (1, 6.0), (1, 5.0), (2, 12.0), (2, 8.0), (2, 5.0), (3, 20.0), (3, 12.0),
+    for sweeps, ratio in ((1, 8.0), (1, 6.0), (1, 5.0), (2, 8.0), (2, 5.0), (2, 4.0), (2, 3.0), (3, 12.0),
                           (3, 8.0)):
         L.call("cfdg_set_mixed_smoother", sweeps, ratio)
         total = 0.0

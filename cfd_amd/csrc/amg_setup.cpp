@@ -8,6 +8,7 @@
 // with -ffp-contract=off: the x86-64 reference has no fused multiply-add either.
 #include "amg_device.hpp"
 #include "common.hpp"
+#include "dense.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -34,6 +35,7 @@ struct AmgHierarchy {
     cfdg_amg_settings settings{};
     std::vector<Level> levels;
     std::vector<double> coarse_inverse; // row-major n_c x n_c
+    int coarse_on_device = 0;           // 1: inverted on the matrix cores
     double setup_ms = 0.0;
 };
 
@@ -465,14 +467,10 @@ std::vector<double> pseudo_inverse(int n, const std::vector<double>& a) {
 }
 
 // Dense inverse of the coarsest operator, column j = A^-1 e_j (amg.cpp:227-240).
-std::vector<double> coarse_inverse(const Csr& a) {
-    const int n = a.n_rows;
-    std::vector<double> m(static_cast<std::size_t>(n) * n, 0.0);
-    for (int i = 0; i < n; ++i) {
-        for (int k = a.rp[static_cast<std::size_t>(i)]; k < a.rp[static_cast<std::size_t>(i) + 1]; ++k) {
-            m[static_cast<std::size_t>(i) * n + a.ci[static_cast<std::size_t>(k)]] = a.va[static_cast<std::size_t>(k)];
-        }
-    }
+// Inverse of a dense row-major n x n matrix as the reference computes the coarsest one
+// (amg.cpp:185-189, 227-240): LDL^T and one solve per column, the pseudo-inverse when LDL^T meets
+// a zero pivot.
+std::vector<double> host_dense_inverse(int n, std::vector<double> m) {
     const std::vector<double> dense = m;
     std::vector<double> d;
     if (!ldlt(n, m, d)) {
@@ -505,6 +503,34 @@ std::vector<double> coarse_inverse(const Csr& a) {
         }
     }
     return inv;
+}
+
+std::vector<double> dense_of(const Csr& a) {
+    const int n = a.n_rows;
+    std::vector<double> m(static_cast<std::size_t>(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        for (int k = a.rp[static_cast<std::size_t>(i)]; k < a.rp[static_cast<std::size_t>(i) + 1]; ++k) {
+            m[static_cast<std::size_t>(i) * n + a.ci[static_cast<std::size_t>(k)]] = a.va[static_cast<std::size_t>(k)];
+        }
+    }
+    return m;
+}
+
+// The coarsest inverse: on the host, or (device setup) on the matrix cores with the host path
+// where the device reports a zero pivot or the order exceeds its limit.
+std::vector<double> coarse_inverse(const Csr& a, bool device, int& on_device) {
+    const int n = a.n_rows;
+    std::vector<double> m = dense_of(a);
+    on_device = 0;
+    if (device && n >= 1) {
+        std::vector<double> inv(static_cast<std::size_t>(n) * n, 0.0);
+        const std::vector<int> done = dense_inverse_batch({n}, {m.data()}, {inv.data()}, nullptr);
+        if (done[0] != 0) {
+            on_device = 1;
+            return inv;
+        }
+    }
+    return host_dense_inverse(n, std::move(m));
 }
 
 // The setup loop (amg.cpp:128-190). With `device`, the sparse products and the first two
@@ -568,7 +594,7 @@ void build(AmgHierarchy& h, Csr a0, bool device = false) {
         h.levels.push_back(std::move(level));
         current = std::move(coarse);
     }
-    h.coarse_inverse = coarse_inverse(h.levels.back().a);
+    h.coarse_inverse = coarse_inverse(h.levels.back().a, device, h.coarse_on_device);
 }
 
 } // namespace
@@ -779,6 +805,43 @@ extern "C" cfdg_status cfdg_amg_view_of(const cfdg_amg* amg, cfdg_amg_level_view
         view->coarse_inverse = amg->h.coarse_inverse.data();
         view->smooth_sweeps = amg->h.settings.smooth_sweeps;
         view->jacobi_weight = amg->h.settings.jacobi_weight;
+    });
+}
+
+extern "C" cfdg_status cfdg_amg_coarse_on_device(const cfdg_amg* amg, int* on_device) {
+    return guarded([&] {
+        require(amg != nullptr && on_device != nullptr, "null argument");
+        *on_device = amg->h.coarse_on_device;
+    });
+}
+
+extern "C" cfdg_status cfdg_dense_inverse_batch(int count, const int* order, const double* const* a,
+                                                double* const* inv, int* on_device) {
+    return guarded([&] {
+        require(count >= 0, "dense inverse: negative count");
+        require(count == 0 || (order != nullptr && a != nullptr && inv != nullptr), "null argument");
+        std::vector<int> n(order, order + count);
+        std::vector<const double*> in(a, a + count);
+        std::vector<double*> out(inv, inv + count);
+        for (int i = 0; i < count; ++i) {
+            require(n[static_cast<std::size_t>(i)] >= 0, "dense inverse: negative order");
+            require(n[static_cast<std::size_t>(i)] == 0 || (in[static_cast<std::size_t>(i)] != nullptr &&
+                                                           out[static_cast<std::size_t>(i)] != nullptr),
+                    "null argument");
+        }
+        require_device();
+        const std::vector<int> done = dense_inverse_batch(n, in, out, nullptr);
+        for (int i = 0; i < count; ++i) {
+            const std::size_t k = static_cast<std::size_t>(i);
+            if (done[k] == 0 && n[k] > 0) {
+                const std::size_t nn = static_cast<std::size_t>(n[k]) * static_cast<std::size_t>(n[k]);
+                const std::vector<double> r = host_dense_inverse(n[k], std::vector<double>(in[k], in[k] + nn));
+                std::copy(r.begin(), r.end(), out[k]);
+            }
+            if (on_device != nullptr) {
+                on_device[k] = done[k];
+            }
+        }
     });
 }
 

@@ -512,6 +512,20 @@ cfdg_status cfdg_amg_view_of(const cfdg_amg* amg, cfdg_amg_level_view* levels,
                              cfdg_amg_view* view);
 /* Setup wall time of the last build in milliseconds. */
 cfdg_status cfdg_amg_setup_ms(const cfdg_amg* amg, double* ms);
+/* 1 when the hierarchy's coarsest inverse was computed on the matrix cores (cfdg_amg_build_device
+ * inverts it with cfdg_dense_inverse_batch's kernel), 0 when on the host. */
+cfdg_status cfdg_amg_coarse_on_device(const cfdg_amg* amg, int* on_device);
+
+/* Batched dense inversion of coarsest-level operators (no single counterpart in the reference,
+ * which inverts its one coarsest operator on the host: n_c LDL^T solves, amg.cpp:185-189,
+ * 227-240). count row-major host matrices a[i] of order order[i] are inverted into inv[i]; on the
+ * device, one workgroup per matrix runs block Gauss-Jordan elimination in 16 x 16 tiles on the
+ * FP64 matrix cores, for orders up to 512. Matrices the device does not take (larger orders, or a
+ * zero or non-finite pivot, where LDL^T fails too) are inverted by the host path, which falls back
+ * to the pseudo-inverse as the reference does. on_device[i] (may be NULL): 1 when the device
+ * produced inv[i]. */
+cfdg_status cfdg_dense_inverse_batch(int count, const int* order, const double* const* a,
+                                     double* const* inv, int* on_device);
 
 /* ------------------------------------------------------------------------------------------
  * The reference's caller policy for the device solver: LduMatrix::solve with a CSR pattern and
