@@ -82,4 +82,8 @@ struct Csr {
 // Validates a borrowed CSR view and copies it.
 Csr copy_csr(const cfdg_csr& v, const char* what);
 
+// R (I - A diag(wd)): the restriction of the residual that one damped-Jacobi sweep from zero
+// leaves (x1 = diag(wd) b), as one matrix (restriction.cpp).
+Csr fused_restriction(const cfdg_csr& r, const cfdg_csr& a, const float* wd);
+
 } // namespace cfdg

@@ -964,12 +964,12 @@ __device__ __forceinline__ void op_residual_x1(const LevelDev& L, const EllView&
     }
 }
 
-// b_{l+1} = R_l v_l[in] for a row of level l + 1; with x1_out >= 0 also
-// v_{l+1}[x1_out] = (f wd) b unless level l + 1 is the coarsest.
+// b_{l+1} = M x for a row of level l + 1 (M = R_l on a level vector, or the fused restriction
+// on b_l); with x1_out >= 0 also v_{l+1}[x1_out] = (f wd) b unless level l + 1 is the coarsest.
 template <class T>
 __device__ __forceinline__ void op_restrict(const LevelDev& L, const LevelDev& N, const EllView& M, int row, int sub,
-                                            int in, int x1_out, T f) {
-    const T y = mat_row<T>(M, row, sub, static_cast<const T*>(L.v[in]));
+                                            const T* x, int x1_out, T f) {
+    const T y = mat_row<T>(M, row, sub, x);
     if (sub == 0 && row < L.n_next) {
         gm(static_cast<T*>(N.b))[row] = y;
         if (x1_out >= 0 && !N.coarsest) {
@@ -1246,7 +1246,17 @@ template <class T>
 __global__ void __launch_bounds__(k_block) k_restrict(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int in,
                                                       int x1_out, T f) {
     CFDG_LEVEL_PROLOGUE(sys, blocks, level, r)
-    op_restrict<T>(L, S.lv[level + 1], M, row, sub, in, x1_out, f);
+    op_restrict<T>(L, S.lv[level + 1], M, row, sub, static_cast<const T*>(L.v[in]), x1_out, f);
+}
+
+// Down leg of a level with one Jacobi sweep from zero as one product: b_{l+1} = R (I - A W) b_l,
+// W = diag(wd) (x1 = W b, r = b - A x1, b_{l+1} = R r in exact arithmetic); rows of level l + 1;
+// x1_out as k_restrict.
+template <class T>
+__global__ void __launch_bounds__(k_block) k_restrict_fused(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level,
+                                                            int x1_out, T f) {
+    CFDG_LEVEL_PROLOGUE(sys, blocks, level, rt)
+    op_restrict<T>(L, S.lv[level + 1], M, row, sub, static_cast<const T*>(L.b), x1_out, f);
 }
 
 // v_l[io] += P_l v_{l+1}[coarse_buf]. X1: the level's single pre-sweep from zero was not

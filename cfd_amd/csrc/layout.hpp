@@ -86,6 +86,7 @@ struct LevelDev {
     EllView a;        // n x n
     EllView p;        // n x n_next
     EllView r;        // n_next x n
+    EllView rt;       // n_next x n: R (I - A diag(wd)), the down leg from x = 0 with one sweep
     int n;
     int n_next;
     int coarsest;     // 1 on the coarsest level

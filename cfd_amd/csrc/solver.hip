@@ -19,6 +19,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <type_traits>
 
 namespace cfdg {
 
@@ -453,7 +454,7 @@ void require_device() {
 }
 
 std::size_t DevLevel::bytes() const {
-    std::size_t b = a.bytes() + p.bytes() + r.bytes() + wd64.bytes() + wd32.bytes() + b64.bytes() +
+    std::size_t b = a.bytes() + p.bytes() + r.bytes() + rt.bytes() + wd64.bytes() + wd32.bytes() + b64.bytes() +
                     b32.bytes() + cinv.bytes();
     for (int i = 0; i < 3; ++i) {
         b += v64[i].bytes() + v32[i].bytes();
@@ -974,6 +975,11 @@ void System::set_amg(const cfdg_amg_view& v) {
     }
     hipStream_t s = stream_.get();
     const double w = v.jacobi_weight;
+    // With one sweep per side in the FP32 cycle (the smoother applying to this hierarchy), the
+    // down leg of levels 1 .. L-2 is the fused restriction R (I - A diag(wd32)) (CFDG_FUSED_RESTRICTION=0:
+    // residual and restriction passes instead).
+    const char* fenv = std::getenv("CFDG_FUSED_RESTRICTION");
+    const bool fuse = mixed_sweeps() == 1 && !(fenv != nullptr && std::string(fenv) == "0");
     std::vector<DevLevel> lv(static_cast<std::size_t>(L));
     for (int l = 0; l < L; ++l) {
         DevLevel& d = lv[static_cast<std::size_t>(l)];
@@ -997,6 +1003,10 @@ void System::set_amg(const cfdg_amg_view& v) {
                 wd32[i] = static_cast<float>(w32 * dinv[i]);
             }
             d.wd32.upload(wd32.data(), wd32.size(), s);
+            if (fuse) {
+                const Csr rt = fused_restriction(v.levels[l].r, a, wd32.data());
+                d.rt.upload_csr(d.n_next, d.n, rt.rp.data(), rt.ci.data(), rt.va.data(), s);
+            }
         }
         if (!coarsest) {
             const cfdg_csr& p = v.levels[l].p;
@@ -1122,6 +1132,9 @@ SysDev System::descriptor(bool fp32, bool use_amg, bool bf16) {
         if (!coarsest) {
             o.p = fp32 ? s.p.view32() : s.p.view64();
             o.r = fp32 ? s.r.view32() : s.r.view64();
+            if (fp32 && s.rt.n_rows > 0) {
+                o.rt = s.rt.view32();
+            }
         }
         for (int i = 0; i < 3; ++i) {
             o.v[i] = fp32 ? static_cast<void*>(s.v32[i].get()) : static_cast<void*>(s.v64[i].get());
@@ -1170,12 +1183,14 @@ struct Tables {
     std::vector<int> cn;
     TableRef t0_single;            // level-0 blocks of one-level hierarchies
     int t0_single_n = 0;
-    // per level and matrix (0 operator, 1 prolongation, 2 restriction): blocks of 256 / tpr
-    // rows of that matrix for the systems whose level l is not the coarsest
-    std::vector<TableRef> tm[3];
-    std::vector<int> tmn[3];
+    // per level and matrix (0 operator, 1 prolongation, 2 restriction, 3 fused restriction):
+    // blocks of 256 / tpr rows of that matrix for the systems whose level l is not the coarsest
+    // (3: those of them that have the fused restriction at level l)
+    std::vector<TableRef> tm[4];
+    std::vector<int> tmn[4];
     std::vector<char> a_ell; // every system's level-l operator is sliced ELL (fused presmooth)
     std::vector<char> x1;    // the level's residual forms x1 = wd b per gather (short rows or small)
+    std::vector<char> fused; // every system has the fused restriction at level l
     DevBuf<int2> arena;      // every table above, back to back
     PinnedBuf<int2> stage;   // host image of the arena for asynchronous uploads
 };
@@ -1203,7 +1218,7 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
     // Host image: ids (two per int2), then the tables; offsets recorded per table.
     std::vector<int2> img((ids.size() + 1) / 2, make_int2(0, 0));
     std::memcpy(img.data(), ids.data(), ids.size() * sizeof(int));
-    std::vector<std::size_t> off_t(L), off_c(L), off_m[3];
+    std::vector<std::size_t> off_t(L), off_c(L), off_m[4];
     tb.tn.assign(L, 0);
     tb.cn.assign(L, 0);
     std::vector<int2> single;
@@ -1245,14 +1260,25 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
     img.insert(img.end(), single.begin(), single.end());
     tb.a_ell.assign(L, 1);
     tb.x1.assign(L, 1);
-    for (int which = 0; which < 3; ++which) {
+    tb.fused.assign(L, 0);
+    for (int which = 0; which < 4; ++which) {
         tb.tmn[which].assign(L, 0);
         off_m[which].assign(L, img.size());
     }
     for (int l = 0; use_amg && l + 1 < tb.l_max; ++l) {
         const std::size_t lu = static_cast<std::size_t>(l);
-        for (int which = 0; which < 3; ++which) {
+        tb.fused[lu] = l >= 1 ? 1 : 0;
+        for (int q = 0; q < tb.n_sys; ++q) {
+            System* x = sys[static_cast<std::size_t>(q)];
+            if (l + 1 < x->n_levels() && !x->has_fused_restriction(l)) {
+                tb.fused[lu] = 0;
+            }
+        }
+        for (int which = 0; which < 4; ++which) {
             off_m[which][lu] = img.size();
+            if (which == 3 && tb.fused[lu] == 0) {
+                continue;
+            }
             for (int q = 0; q < tb.n_sys; ++q) {
                 System* x = sys[static_cast<std::size_t>(q)];
                 if (l + 1 >= x->n_levels()) {
@@ -1269,7 +1295,7 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
                 if (which == 0 && tpr != 1 && static_cast<long long>(x->level_rows(l)) * tpr >= (1LL << 19)) {
                     tb.x1[lu] = 0;
                 }
-                const int rows = which == 2 ? x->level_rows(l + 1) : x->level_rows(l);
+                const int rows = which >= 2 ? x->level_rows(l + 1) : x->level_rows(l);
                 const int nb = ceil_div(static_cast<long long>(rows) * tpr, k_block);
                 for (int b = 0; b < nb; ++b) {
                     img.push_back(make_int2(ids[static_cast<std::size_t>(q)], b));
@@ -1307,7 +1333,7 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
         tb.c[l].p = base + off_c[l];
     }
     tb.t0_single.p = base + off_single;
-    for (int which = 0; which < 3; ++which) {
+    for (int which = 0; which < 4; ++which) {
         tb.tm[which].assign(L, TableRef{});
         for (std::size_t l = 0; l < L; ++l) {
             tb.tm[which][l].p = base + off_m[which][l];
@@ -1417,10 +1443,13 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
     // Where the first sweep from zero of level l is stored as v[x1_slot(l)] = wd b (-1: not
     // stored, it is formed per gathered entry); for l >= 1 the restriction that produces b
     // stores it.
+    // The fused restriction (one sweep per side, FP32 cycle) replaces a level's residual and
+    // restriction passes; nothing reads the level's stored first sweep then.
+    auto fused = [&](int l) { return std::is_same_v<T, float> && sweeps == 1 && tb.fused[static_cast<std::size_t>(l)] != 0; };
     auto x1_slot = [&](int l) {
         const std::size_t li = static_cast<std::size_t>(l);
         if (sweeps == 1) {
-            return tb.x1[li] != 0 ? -1 : 0;
+            return tb.x1[li] != 0 || fused(l) ? -1 : 0;
         }
         if (sweeps >= 2) {
             return tb.a_ell[li] != 0 ? -1 : 2;
@@ -1441,6 +1470,11 @@ void enqueue_vcycle(const SysDev* d, const Tables& tb, int sweeps, hipStream_t s
             // x1 = wd b is formed inside the residual (or stored, for many long rows) and
             // inside the prolongation; bitwise the same as a stored sweep
             cur[li] = 0;
+            if (fused(l)) {
+                CFDG_LAUNCH(k_restrict_fused<T>, tb.tmn[3][li], s, d, tb.tm[3][li].get(), l, x1_next, f1);
+                CFDG_LAUNCH(k_coarse<T>, tb.cn[li + 1], s, d, tb.c[li + 1].get(), l + 1, fin);
+                continue;
+            }
             if (tb.x1[li] != 0) {
                 CFDG_LAUNCH(k_residual_x1<T>, ga, s, d, ba, l, 1, f1);
             } else {

@@ -17,6 +17,7 @@ struct DevLevel {
     DevEll a;
     DevEll p;
     DevEll r;
+    DevEll rt; // fused restriction R (I - A diag(wd32)) (levels >= 1 of the FP32 cycle), if formed
     DevBuf<double> wd64;
     DevBuf<float> wd32;
     DevBuf<double> v64[3];
@@ -105,7 +106,11 @@ public:
         if (which == 0) {
             return l == 0 ? 1 : d.a.tpr;
         }
-        return which == 1 ? d.p.tpr : d.r.tpr;
+        return which == 1 ? d.p.tpr : (which == 2 ? d.r.tpr : d.rt.tpr);
+    }
+    // Whether level l has the fused restriction of the one-sweep FP32 cycle.
+    [[nodiscard]] bool has_fused_restriction(int l) const {
+        return levels_[static_cast<std::size_t>(l)].rt.n_rows > 0;
     }
 
     // Descriptor of this system for kernels of the given precision (FP64 or FP32 V-cycle).
