@@ -566,13 +566,11 @@ void download(const DCsr& d, Csr& a, hipStream_t s) {
     a.rp.resize(static_cast<std::size_t>(d.n_rows) + 1);
     a.ci.resize(static_cast<std::size_t>(d.nnz));
     a.va.resize(static_cast<std::size_t>(d.nnz));
-    hip_check(hipMemcpyAsync(a.rp.data(), d.rp.get(), a.rp.size() * sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+    d2h_staged(a.rp.data(), d.rp.get(), a.rp.size() * sizeof(int), s);
     if (d.nnz > 0) {
-        hip_check(hipMemcpyAsync(a.ci.data(), d.ci.get(), a.ci.size() * sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
-        hip_check(hipMemcpyAsync(a.va.data(), d.va.get(), a.va.size() * sizeof(double), hipMemcpyDeviceToHost, s),
-                  "D2H");
+        d2h_staged(a.ci.data(), d.ci.get(), a.ci.size() * sizeof(int), s);
+        d2h_staged(a.va.data(), d.va.get(), a.va.size() * sizeof(double), s);
     }
-    hip_check(hipStreamSynchronize(s), "D2H sync");
 }
 
 long long last_of(const long long* dev_off, const long long* dev_cnt, int n, hipStream_t s) {
@@ -794,7 +792,9 @@ void DeviceSetup::level(const std::vector<double>& inv_diag, const std::vector<i
     dinv.upload(inv_diag.data(), inv_diag.size(), s);
     // S = D^-1 A P_tent, then P = P_tent - omega S, pruned (amg.cpp:169-172)
     DCsr sm;
+    setup_phase("device: tentative + uploads", -1);
     spgemm(a, pt, dinv.get(), false, sm, s);
+    setup_phase("device: D^-1 A P_tent", -1);
     DCsr p;
     {
         DevBuf<long long> cnt(static_cast<std::size_t>(n));
@@ -811,6 +811,7 @@ void DeviceSetup::level(const std::vector<double>& inv_diag, const std::vector<i
         hip_check(hipGetLastError(), "prolongation launch");
         entries_to_csr(key, val, static_cast<int>(total), n, n_agg, false, p, s);
     }
+    setup_phase("device: P", -1);
     // R = P^T, rows listing the fine rows in ascending order (amg.cpp:175)
     DCsr r;
     {
@@ -826,14 +827,18 @@ void DeviceSetup::level(const std::vector<double>& inv_diag, const std::vector<i
         sort_pairs(key, val, p.nnz, 32 + bits_for(n_agg), s);
         entries_to_csr(key, val, p.nnz, n_agg, n, false, r, s);
     }
+    setup_phase("device: R = P^T", -1);
     // A_c = R (A P), pruned (amg.cpp:176-178)
     DCsr ap;
     spgemm(a, p, nullptr, false, ap, s);
+    setup_phase("device: A P", -1);
     auto coarse = std::make_unique<DCsr>();
     spgemm(r, ap, nullptr, true, *coarse, s);
+    setup_phase("device: R (A P)", -1);
     download(p, p_out, s);
     download(r, r_out, s);
     download(*coarse, coarse_out, s);
+    setup_phase("device: downloads", -1);
     cur_ = std::move(coarse);
 }
 

@@ -16,6 +16,7 @@
 #include <string>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
 #include <memory>
 
 namespace cfdg {
@@ -30,6 +31,17 @@ struct Level {
 };
 
 } // namespace
+
+void setup_phase(const char* what, int level) {
+    static const bool on = std::getenv("CFDG_SETUP_TIMES") != nullptr;
+    thread_local std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+    const auto now = std::chrono::steady_clock::now();
+    if (on) {
+        std::fprintf(stderr, "setup %s level %d: %.2f ms\n", what, level,
+                     std::chrono::duration<double, std::milli>(now - last).count());
+    }
+    last = now;
+}
 
 struct AmgHierarchy {
     cfdg_amg_settings settings{};
@@ -348,7 +360,7 @@ Csr transpose(const Csr& a) { // R = P^T with sorted columns (amg.cpp:175)
 
 void prune_zeros(Csr& a) { // prune(0.0): drops entries with |v| <= 0 (amg.cpp:172, 178)
     int out = 0;
-    std::vector<int> rp(static_cast<std::size_t>(a.n_rows) + 1, 0);
+    RawVec<int> rp(static_cast<std::size_t>(a.n_rows) + 1, 0);
     for (int i = 0; i < a.n_rows; ++i) {
         for (int k = a.rp[static_cast<std::size_t>(i)]; k < a.rp[static_cast<std::size_t>(i) + 1]; ++k) {
             const double v = a.va[static_cast<std::size_t>(k)];
@@ -543,11 +555,14 @@ void build(AmgHierarchy& h, Csr a0, bool device = false) {
     if (device) {
         dev = std::make_unique<DeviceSetup>(a0);
     }
-    Csr current = std::move(a0);
+    Csr next = std::move(a0);
+    setup_phase("start (device setup object)", 0);
     for (int l = 0; l < s.max_levels; ++l) {
         Level level;
-        level.a = current;
+        level.a = std::move(next); // the level keeps its operator; no copy
+        const Csr& current = level.a;
         const std::vector<double> d = diagonal(current);
+        setup_phase("diagonal", l);
         std::vector<double> inv_diag(d.size());
         for (std::size_t i = 0; i < d.size(); ++i) {
             inv_diag[i] = d[i] != 0.0 ? 1.0 / d[i] : 0.0; // amg.cpp:135
@@ -566,11 +581,14 @@ void build(AmgHierarchy& h, Csr a0, bool device = false) {
         } else {
             agg = aggregate(current, d, theta, n_agg);
         }
+        setup_phase("aggregation", l);
         if (n_agg >= current.n_rows || n_agg == 0) { // amg.cpp:145-148
             h.levels.push_back(std::move(level));
             break;
         }
-        double rho = 0.0; // Gershgorin bound (amg.cpp:160-168)
+        double rho = 0.0; // Gershgorin bound (amg.cpp:160-168); rows summed in column order, the
+                          // maximum taken in any order (exact)
+#pragma omp parallel for schedule(static) reduction(max : rho) if (current.n_rows > 20000)
         for (int i = 0; i < current.n_rows; ++i) {
             double row = 0.0;
             for (int k = current.rp[static_cast<std::size_t>(i)]; k < current.rp[static_cast<std::size_t>(i) + 1]; ++k) {
@@ -579,6 +597,7 @@ void build(AmgHierarchy& h, Csr a0, bool device = false) {
             rho = std::max(rho, row * std::abs(inv_diag[static_cast<std::size_t>(i)]));
         }
         const double omega = rho > 0.0 ? 4.0 / (3.0 * rho) : 0.0;
+        setup_phase("rho", l);
         Csr coarse;
         if (dev) {
             dev->level(inv_diag, agg, n_agg, omega, level.p, level.r, coarse);
@@ -590,11 +609,13 @@ void build(AmgHierarchy& h, Csr a0, bool device = false) {
             coarse = spgemm(level.r, ap);
             prune_zeros(coarse);
         }
+        setup_phase("products", l);
         level.agg = std::move(agg);
         h.levels.push_back(std::move(level));
-        current = std::move(coarse);
+        next = std::move(coarse);
     }
     h.coarse_inverse = coarse_inverse(h.levels.back().a, device, h.coarse_on_device);
+    setup_phase("coarse inverse", static_cast<int>(h.levels.size()) - 1);
 }
 
 } // namespace
@@ -609,20 +630,33 @@ Csr copy_csr(const cfdg_csr& v, const char* what) {
     }
     require(v.row_ptr != nullptr && (v.nnz == 0 || (v.col != nullptr && v.val != nullptr)), what);
     require(v.row_ptr[0] == 0 && v.row_ptr[v.n_rows] == v.nnz, what);
-    c.rp.assign(v.row_ptr, v.row_ptr + v.n_rows + 1);
-    c.ci.assign(v.col, v.col + v.nnz);
-    c.va.assign(v.val, v.val + v.nnz);
+    c.rp.resize(static_cast<std::size_t>(v.n_rows) + 1);
+    c.ci.resize(static_cast<std::size_t>(v.nnz));
+    c.va.resize(static_cast<std::size_t>(v.nnz));
+    parallel_copy(c.rp.data(), v.row_ptr, c.rp.size() * sizeof(int));
+    parallel_copy(c.ci.data(), v.col, c.ci.size() * sizeof(int));
+    parallel_copy(c.va.data(), v.val, c.va.size() * sizeof(double));
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad) if (c.n_rows > 20000)
     for (int i = 0; i < c.n_rows; ++i) {
-        require(c.rp[static_cast<std::size_t>(i) + 1] >= c.rp[static_cast<std::size_t>(i)], what);
+        bad |= c.rp[static_cast<std::size_t>(i) + 1] < c.rp[static_cast<std::size_t>(i)] ? 1 : 0;
     }
-    for (int k = 0; k < c.nnz(); ++k) {
-        require(c.ci[static_cast<std::size_t>(k)] >= 0 && c.ci[static_cast<std::size_t>(k)] < c.n_cols, what);
+    const int nnz = c.nnz();
+    const int nc = c.n_cols;
+#pragma omp parallel for schedule(static) reduction(| : bad) if (nnz > 100000)
+    for (int k = 0; k < nnz; ++k) {
+        const int j = c.ci[static_cast<std::size_t>(k)];
+        bad |= (j < 0 || j >= nc) ? 1 : 0;
     }
+    require(bad == 0, what);
     return c;
 }
 
-// Sorts the columns of every row (values follow); rejects duplicate columns.
+// Sorts the columns of every row (values follow); rejects duplicate columns. Rows are
+// independent (host threads over rows).
 void sort_rows(Csr& a) {
+    int dup = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(| : dup) if (a.n_rows > 20000)
     for (int i = 0; i < a.n_rows; ++i) {
         const int b = a.rp[static_cast<std::size_t>(i)];
         const int e = a.rp[static_cast<std::size_t>(i) + 1];
@@ -646,10 +680,10 @@ void sort_rows(Csr& a) {
             a.va[static_cast<std::size_t>(k)] = row[static_cast<std::size_t>(k - b)].second;
         }
         for (int k = b + 1; k < e; ++k) {
-            require(a.ci[static_cast<std::size_t>(k)] != a.ci[static_cast<std::size_t>(k) - 1],
-                    "amg setup: duplicate column in a row");
+            dup |= a.ci[static_cast<std::size_t>(k)] == a.ci[static_cast<std::size_t>(k) - 1] ? 1 : 0;
         }
     }
+    require(dup == 0, "amg setup: duplicate column in a row");
 }
 
 } // namespace cfdg

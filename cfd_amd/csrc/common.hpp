@@ -4,6 +4,8 @@
 #include <cfdgpu.h>
 
 #include <cstdint>
+#include <memory>
+#include <utility>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -69,18 +71,49 @@ inline void require(bool ok, const char* what) {
     }
 }
 
+// Allocator whose resize leaves new elements default-initialised (indeterminate for int and
+// double): large CSR arrays are written in full right after they are sized, and value-initialising
+// them first would cost a single-threaded pass over the memory (and its page faults).
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <class U, class... Args>
+    void construct(U* p, Args&&... args) {
+        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+    }
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+};
+template <class T>
+using RawVec = std::vector<T, NoInitAlloc<T>>;
+
 // Owned CSR matrix (row-major, sorted columns).
 struct Csr {
     int n_rows = 0;
     int n_cols = 0;
-    std::vector<int> rp{0};
-    std::vector<int> ci;
-    std::vector<double> va;
+    RawVec<int> rp{0};
+    RawVec<int> ci;
+    RawVec<double> va;
     [[nodiscard]] int nnz() const { return static_cast<int>(ci.size()); }
 };
 
 // Validates a borrowed CSR view and copies it.
 Csr copy_csr(const cfdg_csr& v, const char* what);
+
+// memcpy of a large block split over the host threads (staging.cpp).
+void parallel_copy(void* dst, const void* src, std::size_t bytes);
+
+// With CFDG_SETUP_TIMES set, prints "setup <what> level <l>: <ms>" to stderr: the host time since
+// the previous call on this thread (the setup's phase breakdown).
+void setup_phase(const char* what, int level);
 
 // R (I - A diag(wd)): the restriction of the residual that one damped-Jacobi sweep from zero
 // leaves (x1 = diag(wd) b), as one matrix (restriction.cpp).

@@ -168,12 +168,10 @@ __device__ __forceinline__ void add_grad_slot(const GradSlot<Lsq, NC>& sl, const
 // ostride 1 and qstride n_cells: component-major). Least squares: A (sum_f d_f dphi_f), A the cell's
 // column-major inverse; Green-Gauss: sum_f S_f phi_f / V with phi_f = w phi_P + (1 - w) phi_N
 // (fv_cell.inc:25-73). Slots are loaded two at a time ahead of their terms, which are added in
-// slot (ascending face) order; padding slots add nothing. With cm != nullptr every derivative is
-// also written component-major, derivative q of cell c at cm[q * n_cells + c] (the face passes
-// gather it from consecutive cells).
+// slot (ascending face) order; padding slots add nothing.
 template <bool Lsq, int NC>
 __device__ __forceinline__ void cell_gradient(const FvSlots& m, int c, const double* phi, const double* bv, int stride,
-                                              double* out, int ostride, int qstride, double* cm = nullptr) {
+                                              double* out, int ostride, int qstride) {
     const auto fp = g(phi);
     double g0[NC];
     double g1[NC];
@@ -207,33 +205,17 @@ __device__ __forceinline__ void cell_gradient(const FvSlots& m, int c, const dou
                      a8 = a[8];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            const double d0 = a0 * g0[i] + a3 * g1[i] + a6 * g2[i];
-            const double d1 = a1 * g0[i] + a4 * g1[i] + a7 * g2[i];
-            const double d2 = a2 * g0[i] + a5 * g1[i] + a8 * g2[i];
-            out_c[static_cast<long long>(3 * i) * qstride] = d0;
-            out_c[static_cast<long long>(3 * i + 1) * qstride] = d1;
-            out_c[static_cast<long long>(3 * i + 2) * qstride] = d2;
-            if (cm != nullptr) {
-                cm[static_cast<long long>(3 * i) * m.nc + c] = d0;
-                cm[static_cast<long long>(3 * i + 1) * m.nc + c] = d1;
-                cm[static_cast<long long>(3 * i + 2) * m.nc + c] = d2;
-            }
+            out_c[static_cast<long long>(3 * i) * qstride] = a0 * g0[i] + a3 * g1[i] + a6 * g2[i];
+            out_c[static_cast<long long>(3 * i + 1) * qstride] = a1 * g0[i] + a4 * g1[i] + a7 * g2[i];
+            out_c[static_cast<long long>(3 * i + 2) * qstride] = a2 * g0[i] + a5 * g1[i] + a8 * g2[i];
         }
     } else {
         const double v = g(m.vol)[c];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            const double d0 = g0[i] / v;
-            const double d1 = g1[i] / v;
-            const double d2 = g2[i] / v;
-            out_c[static_cast<long long>(3 * i) * qstride] = d0;
-            out_c[static_cast<long long>(3 * i + 1) * qstride] = d1;
-            out_c[static_cast<long long>(3 * i + 2) * qstride] = d2;
-            if (cm != nullptr) {
-                cm[static_cast<long long>(3 * i) * m.nc + c] = d0;
-                cm[static_cast<long long>(3 * i + 1) * m.nc + c] = d1;
-                cm[static_cast<long long>(3 * i + 2) * m.nc + c] = d2;
-            }
+            out_c[static_cast<long long>(3 * i) * qstride] = g0[i] / v;
+            out_c[static_cast<long long>(3 * i + 1) * qstride] = g1[i] / v;
+            out_c[static_cast<long long>(3 * i + 2) * qstride] = g2[i] / v;
         }
     }
 }
@@ -243,11 +225,10 @@ __device__ __forceinline__ void cell_gradient(const FvSlots& m, int c, const dou
 // 3 NC derivatives are contiguous (the component stride is the constant 1).
 template <bool Lsq, int NC, bool CompMajor = false>
 __global__ void __launch_bounds__(k_fv_block) k_fv_gradient(FvSlots m, const double* phi, const double* bv, int stride,
-                                                            double* out, int ostride, int qstride,
-                                                            double* cm = nullptr) {
+                                                            double* out, int ostride, int qstride) {
     const int c = fv_cell();
     if (c < m.nc) {
-        cell_gradient<Lsq, NC>(m, c, phi, bv, stride, out, ostride, CompMajor ? qstride : 1, cm);
+        cell_gradient<Lsq, NC>(m, c, phi, bv, stride, out, ostride, CompMajor ? qstride : 1);
     }
 }
 
@@ -383,7 +364,7 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_scalar_face(FvFaces fa, const
 // [+ nu_f g_f S_f], g_f the interpolated 3x3 gradient; component i goes to e_out[i * nif + f] and
 // the cells add it with k_fv_face_sum.
 __global__ void __launch_bounds__(k_fv_block) k_fv_momentum_face(FvFaces fa, const double* __restrict__ cc,
-                                                                 const double* __restrict__ gcm, int nc,
+                                                                 const double* __restrict__ grad,
                                                                  const double* __restrict__ lim, int limited,
                                                                  const double* __restrict__ flux,
                                                                  const double* __restrict__ nu_face,
@@ -394,8 +375,7 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_momentum_face(FvFaces fa, con
     if (f >= fa.nif) {
         return;
     }
-    const auto gg = g(gcm); // derivative q of cell c at gg[q * nc + c]
-    const long long ncl = nc;
+    const auto gg = g(grad);
     const auto fu = g(u);
     const auto cg = g(cc);
     const int p = g(fa.own)[f];
@@ -408,10 +388,10 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_momentum_face(FvFaces fa, con
         const double r0 = g(fa.cf)[3LL * f] - cg[3 * up];
         const double r1 = g(fa.cf)[3LL * f + 1] - cg[3 * up + 1];
         const double r2 = g(fa.cf)[3LL * f + 2] - cg[3 * up + 2];
-        const auto gu = gg + up;
+        const auto gu = gg + 9LL * up;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            double d = gu[(3 * i) * ncl] * r0 + gu[(3 * i + 1) * ncl] * r1 + gu[(3 * i + 2) * ncl] * r2;
+            double d = gu[3 * i] * r0 + gu[3 * i + 1] * r1 + gu[3 * i + 2] * r2;
             if (central > 0.0) {
                 const double uf = wf * fu[3 * p + i] + (1.0 - wf) * fu[3 * n + i];
                 d = (1.0 - central) * d + central * (uf - fu[3 * up + i]);
@@ -425,7 +405,7 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_momentum_face(FvFaces fa, con
     double gf[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
-        gf[j] = wf * gg[j * ncl + p] + (1.0 - wf) * gg[j * ncl + n];
+        gf[j] = wf * gg[9LL * p + j] + (1.0 - wf) * gg[9LL * n + j];
     }
     const double nu = g(nu_face)[f];
     const double k0 = g(fa.kv)[3LL * f];
@@ -585,8 +565,7 @@ __device__ __forceinline__ void add_gauss_slot(const GaussSlot<NC>& sl, double* 
 template <int NC>
 __global__ void __launch_bounds__(k_fv_block) k_fv_gauss_from_faces(FvSlots m, const double* __restrict__ pf_f,
                                                                     const double* __restrict__ bv, int stride,
-                                                                    double* __restrict__ out, int ostride,
-                                                                    double* __restrict__ cm = nullptr) {
+                                                                    double* __restrict__ out, int ostride) {
     const int c = fv_cell();
     if (c >= m.nc) {
         return;
@@ -619,17 +598,9 @@ __global__ void __launch_bounds__(k_fv_block) k_fv_gauss_from_faces(FvSlots m, c
     const double v = g(m.vol)[c];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
-        const double d0 = g0[i] / v;
-        const double d1 = g1[i] / v;
-        const double d2 = g2[i] / v;
-        out_c[3 * i] = d0;
-        out_c[3 * i + 1] = d1;
-        out_c[3 * i + 2] = d2;
-        if (cm != nullptr) {
-            cm[static_cast<long long>(3 * i) * m.nc + c] = d0;
-            cm[static_cast<long long>(3 * i + 1) * m.nc + c] = d1;
-            cm[static_cast<long long>(3 * i + 2) * m.nc + c] = d2;
-        }
+        out_c[3 * i] = g0[i] / v;
+        out_c[3 * i + 1] = g1[i] / v;
+        out_c[3 * i + 2] = g2[i] / v;
     }
 }
 
@@ -1062,7 +1033,6 @@ public:
         vphi_.resize(3 * c);
         vbv_.resize(3 * nb);
         vgrad_.resize(9 * c);
-        vgrad_cm_.resize(9 * c);
         lim_.resize(3 * c);
         flux_.resize(at_least_one(nif_));
         coef_.resize(at_least_one(nif_));
@@ -1090,7 +1060,7 @@ public:
         begin();
         put(vphi_, u, 3LL * nc_);
         put(vbv_, ub, 3LL * nb());
-        gradient_loop<3>(vphi_.get(), vbv_.get(), scheme, vgrad_.get(), 9, vgrad_cm_.get());
+        gradient_loop<3>(vphi_.get(), vbv_.get(), scheme, vgrad_.get(), 9);
         get(vgrad_, grad, 9LL * nc_);
         have_vector_ = true;
     }
@@ -1152,7 +1122,7 @@ public:
         if (nif_ > 0) { // no internal face: nothing to add
             launch([&](hipStream_t s) {
                 k_fv_momentum_face<<<face_blocks(nif_), k_fv_block, 0, s>>>(
-                    faces(), cc_.get(), vgrad_cm_.get(), nc_, lim_.get(), use_lim ? 1 : 0, flux_.get(), coef_.get(),
+                    faces(), cc_.get(), vgrad_.get(), lim_.get(), use_lim ? 1 : 0, flux_.get(), coef_.get(),
                     linear_upwind ? 1 : 0, central, vphi_.get(), transpose_term ? 1 : 0, eface_.get());
             });
         }
@@ -1323,8 +1293,7 @@ private:
     // or least squares into tmp (component-major, so that the face pass gathers each derivative
     // of the two cells coalesced) followed by the corrected Gauss gradient.
     template <int NC>
-    void gradient_loop(const double* phi, const double* bv, int scheme, double* out, int ostride,
-                       double* cm = nullptr) {
+    void gradient_loop(const double* phi, const double* bv, int scheme, double* out, int ostride) {
         const FvSlots m = mesh();
         if (scheme == 2) {
             double* tmp = tmp_.get();
@@ -1342,17 +1311,17 @@ private:
                 });
             }
             launch([&](hipStream_t s) {
-                k_fv_gauss_from_faces<NC><<<blocks(), k_fv_block, 0, s>>>(m, eface_.get(), bv, NC, out, ostride, cm);
+                k_fv_gauss_from_faces<NC><<<blocks(), k_fv_block, 0, s>>>(m, eface_.get(), bv, NC, out, ostride);
             });
             return;
         }
         if (scheme == 1) {
             launch([&](hipStream_t s) {
-                k_fv_gradient<true, NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, out, ostride, 1, cm);
+                k_fv_gradient<true, NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, out, ostride, 1);
             });
         } else {
             launch([&](hipStream_t s) {
-                k_fv_gradient<false, NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, out, ostride, 1, cm);
+                k_fv_gradient<false, NC><<<blocks(), k_fv_block, 0, s>>>(m, phi, bv, NC, out, ostride, 1);
             });
         }
     }
@@ -1406,7 +1375,6 @@ private:
     DevBuf<double> sv_[12]; // sf, lsq_d, face centre, correction: three components each
     DevBuf<double> cc_, vol_, lsq_;
     DevBuf<double> phi_, bv_, sgrad_, vphi_, vbv_, vgrad_, lim_, flux_, coef_, src_, tmp_;
-    DevBuf<double> vgrad_cm_; // the last vector gradient component-major, for the momentum face pass
     // pressure equation: momentum inputs, boundary data, intermediate and output arrays
     DevBuf<double> mupper_, mlower_, mdiag_, msrc_, mu_, dc_, fixed_u_, outlet_p_, uprev_, phiprev_;
     DevBuf<int> kind_;

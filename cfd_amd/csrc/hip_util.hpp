@@ -35,6 +35,12 @@ void pool_free(void* p) noexcept;
 std::size_t pool_release_cached();
 std::size_t pool_reserved_bytes();
 
+// Synchronous copies between pageable host memory and the device through pinned staging
+// (staging.cpp); worth it from a few MiB on.
+void h2d_staged(void* dst, const void* src, std::size_t bytes, hipStream_t s);
+void d2h_staged(void* dst, const void* src, std::size_t bytes, hipStream_t s);
+constexpr std::size_t k_staged_min_bytes = std::size_t(4) << 20;
+
 // Owning device allocation from the pool; the owner must have no device work pending on it
 // when it is released or resized.
 template <class T>
@@ -72,7 +78,9 @@ public:
     }
     void upload(const T* h, std::size_t n, hipStream_t s) {
         resize(n);
-        if (n > 0) {
+        if (n * sizeof(T) >= k_staged_min_bytes) {
+            h2d_staged(p_, h, n * sizeof(T), s);
+        } else if (n > 0) {
             hip_check(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s), "H2D copy");
             hip_check(hipStreamSynchronize(s), "H2D copy sync");
         }

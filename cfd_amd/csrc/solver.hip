@@ -975,11 +975,17 @@ void System::set_amg(const cfdg_amg_view& v) {
     }
     hipStream_t s = stream_.get();
     const double w = v.jacobi_weight;
+    setup_phase("set_amg: checks", 0);
     // With one sweep per side in the FP32 cycle (the smoother applying to this hierarchy), the
-    // down leg of levels 1 .. L-2 is the fused restriction R (I - A diag(wd32)) (CFDG_FUSED_RESTRICTION=0:
-    // residual and restriction passes instead).
-    const char* fenv = std::getenv("CFDG_FUSED_RESTRICTION");
-    const bool fuse = mixed_sweeps() == 1 && !(fenv != nullptr && std::string(fenv) == "0");
+    // down leg of levels 1 .. L-2 with at most fuse_rows rows is the fused restriction
+    // R (I - A diag(wd32)): on those levels a launch costs more than the bytes, while on larger
+    // ones the product moves about as many bytes as the two passes and takes the longest to form.
+    // CFDG_FUSED_RESTRICTION: 0 off, N > 1 the row limit (default 65536).
+    long long fuse_rows = 65536;
+    if (const char* fenv = std::getenv("CFDG_FUSED_RESTRICTION")) {
+        fuse_rows = std::atoll(fenv);
+    }
+    const bool fuse = mixed_sweeps() == 1 && fuse_rows > 0;
     std::vector<DevLevel> lv(static_cast<std::size_t>(L));
     for (int l = 0; l < L; ++l) {
         DevLevel& d = lv[static_cast<std::size_t>(l)];
@@ -997,15 +1003,19 @@ void System::set_amg(const cfdg_amg_view& v) {
             d.wd64.upload(wd.data(), wd.size(), s);
             // Upper end of the spectrum of D^-1 A: the power-iteration estimate (4-8% low on the
             // box hierarchies) with a 15% margin.
+            setup_phase("set_amg: operator upload", l);
             const double w32 = mixed_ratio() == 0.0 ? w : mixed_weight(1.15 * device_lambda(d.a, d.wd64, w, s), w);
+            setup_phase("set_amg: spectrum estimate", l);
             std::vector<float> wd32(dinv.size());
             for (std::size_t i = 0; i < dinv.size(); ++i) {
                 wd32[i] = static_cast<float>(w32 * dinv[i]);
             }
             d.wd32.upload(wd32.data(), wd32.size(), s);
-            if (fuse) {
+            if (fuse && d.n <= fuse_rows) {
                 const Csr rt = fused_restriction(v.levels[l].r, a, wd32.data());
+                setup_phase("set_amg: fused restriction product", l);
                 d.rt.upload_csr(d.n_next, d.n, rt.rp.data(), rt.ci.data(), rt.va.data(), s);
+                setup_phase("set_amg: fused restriction upload", l);
             }
         }
         if (!coarsest) {
@@ -1013,6 +1023,7 @@ void System::set_amg(const cfdg_amg_view& v) {
             const cfdg_csr& r = v.levels[l].r;
             d.p.upload_csr(d.n, d.n_next, p.row_ptr, p.col, p.val, s);
             d.r.upload_csr(d.n_next, d.n, r.row_ptr, r.col, r.val, s);
+            setup_phase("set_amg: transfer upload", l);
         } else {
             d.cinv.upload(v.coarse_inverse, static_cast<std::size_t>(d.n) * static_cast<std::size_t>(d.n), s);
         }
