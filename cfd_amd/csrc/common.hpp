@@ -108,6 +108,12 @@ struct Csr {
 // Validates a borrowed CSR view and copies it.
 Csr copy_csr(const cfdg_csr& v, const char* what);
 
+// Whether this host thread may fan work out over OpenMP threads (staging.cpp). Threads that
+// already run side by side (the server's staging threads) turn it off, so that concurrent calls
+// do not each start a full team.
+bool host_parallel();
+void set_host_parallel(bool on);
+
 // memcpy of a large block split over the host threads (staging.cpp).
 void parallel_copy(void* dst, const void* src, std::size_t bytes);
 

@@ -60,7 +60,7 @@ struct cfdg_amg_cache {
             std::vector<cfdg_amg_level_view> lv(static_cast<std::size_t>(nl));
             cfdg_amg_view v{};
             rethrow(cfdg_amg_view_of(h, lv.data(), &v));
-            sys->set_amg(v);
+            sys->set_amg(v, false); // rebuilt every max_age solves: no fused products
         }
         if (amg != nullptr) {
             (void)cfdg_amg_destroy(amg);

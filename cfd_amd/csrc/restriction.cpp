@@ -22,10 +22,10 @@ Csr fused_restriction(const cfdg_csr& r, const cfdg_csr& a, const float* wd) {
     c.n_rows = n;
     c.n_cols = cols;
     std::vector<int> count(static_cast<std::size_t>(n) + 1, 0);
-#pragma omp parallel
+#pragma omp parallel if (host_parallel())
     {
         std::vector<int> mark(static_cast<std::size_t>(cols), -1);
-#pragma omp for schedule(dynamic, 64)
+#pragma omp for schedule(dynamic, 1)
         for (int j = 0; j < n; ++j) {
             int cnt = 0;
             auto touch = [&](int col) {
@@ -51,11 +51,11 @@ Csr fused_restriction(const cfdg_csr& r, const cfdg_csr& a, const float* wd) {
     const std::size_t nnz = static_cast<std::size_t>(c.rp[static_cast<std::size_t>(n)]);
     c.ci.resize(nnz);
     c.va.resize(nnz);
-#pragma omp parallel
+#pragma omp parallel if (host_parallel())
     {
         std::vector<int> mark(static_cast<std::size_t>(cols), -1);
         std::vector<double> acc(static_cast<std::size_t>(cols), 0.0);
-#pragma omp for schedule(dynamic, 64)
+#pragma omp for schedule(dynamic, 1)
         for (int j = 0; j < n; ++j) {
             int* cj = c.ci.data() + c.rp[static_cast<std::size_t>(j)];
             int cnt = 0;

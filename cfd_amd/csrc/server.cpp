@@ -301,7 +301,7 @@ private:
         sys->assemble(q.diag, q.upper, q.lower);
         const bool bf16 = amg_ && opt_.precision == CFDG_PREC_MIXED_BF16;
         if (amg_) {
-            sys->set_amg(*q.amg);
+            sys->set_amg(*q.amg, false); // a request's few steps do not repay the fused products
             if (bf16) {
                 sys->ensure_bf16();
             }
@@ -392,6 +392,7 @@ private:
     }
 
     void stager_loop() {
+        set_host_parallel(false); // several staging threads run at once
         for (;;) {
             Request* job = nullptr;
             bool finishing = false;

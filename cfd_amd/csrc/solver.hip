@@ -950,7 +950,7 @@ void DevEll::build_on_device(int rows, int cols, const int* rp, const int* ci, c
     hip_check(hipStreamSynchronize(s), "ELL build sync"); // the staging buffers go out of scope
 }
 
-void System::set_amg(const cfdg_amg_view& v) {
+void System::set_amg(const cfdg_amg_view& v, bool fuse_wanted) {
     require(v.n_levels >= 1 && v.levels != nullptr,
             "DeviceSolver::set_amg: empty hierarchy");
     require(v.n_levels <= k_max_levels, "set_amg: too many levels");
@@ -985,7 +985,7 @@ void System::set_amg(const cfdg_amg_view& v) {
     if (const char* fenv = std::getenv("CFDG_FUSED_RESTRICTION")) {
         fuse_rows = std::atoll(fenv);
     }
-    const bool fuse = mixed_sweeps() == 1 && fuse_rows > 0;
+    const bool fuse = fuse_wanted && mixed_sweeps() == 1 && fuse_rows > 0;
     std::vector<DevLevel> lv(static_cast<std::size_t>(L));
     for (int l = 0; l < L; ++l) {
         DevLevel& d = lv[static_cast<std::size_t>(l)];
@@ -2804,7 +2804,7 @@ extern "C" cfdg_status cfdg_solve_amg_cg_prec(const cfdg_amg_view* amg, const do
         require(amg->n_levels >= 1 && amg->levels != nullptr, "solve_amg_cg: empty hierarchy");
         check_control(control);
         System s(amg->levels[0].a);
-        s.set_amg(*amg);
+        s.set_amg(*amg, false); // one solve: the fused products would not pay off
         s.set_rhs(b);
         s.set_x(x);
         *stats = solve_one(s, CFDG_METHOD_CG, true, precision, *control);

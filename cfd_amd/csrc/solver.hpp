@@ -70,7 +70,10 @@ public:
     System(const System&) = delete;
     System& operator=(const System&) = delete;
 
-    void set_amg(const cfdg_amg_view& v);
+    // fuse = false: no fused restrictions are formed (their host product pays off only over
+    // many solves with the hierarchy; the server's requests and the AmgCache, which rebuilds
+    // every max_age solves, pass false).
+    void set_amg(const cfdg_amg_view& v, bool fuse = true);
     void assemble(const double* diag, const double* upper, const double* lower);
     // The same from device arrays (diag, upper, lower of the system's pattern) plus the
     // right-hand side b, copied within HBM.

@@ -20,6 +20,8 @@ struct Area {
     int device = -1;
 };
 
+thread_local bool t_host_parallel = true;
+
 std::mutex g_mutex;
 std::vector<Area*> g_free;
 
@@ -53,7 +55,7 @@ void release(Area* a) {
 void host_copy(char* dst, const char* src, std::size_t n) {
     constexpr std::size_t piece = std::size_t(1) << 20;
     const long long pieces = static_cast<long long>((n + piece - 1) / piece);
-    if (pieces <= 1) {
+    if (pieces <= 1 || !t_host_parallel) {
         std::memcpy(dst, src, n);
         return;
     }
@@ -65,6 +67,10 @@ void host_copy(char* dst, const char* src, std::size_t n) {
 }
 
 } // namespace
+
+bool host_parallel() { return t_host_parallel; }
+
+void set_host_parallel(bool on) { t_host_parallel = on; }
 
 void parallel_copy(void* dst, const void* src, std::size_t bytes) {
     host_copy(static_cast<char*>(dst), static_cast<const char*>(src), bytes);
