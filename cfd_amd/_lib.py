@@ -208,7 +208,7 @@ class AmgSettings(C.Structure):
 
 
 class AmgCacheOptions(C.Structure):
-    _fields_ = [("max_age", C.c_int), ("stale_rebuild", C.c_int), ("amg", AmgSettings)]
+    _fields_ = [("max_age", C.c_int), ("stale_rebuild", C.c_int), ("amg", AmgSettings), ("device_setup", C.c_int)]
 
 
 class LinearSettings(C.Structure):

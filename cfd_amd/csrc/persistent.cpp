@@ -52,7 +52,7 @@ struct cfdg_amg_cache {
         a.col = col.data();
         a.val = val.data();
         cfdg_amg* h = nullptr;
-        rethrow(cfdg_amg_build(&a, &opt.amg, &h));
+        rethrow(opt.device_setup != 0 ? cfdg_amg_build_device(&a, &opt.amg, &h) : cfdg_amg_build(&a, &opt.amg, &h));
         std::unique_ptr<cfdg_amg, cfdg_status (*)(cfdg_amg*)> keep(h, cfdg_amg_destroy);
         int nl = 0;
         rethrow(cfdg_amg_n_levels(h, &nl));
@@ -105,6 +105,7 @@ extern "C" cfdg_status cfdg_amg_cache_default_options(cfdg_amg_cache_options* ou
         *out = cfdg_amg_cache_options{};
         out->max_age = 20;
         out->stale_rebuild = 0;
+        out->device_setup = 1;
         cfdg_amg_cache::rethrow(cfdg_amg_default_settings(&out->amg));
     });
 }

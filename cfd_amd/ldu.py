@@ -47,14 +47,17 @@ class LinearSolveResult:
 class AmgCache:
     """AmgCache (ldu_matrix.hpp:61-75) with the device system it drives: solve() is
     LduMatrix::solve(x, settings, &cache) for GpuCg / GpuBiCgStab with a CSR pattern.
-    stale_rebuild adds the host AmgCg path's stale-hierarchy rule (ldu_matrix.cpp:242-256)."""
+    stale_rebuild adds the host AmgCg path's stale-hierarchy rule (ldu_matrix.cpp:242-256).
+    device_setup: rebuilds with the sparse products and the coarsest inverse on the GPU (the
+    default; the same hierarchy, the coarsest inverse equal to rounding), else on the host."""
 
     def __init__(self, pattern: LduPattern, max_age: int = 20, stale_rebuild: bool = False,
-                 amg: AmgSettings | None = None):
+                 amg: AmgSettings | None = None, device_setup: bool = True):
         o = L.AmgCacheOptions()
         L.call("cfdg_amg_cache_default_options", C.byref(o))
         o.max_age = int(max_age)
         o.stale_rebuild = int(bool(stale_rebuild))
+        o.device_setup = int(bool(device_setup))
         if amg is not None:
             o.amg = amg.c()
         self.pattern = pattern

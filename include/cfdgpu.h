@@ -547,6 +547,9 @@ typedef struct cfdg_amg_cache_options {
     int max_age;                /* rebuild after this many solves (ldu_matrix.hpp:74: 20) */
     int stale_rebuild;          /* 0: solve_persistent (the device path); 1: also the rule above */
     cfdg_amg_settings amg;      /* hierarchy settings (cfdg_amg_default_settings) */
+    int device_setup;           /* 1 (default): rebuilds run the setup's sparse products and the
+                                   coarsest inverse on the GPU (cfdg_amg_build_device: the same
+                                   hierarchy, the inverse equal to rounding); 0: the host setup */
 } cfdg_amg_cache_options;
 
 /* LinearSolverSettings (ldu_matrix.hpp:23-28) for the device solver types. */

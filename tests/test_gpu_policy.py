@@ -5,7 +5,8 @@ solved, whether AMG ran, the hierarchies built, the cache's age, the initial nor
 and the mapped target (1e-12 relative), the iterations (+-1) and the solution (1e-10 relative,
 or within the residual bound where a stale hierarchy amplifies rounding differences);
 and the device normalised residual (cfdg_system_normalised_residual, ldu_matrix.cpp:65-103)
-against the oracle's."""
+against the oracle's. The cache rebuilds with the device setup (its default) unless a test asks
+for the host setup."""
 from __future__ import annotations
 
 import numpy as np
@@ -33,12 +34,12 @@ def _lambda_min(a) -> float:
 
 
 def _run(oracle, mesh, seq, method, tol, rel, max_it, max_age=20, stale=False,
-         precision=Precision.FP64):
+         precision=Precision.FP64, device_setup=True):
     pat = cases.csr_pattern(mesh.n_cells, mesh.owner, mesh.neighbour)
     opat = oracle.csr_pattern(mesh.n_cells, mesh.owner, mesh.neighbour)
     cache_o = oracle.AmgCacheState(max_age=max_age, stale_rebuild=stale)
     outs = []
-    with AmgCache(pat, max_age=max_age, stale_rebuild=stale) as cache:
+    with AmgCache(pat, max_age=max_age, stale_rebuild=stale, device_setup=device_setup) as cache:
         for k, (diag, up, lo, b, x0) in enumerate(seq):
             x = x0.copy()
             r = cache.solve(diag, up, lo, b, x, LinearSolverSettings(method, tol, rel, max_it, precision))
@@ -92,12 +93,14 @@ def test_amg_cache_reuse(oracle):
         assert r.used_amg and np.abs(x - xe).max() < 1e-6
 
 
-def test_rebuild_every_max_age_solves(oracle):
+@pytest.mark.parametrize("device_setup", [True, False])
+def test_rebuild_every_max_age_solves(oracle, device_setup):
+    """Rebuilds at age 20 with the setup on the GPU (the default) and on the host."""
     mesh = cases.box_mesh(20, 20, 2)
     diag0, up, lo = build_poisson_ldu(mesh, True)
     seq = [(diag0 + 0.01 * k, up, lo, np.cos(0.1 * np.arange(mesh.n_cells) + k), np.zeros(mesh.n_cells))
            for k in range(45)]
-    outs = _run(oracle, mesh, seq, Method.Cg, 1e-10, 0.0, 500)
+    outs = _run(oracle, mesh, seq, Method.Cg, 1e-10, 0.0, 500, device_setup=device_setup)
     assert [k for k, (r, _) in enumerate(outs) if r.setups] == [0, 20, 40]
 
 
