@@ -687,6 +687,11 @@ def main(argv=None) -> int:
     view = hier.view()
     dev = DeviceSolver(pat)
     dev.assemble(p_sys.diag, p_sys.upper, p_sys.lower)
+    sizes = hier.level_sizes()
+    # the FP32 cycle's dense tail: the first level below the finest, not the coarsest, of <= 512 rows
+    tail_m = next((l for l in range(1, len(sizes) - 1) if sizes[l] <= 512), 0)
+    if os.environ.get("CFDG_DENSE_TAIL") == "0":
+        tail_m = 0
     t_set = time.perf_counter()
     dev.set_amg(view)  # uploads the levels, estimates the FP32 weights, forms the fused restrictions
     set_amg_ms = 1e3 * (time.perf_counter() - t_set)
@@ -816,7 +821,11 @@ def main(argv=None) -> int:
             "strength_threshold": "0.08 * 0.5^level",
             "vcycle_mixed": f"FP32 V-cycle, {sweeps_m}+{sweeps_m} damped-Jacobi sweeps per level with "
                             f"w_l = 2 / (h_l (1 + 1/{ratio_m:g})) (degree-1 Chebyshev, h_l the top of the "
-                            "D^-1 A spectrum), dense FP64 coarsest inverse, in FP64 flexible CG",
+                            "D^-1 A spectrum); levels 1..L-2 of at most 65536 rows restrict with the "
+                            "fused product R (I - A W); " + (
+                                f"ends at level {tail_m} ({hier.level_sizes()[tail_m]} rows) with its dense "
+                                "FP64 inverse (the FP32 cycle's dense tail)" if tail_m else
+                                "dense FP64 coarsest inverse") + ", in FP64 flexible CG",
             "vcycle_fp64": "FP64 V-cycle, 2+2 damped-Jacobi sweeps at w = 0.7 (the reference), dense "
                            "FP64 coarsest inverse, in standard PCG",
         },

@@ -8,6 +8,7 @@
 #include "solver.hpp"
 #include "fv.hpp"
 #include "kernels.hpp"
+#include "dense.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -1028,11 +1029,42 @@ void System::set_amg(const cfdg_amg_view& v, bool fuse_wanted) {
             d.cinv.upload(v.coarse_inverse, static_cast<std::size_t>(d.n) * static_cast<std::size_t>(d.n), s);
         }
     }
+    // Dense tail of the FP32 cycle: the first level below the finest that is not the coarsest and
+    // has at most kDenseMaxOrder rows is solved exactly with its dense inverse (computed on the
+    // matrix cores) instead of being smoothed and restricted further: the levels under it are
+    // launch-bound, and one dense product replaces their launches (CFDG_DENSE_TAIL=0: off).
+    int tail = 0;
+    const char* tenv = std::getenv("CFDG_DENSE_TAIL");
+    if (!(tenv != nullptr && std::string(tenv) == "0")) {
+        for (int l = 1; l + 1 < L; ++l) {
+            const int nl = rows[static_cast<std::size_t>(l)];
+            if (nl > kDenseMaxOrder) {
+                continue;
+            }
+            const cfdg_csr& a = v.levels[l].a;
+            std::vector<double> dense(static_cast<std::size_t>(nl) * static_cast<std::size_t>(nl), 0.0);
+            for (int i = 0; i < nl; ++i) {
+                for (int k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
+                    dense[static_cast<std::size_t>(i) * static_cast<std::size_t>(nl) + static_cast<std::size_t>(a.col[k])] +=
+                        a.val[k];
+                }
+            }
+            std::vector<double> inv(dense.size());
+            const std::vector<int> ok = dense_inverse_batch({nl}, {dense.data()}, {inv.data()}, s);
+            if (ok[0] != 0) {
+                lv[static_cast<std::size_t>(l)].cinv_tail.upload(inv.data(), inv.size(), s);
+                tail = l;
+            }
+            break;
+        }
+        setup_phase("set_amg: dense tail", tail);
+    }
     // Level 0's operator is the assembled one (it may differ from the hierarchy's level 0, which
     // only serves the bound); diffusion operators have lambda(D^-1 A) <= 2.
     const cfdg_csr& a0 = v.levels[0].a;
     const double lambda0 = (L > 1 && a0.row_ptr != nullptr && a0.n_rows == n_) ? gershgorin(a0) : 2.0;
     levels_ = std::move(lv);
+    tail_ = tail;
     sweeps_ = v.smooth_sweeps;
     weight_ = w;
     weight0_32_ = mixed_weight(lambda0, w);
@@ -1120,7 +1152,7 @@ SysDev System::descriptor(bool fp32, bool use_amg, bool bf16) {
     }
     require(has_amg_, "solve: use_amg requested but no hierarchy was set");
     ensure_level_vectors(fp32);
-    const int L = static_cast<int>(levels_.size());
+    const int L = n_levels(fp32);
     d.n_levels = L;
     for (int l = 0; l < L; ++l) {
         DevLevel& s = levels_[static_cast<std::size_t>(l)];
@@ -1150,7 +1182,7 @@ SysDev System::descriptor(bool fp32, bool use_amg, bool bf16) {
         for (int i = 0; i < 3; ++i) {
             o.v[i] = fp32 ? static_cast<void*>(s.v32[i].get()) : static_cast<void*>(s.v64[i].get());
         }
-        o.cinv = s.cinv.get();
+        o.cinv = (coarsest && l != static_cast<int>(levels_.size()) - 1) ? s.cinv_tail.get() : s.cinv.get();
     }
     return d;
 }
@@ -1210,7 +1242,7 @@ struct Tables {
 // array: its position in `sys`, or slots[position] when slots are given. A synchronous build
 // returns with the tables in device memory; an asynchronous one queues the upload on `s`
 // (the tables' previous contents must no longer be in use by queued work).
-void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hipStream_t s,
+void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, bool fp32, hipStream_t s,
                   const std::vector<int>* slots = nullptr, bool async = false) {
     tb.n_sys = static_cast<int>(sys.size());
     std::vector<int> ids(sys.size());
@@ -1222,7 +1254,7 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
     tb.l_max = 1;
     if (use_amg) {
         for (System* x : sys) {
-            tb.l_max = std::max(tb.l_max, x->n_levels());
+            tb.l_max = std::max(tb.l_max, x->n_levels(fp32));
         }
     }
     const std::size_t L = static_cast<std::size_t>(tb.l_max);
@@ -1240,7 +1272,7 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
         for (int q = 0; q < tb.n_sys; ++q) {
             System* x = sys[static_cast<std::size_t>(q)];
             const int id = ids[static_cast<std::size_t>(q)];
-            const int nl = use_amg ? x->n_levels() : 1;
+            const int nl = use_amg ? x->n_levels(fp32) : 1;
             if (l >= nl) {
                 continue;
             }
@@ -1281,7 +1313,7 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
         tb.fused[lu] = l >= 1 ? 1 : 0;
         for (int q = 0; q < tb.n_sys; ++q) {
             System* x = sys[static_cast<std::size_t>(q)];
-            if (l + 1 < x->n_levels() && !x->has_fused_restriction(l)) {
+            if (l + 1 < x->n_levels(fp32) && !x->has_fused_restriction(l)) {
                 tb.fused[lu] = 0;
             }
         }
@@ -1292,7 +1324,7 @@ void build_tables(Tables& tb, const std::vector<System*>& sys, bool use_amg, hip
             }
             for (int q = 0; q < tb.n_sys; ++q) {
                 System* x = sys[static_cast<std::size_t>(q)];
-                if (l + 1 >= x->n_levels()) {
+                if (l + 1 >= x->n_levels(fp32)) {
                     continue; // level l is this system's coarsest (or absent)
                 }
                 const int tpr = x->tpr(l, which);
@@ -1688,7 +1720,7 @@ public:
         }
         precision_ = amg ? opt.precision : CFDG_PREC_FP64;
         const bool bf16 = amg && opt.precision == CFDG_PREC_MIXED_BF16;
-        build_tables(tb_, sys_, amg, s_);
+        build_tables(tb_, sys_, amg, pl_.fp32, s_);
         std::vector<SysDev> desc(sys_.size());
         for (std::size_t i = 0; i < sys_.size(); ++i) {
             desc[i] = sys_[i]->descriptor(fp32, amg, bf16);
@@ -2019,7 +2051,7 @@ int SlotRunner::start(System* x, const cfdg_solve_control& c) {
     if (m.single_of[ju] != x->uid() || m.single_gen[ju] != x->generation()) {
         std::vector<System*> one{x};
         std::vector<int> ids{j};
-        build_tables(m.single[ju], one, m.pl.amg, m.s, &ids, true);
+        build_tables(m.single[ju], one, m.pl.amg, m.pl.fp32, m.s, &ids, true);
         m.single_of[ju] = x->uid();
         m.single_gen[ju] = x->generation();
     }
@@ -2052,7 +2084,7 @@ void SlotRunner::step(std::vector<Done>& done) {
                 xs.push_back(m.sys[static_cast<std::size_t>(j)]);
             }
             m.cur ^= 1; // the other table set's last users finished before the previous step
-            build_tables(m.merged[m.cur], xs, m.pl.amg, m.s, &m.running, true);
+            build_tables(m.merged[m.cur], xs, m.pl.amg, m.pl.fp32, m.s, &m.running, true);
             m.dirty = false;
         }
         const Tables& tb = m.merged[m.cur];

@@ -25,6 +25,7 @@ struct DevLevel {
     DevBuf<float> v32[3];
     DevBuf<float> b32;
     DevBuf<double> cinv; // coarsest level only
+    DevBuf<double> cinv_tail; // dense inverse of this level's operator where the FP32 cycle ends
     [[nodiscard]] std::size_t bytes() const;
 };
 
@@ -91,7 +92,13 @@ public:
 
     [[nodiscard]] int n() const { return n_; }
     [[nodiscard]] int n_faces() const { return nf_; }
-    [[nodiscard]] int n_levels() const { return has_amg_ ? static_cast<int>(levels_.size()) : 0; }
+    // Levels of the cycle: the hierarchy's, or for the FP32 cycle up to its dense tail level.
+    [[nodiscard]] int n_levels(bool fp32 = false) const {
+        if (!has_amg_) {
+            return 0;
+        }
+        return fp32 && tail_ > 0 ? tail_ + 1 : static_cast<int>(levels_.size());
+    }
     [[nodiscard]] bool has_amg() const { return has_amg_; }
     // V-cycle sweeps per side: the hierarchy's (FP64, the reference algorithm) or the mixed
     // smoother's (FP32 V-cycle).
@@ -153,6 +160,7 @@ private:
     DevBuf<double> norms_; // sum of x and the three residual norms
     std::vector<DevLevel> levels_;
     bool has_amg_ = false;
+    int tail_ = 0; // > 0: the FP32 cycle solves level tail_ exactly (cinv_tail) and stops there
     bool assembled_ = false;
     int sweeps_ = 2;
     double weight_ = 0.7;

@@ -1,0 +1,100 @@
+"""Dense tail of the FP32 V-cycle: the first level below the finest that is not the coarsest and
+has at most 512 rows is solved exactly with its dense inverse (computed on the matrix cores when
+the hierarchy is set) and the FP32 cycle stops there, while the FP64 cycle (the reference's,
+krylov_kernels.inc:209-238) keeps every level. Mixed solves with the tail (the default) and
+without it (CFDG_DENSE_TAIL=0) reach the tolerance and solutions within the tolerance-implied
+bound of the FP64 oracle's; the tail needs no more iterations (+1 for rounding); a batch that
+mixes systems with and without a tail level gives every member the solve it gets alone, bit for
+bit; FP64 solves are unchanged by the switch."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cfd_amd
+from cfd_amd import AmgHierarchy, AmgSettings, Method, Precision, SolveControl, cases
+
+pytestmark = pytest.mark.gpu
+
+# (40, 40, 40) jittered: levels [64000, 7809, 437, 19] (level 2 becomes the tail);
+# (20, 20, 60): [24000, 2942, 170] (no level qualifies: the coarsest is the tail already)
+SHAPES = [((40, 40, 40), 0.2, 1.0), ((20, 20, 60), 0.0, 3.0), ((36, 30, 44), 0.3, 1.0)]
+
+
+def _system(shape, jitter, seed, lz):
+    mesh = cases.box_mesh(*shape, hi=(1.0, 1.0, lz), jitter=jitter, seed=seed)
+    s = cases.pressure_system(mesh, rhs_seed=seed + 1)
+    pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
+    a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
+    return s, pat, a, AmgHierarchy(a, AmgSettings(strength_decay=0.5))
+
+
+def _lambda_min(a) -> float:
+    import scipy.sparse.linalg as sla
+
+    return float(sla.eigsh(a.to_scipy().tocsc(), k=1, sigma=0, which="LM", return_eigenvectors=False)[0])
+
+
+def _solve(systems, prec, batch):
+    devs = []
+    try:
+        for s, pat, _, h in systems:
+            dev = cfd_amd.DeviceSolver(pat)
+            devs.append(dev)
+            dev.assemble(s.diag, s.upper, s.lower)
+            dev.set_amg(h.view())
+            dev.set_rhs(s.source)
+            dev.set_x(None)
+        ctls = [SolveControl(1e-8 * float(np.linalg.norm(t[0].source)), 500) for t in systems]
+        if batch:
+            with cfd_amd.Batch(devs, Method.Cg, True, prec) as b:
+                stats = b.solve(ctls)
+        else:
+            stats = [d.solve_resident(Method.Cg, True, prec, c) for d, c in zip(devs, ctls)]
+        return [(st, d.get_x()) for st, d in zip(stats, devs)]
+    finally:
+        for d in devs:
+            d.close()
+
+
+def test_tail_against_the_full_cycle_and_the_oracle(monkeypatch, oracle):
+    systems = [_system(shape, jitter, 7 + i, lz) for i, (shape, jitter, lz) in enumerate(SHAPES)]
+    sizes = [h.level_sizes() for *_, h in systems]
+    assert any(any(n <= 512 for n in ls[1:-1]) for ls in sizes)  # at least one system has a tail level
+    monkeypatch.delenv("CFDG_DENSE_TAIL", raising=False)
+    tail = _solve(systems, Precision.MIXED_FP32, False)
+    monkeypatch.setenv("CFDG_DENSE_TAIL", "0")
+    full = _solve(systems, Precision.MIXED_FP32, False)
+    for (s, _, a, h), (st, xt), (sf, xf) in zip(systems, tail, full):
+        oa = oracle.Csr(a.n, a.n, a.row_ptr, a.col, a.val)
+        tol = 1e-8 * float(np.linalg.norm(s.source))
+        xo = np.zeros(s.n)
+        so = oracle.Hierarchy(oa, decay=0.5).amg_cg(s.source, xo, tol, 500)
+        res = lambda x: float(np.linalg.norm(s.source - oracle.spmv(oa, x)))
+        lam = _lambda_min(a)
+        print(f"levels {h.level_sizes()}: tail it {st.iterations}, full cycle it {sf.iterations}, "
+              f"oracle FP64 it {so.iterations}; |xt - xo| {np.linalg.norm(xt - xo):.2e} "
+              f"bound {(res(xt) + res(xo)) / (0.9 * lam):.2e}")
+        assert st.converged and sf.converged
+        assert st.iterations <= sf.iterations + 1
+        assert res(xt) <= 1.05 * tol
+        assert np.linalg.norm(xt - xo) <= (res(xt) + res(xo)) / (0.9 * lam)
+
+
+def test_batch_with_and_without_tail_levels_matches_single_solves():
+    systems = [_system(shape, jitter, 17 + i, lz) for i, (shape, jitter, lz) in enumerate(SHAPES)]
+    alone = _solve(systems, Precision.MIXED_FP32, False)
+    together = _solve(systems, Precision.MIXED_FP32, True)
+    for (sa, xa), (sb, xb) in zip(alone, together):
+        assert sa.iterations == sb.iterations and sa.residual == sb.residual
+        assert np.array_equal(xa, xb)
+
+
+def test_fp64_cycle_is_unaffected(monkeypatch):
+    systems = [_system((40, 40, 40), 0.2, 3, 1.0)]
+    monkeypatch.delenv("CFDG_DENSE_TAIL", raising=False)
+    (sa, xa), = _solve(systems, Precision.FP64, False)
+    monkeypatch.setenv("CFDG_DENSE_TAIL", "0")
+    (sb, xb), = _solve(systems, Precision.FP64, False)
+    assert sa.iterations == sb.iterations and sa.residual == sb.residual
+    assert np.array_equal(xa, xb)
