@@ -300,7 +300,7 @@ def roofline(probe, kernel, nnz, traffic=None, working_set=None):
     return out
 
 
-def stress(n_side: int = 160, reps: int = 3) -> dict:
+def stress(n_side: int = 160, reps: int = 5) -> dict:
     """Precision stress: 160^3 jittered box, rho = 1000 inside a seeded sphere of radius 0.2 and
     1 outside (face coefficients with harmonic averaging), solved to 1e-10 relative residual by
     the FP64 AMG-PCG and by the mixed solvers (FP32 V-cycle, optionally BF16 level-0
@@ -324,11 +324,14 @@ def stress(n_side: int = 160, reps: int = 3) -> dict:
     out, x64, xs = {}, None, {}
     for prec in (Precision.FP64, Precision.MIXED_FP32, Precision.MIXED_BF16):
         times = []
-        for _ in range(reps + 1):
+        gc.collect()
+        gc.disable()
+        for _ in range(reps + 2):  # the first two solve eagerly and capture the graphs
             dev.set_x(None)
             t = time.perf_counter()
             st = dev.solve_resident(Method.Cg, True, prec, ctl)
             times.append(time.perf_counter() - t)
+        gc.enable()
         x = dev.get_x()
         if prec == Precision.FP64:
             x64 = x
@@ -337,7 +340,7 @@ def stress(n_side: int = 160, reps: int = 3) -> dict:
             "iterations": st.iterations, "converged": st.converged,
             "true_rel_residual": st.true_residual / bnorm,
             "rel_error_vs_fp64": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
-            "solves_per_s": round(1.0 / float(np.median(times[1:])), 3)}
+            "solves_per_s": round(1.0 / float(np.median(times[2:])), 3)}
     # The mixed solve's two heaviest level-0 kernels at a working set larger than the 256 MB
     # Infinity Cache (the FP64 operator alone is 228 MB of values).
     traffic = _traffic()
@@ -794,8 +797,9 @@ def main(argv=None) -> int:
     mdev.assemble(m_sys.diag, m_sys.upper, m_sys.lower)
     mdev.set_rhs(m_sys.source)
     mtol = 1e-8 * float(np.linalg.norm(m_sys.source))
-    mdev.set_x(None)
-    mdev.solve_resident(Method.BiCgStab, False, Precision.FP64, SolveControl(mtol, 2000))
+    for _ in range(2):  # the first solve launches kernel by kernel, the second captures the graphs
+        mdev.set_x(None)
+        mdev.solve_resident(Method.BiCgStab, False, Precision.FP64, SolveControl(mtol, 2000))
     t_mom = time.perf_counter()
     k_mom = max(1, args.steps // 2)
     for _ in range(k_mom):

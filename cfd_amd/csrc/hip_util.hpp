@@ -165,6 +165,21 @@ private:
     hipStream_t s_ = nullptr;
 };
 
+// Waits for an event by polling it. A blocking wait (hipEventSynchronize) was measured to
+// oversleep now and then by about one scheduler tick (~4 ms), leaving the device idle with only
+// the next iteration queued; the solve loops poll instead (one host core busy while they wait).
+inline void spin_wait(hipEvent_t e, const char* what) {
+    for (;;) {
+        const hipError_t q = hipEventQuery(e);
+        if (q == hipSuccess) {
+            return;
+        }
+        if (q != hipErrorNotReady) {
+            hip_check(q, what);
+        }
+    }
+}
+
 class Event {
 public:
     Event() { hip_check(hipEventCreate(&e_), "event create"); }

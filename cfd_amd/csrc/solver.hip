@@ -1461,6 +1461,15 @@ struct SweepFactors {
     [[nodiscard]] double post(int j, int k) const { return pre[k - 1 - j]; }
 };
 
+// Iterations a runner keeps queued ahead of its convergence read-back (CFDG_QUEUE_DEPTH, 1 or 2).
+int queue_depth() {
+    static const int d = [] {
+        const char* e = std::getenv("CFDG_QUEUE_DEPTH");
+        return e != nullptr && std::atoi(e) == 2 ? 2 : 1;
+    }();
+    return d;
+}
+
 SweepFactors sweep_factors(int k, double ratio) {
     SweepFactors f{};
     const double q = ratio > 1.0 ? (ratio - 1.0) / (ratio + 1.0) : 0.0;
@@ -1804,21 +1813,25 @@ public:
             enqueue_init(pl_, d, tb_, s_);
         }
         hip_check(hipGetLastError(), "init launch");
-        // Iterations are enqueued one ahead of the convergence read-back; surplus iterations are
-        // no-ops on the device because every kernel tests the system's status first.
-        for (int k = 1; k <= max_it + 2; ++k) {
+        // Iterations are enqueued `depth` ahead of the convergence read-back (CFDG_QUEUE_DEPTH,
+        // 1 or 2; default 1); surplus iterations are no-ops on the device because every kernel
+        // tests the system's status first. Iteration k reports into status slot k & 1; with
+        // depth 2 the read after iteration k - 2's event may see iteration k's report in the same
+        // slot, which is only more recent (a status only ever leaves RUNNING).
+        const int depth = queue_depth();
+        for (int k = 1; k <= max_it + 1 + depth; ++k) {
             const int slot = k & 1;
             if (graphs) {
                 hip_check(hipGraphLaunch(iter_exec_[slot], s_), "graph launch");
             } else {
                 enqueue_iteration(pl_, d, tb_, s_, slot, running_dev_ + slot * n_sys);
             }
-            hip_check(hipEventRecord(ev_[slot].get(), s_), "event record");
+            hip_check(hipEventRecord(ev_[k % 3].get(), s_), "event record");
             hip_check(hipGetLastError(), "iteration launch");
-            if (k >= 2) {
-                const int prev = (k - 1) & 1;
-                hip_check(hipEventSynchronize(ev_[prev].get()), "event sync");
-                if (!any_running(prev)) {
+            if (k > depth) {
+                const int j = k - depth;
+                spin_wait(ev_[j % 3].get(), "event sync");
+                if (!any_running(j & 1)) {
                     break;
                 }
             }
@@ -1830,6 +1843,8 @@ public:
         hip_check(hipMemcpyAsync(out_host_.get(), d_out_.get(), sys_.size() * sizeof(SolveState),
                                  hipMemcpyDeviceToHost, s_),
                   "D2H states");
+        hip_check(hipEventRecord(done_.get(), s_), "event record");
+        spin_wait(done_.get(), "solve sync");
         hip_check(hipStreamSynchronize(s_), "solve sync");
         float ms = 0.0f;
         hip_check(hipEventElapsedTime(&ms, t0_.get(), t1_.get()), "event elapsed");
@@ -1912,7 +1927,8 @@ private:
     int* running_dev_ = nullptr;
     PinnedBuf<SolveState> init_host_;
     PinnedBuf<SolveState> out_host_;
-    Event ev_[2];
+    Event ev_[3];
+    Event done_;
     Event t0_;
     Event t1_;
     bool use_graphs_ = true;
@@ -2105,7 +2121,7 @@ void SlotRunner::step(std::vector<Done>& done) {
         const long long h = m.inflight.front();
         m.inflight.erase(m.inflight.begin());
         const int hp = static_cast<int>(h & 1);
-        hip_check(hipEventSynchronize(m.ev[hp].get()), "event sync");
+        spin_wait(m.ev[hp].get(), "event sync");
         const int* flags = m.status.get() + static_cast<std::size_t>(hp) * m.k;
         for (const auto& [j, ep] : m.covered[hp]) {
             const std::size_t ju = static_cast<std::size_t>(j);
@@ -2137,7 +2153,7 @@ void SlotRunner::step(std::vector<Done>& done) {
             continue;
         }
         if (idle) {
-            hip_check(hipEventSynchronize(m.done_ev[ju]->get()), "event sync");
+            spin_wait(m.done_ev[ju]->get(), "event sync");
         } else {
             const hipError_t q = hipEventQuery(m.done_ev[ju]->get());
             if (q == hipErrorNotReady) {
