@@ -692,8 +692,8 @@ def main(argv=None) -> int:
     dev = DeviceSolver(pat)
     dev.assemble(p_sys.diag, p_sys.upper, p_sys.lower)
     sizes = hier.level_sizes()
-    # the FP32 cycle's dense tail: the first level below the finest, not the coarsest, of <= 512 rows
-    tail_m = next((l for l in range(1, len(sizes) - 1) if sizes[l] <= 512), 0)
+    # the FP32 cycle's dense tail: the first level below the finest, not the coarsest, of <= 2048 rows
+    tail_m = next((l for l in range(1, len(sizes) - 1) if sizes[l] <= 2048), 0)
     if os.environ.get("CFDG_DENSE_TAIL") == "0":
         tail_m = 0
     t_set = time.perf_counter()

@@ -1030,15 +1030,15 @@ void System::set_amg(const cfdg_amg_view& v, bool fuse_wanted) {
         }
     }
     // Dense tail of the FP32 cycle: the first level below the finest that is not the coarsest and
-    // has at most kDenseMaxOrder rows is solved exactly with its dense inverse (computed on the
-    // matrix cores) instead of being smoothed and restricted further: the levels under it are
+    // has at most kDenseWideMaxOrder rows is solved exactly with its dense inverse (computed on
+    // the matrix cores) instead of being smoothed and restricted further: the levels under it are
     // launch-bound, and one dense product replaces their launches (CFDG_DENSE_TAIL=0: off).
     int tail = 0;
     const char* tenv = std::getenv("CFDG_DENSE_TAIL");
     if (!(tenv != nullptr && std::string(tenv) == "0")) {
         for (int l = 1; l + 1 < L; ++l) {
             const int nl = rows[static_cast<std::size_t>(l)];
-            if (nl > kDenseMaxOrder) {
+            if (nl > kDenseWideMaxOrder) {
                 continue;
             }
             const cfdg_csr& a = v.levels[l].a;

@@ -1,9 +1,10 @@
 """Batched dense inversion on the FP64 matrix cores (cfdg_dense_inverse_batch; the coarsest-level
 inverse of the AMG setup, amg.cpp:185-189, 227-240): one batch of symmetric positive definite
-matrices of orders 1 to 512 (16-tile edges and their neighbours) and of coarsest operators of
-benchmark hierarchies, against numpy's LU inverse and the oracle's LDL^T inverse; orders above 512
-and a matrix with an exactly zero pivot take the host path (the pseudo-inverse, as the
-reference); solves with a device-built hierarchy against the oracle."""
+matrices of orders 1 to 512 (one workgroup each; 16-tile edges and their neighbours) and 513 to
+2048 (the whole device, two launches per block step), and coarsest operators of benchmark
+hierarchies, against numpy's LU inverse and the oracle's LDL^T inverse; orders above 2048 and a
+matrix with an exactly zero pivot take the host path (the pseudo-inverse, as the reference);
+solves with a device-built hierarchy against the oracle."""
 from __future__ import annotations
 
 import numpy as np
@@ -23,7 +24,7 @@ def _spd(n, rng, cond=1e3):
 
 def test_batch_of_spd_matrices_against_numpy():
     rng = np.random.default_rng(5)
-    orders = [1, 2, 15, 16, 17, 31, 48, 100, 257, 400, 512]
+    orders = [1, 2, 15, 16, 17, 31, 48, 100, 257, 400, 512, 513, 527, 944, 1636, 2048]
     mats = [_spd(n, rng) for n in orders]
     invs, on_dev = dense_inverse_batch(mats)
     for n, m, inv, dev in zip(orders, mats, invs, on_dev):
@@ -37,14 +38,14 @@ def test_batch_of_spd_matrices_against_numpy():
 
 def test_large_and_singular_matrices_take_the_host_path(oracle):
     rng = np.random.default_rng(6)
-    big = _spd(600, rng)
+    big = _spd(2100, rng)
     sing = _spd(40, rng)
     sing[7, :] = 0.0
     sing[:, 7] = 0.0  # an exactly zero pivot: LDL^T fails, the reference takes the pseudo-inverse
     ok = _spd(33, rng)
     invs, on_dev = dense_inverse_batch([big, sing, ok])
     assert on_dev == [False, False, True]
-    assert np.abs(big @ invs[0] - np.eye(600)).max() <= 1e-11
+    assert np.abs(big @ invs[0] - np.eye(2100)).max() <= 1e-11
     pinv = np.linalg.pinv(sing)
     assert np.abs(invs[1] - pinv).max() <= 1e-10 * np.abs(pinv).max()
     assert np.abs(ok @ invs[2] - np.eye(33)).max() <= 1e-11
@@ -65,7 +66,7 @@ def test_coarsest_operators_of_benchmark_hierarchies(oracle, shape, decay):
     dc = d.view().coarse_inverse
     print(f"{shape}: levels {d.level_sizes()}, coarsest on device {d.coarse_on_device}, "
           f"rel. difference to the oracle {np.abs(dc - oc).max() / np.abs(oc).max():.2e}")
-    assert d.coarse_on_device == (d.level_sizes()[-1] <= 512)
+    assert d.coarse_on_device == (d.level_sizes()[-1] <= 2048)
     assert np.abs(dc - oc).max() <= 1e-11 * np.abs(oc).max()
     tol = 1e-8 * np.linalg.norm(s.source)
     x = np.zeros(s.n)

@@ -1,5 +1,5 @@
 """Dense tail of the FP32 V-cycle: the first level below the finest that is not the coarsest and
-has at most 512 rows is solved exactly with its dense inverse (computed on the matrix cores when
+has at most 2048 rows is solved exactly with its dense inverse (computed on the matrix cores when
 the hierarchy is set) and the FP32 cycle stops there, while the FP64 cycle (the reference's,
 krylov_kernels.inc:209-238) keeps every level. Mixed solves with the tail (the default) and
 without it (CFDG_DENSE_TAIL=0) reach the tolerance and solutions within the tolerance-implied
@@ -17,8 +17,9 @@ from cfd_amd import AmgHierarchy, AmgSettings, Method, Precision, SolveControl, 
 pytestmark = pytest.mark.gpu
 
 # (40, 40, 40) jittered: levels [64000, 7809, 437, 19] (level 2 becomes the tail);
-# (20, 20, 60): [24000, 2942, 170] (no level qualifies: the coarsest is the tail already)
-SHAPES = [((40, 40, 40), 0.2, 1.0), ((20, 20, 60), 0.0, 3.0), ((36, 30, 44), 0.3, 1.0)]
+# (20, 20, 60): [24000, 2942, 170] (no level qualifies: the coarsest is the tail already);
+# (48, 48, 48): [110592, 13310, 746, 32] (a tail above 512 rows: the multi-workgroup inverse)
+SHAPES = [((40, 40, 40), 0.2, 1.0), ((20, 20, 60), 0.0, 3.0), ((36, 30, 44), 0.3, 1.0), ((48, 48, 48), 0.2, 1.0)]
 
 
 def _system(shape, jitter, seed, lz):
@@ -60,7 +61,8 @@ def _solve(systems, prec, batch):
 def test_tail_against_the_full_cycle_and_the_oracle(monkeypatch, oracle):
     systems = [_system(shape, jitter, 7 + i, lz) for i, (shape, jitter, lz) in enumerate(SHAPES)]
     sizes = [h.level_sizes() for *_, h in systems]
-    assert any(any(n <= 512 for n in ls[1:-1]) for ls in sizes)  # at least one system has a tail level
+    assert any(any(n <= 512 for n in ls[1:-1]) for ls in sizes)  # a one-workgroup tail inverse
+    assert any(any(512 < n <= 2048 for n in ls[1:-1]) for ls in sizes)  # a multi-workgroup one
     monkeypatch.delenv("CFDG_DENSE_TAIL", raising=False)
     tail = _solve(systems, Precision.MIXED_FP32, False)
     monkeypatch.setenv("CFDG_DENSE_TAIL", "0")
