@@ -12,6 +12,8 @@ namespace cfdg {
 // 6 ms), and the largest inverted on the device at all (two launches per 16-row block step).
 constexpr int kDenseMaxOrder = 256;
 constexpr int kDenseWideMaxOrder = 2048;
+// Dense tails of the FP32 cycle up to this order are taken whatever the finest level's size.
+constexpr int kTailOrder = 512;
 
 // Inverts the row-major order[i] x order[i] host matrices mats[i] into out[i] on the device
 // (block Gauss-Jordan without pivoting, FP64 MFMA): one workgroup per matrix up to kDenseMaxOrder,

@@ -1029,16 +1029,24 @@ void System::set_amg(const cfdg_amg_view& v, bool fuse_wanted) {
             d.cinv.upload(v.coarse_inverse, static_cast<std::size_t>(d.n) * static_cast<std::size_t>(d.n), s);
         }
     }
-    // Dense tail of the FP32 cycle: the first level below the finest that is not the coarsest and
-    // has at most kDenseWideMaxOrder rows is solved exactly with its dense inverse (computed on
-    // the matrix cores) instead of being smoothed and restricted further: the levels under it are
-    // launch-bound, and one dense product replaces their launches (CFDG_DENSE_TAIL=0: off).
+    // Dense tail of the FP32 cycle: a level below the finest that is not the coarsest is solved
+    // exactly with its dense inverse (computed on the matrix cores) instead of being smoothed and
+    // restricted further: the levels under it are launch-bound, and one dense product replaces
+    // their launches. The first level with at most kTailOrder rows takes it, or one with at most
+    // kDenseWideMaxOrder rows whose inverse holds at most a tenth as many entries as the finest
+    // operator: in a batch the launches are shared by all members while every member reads its
+    // own inverse, so the larger tails pay off only against a large finest level (CFDG_DENSE_TAIL=0:
+    // off).
     int tail = 0;
     const char* tenv = std::getenv("CFDG_DENSE_TAIL");
     if (!(tenv != nullptr && std::string(tenv) == "0")) {
+        const long long nnz0 = v.levels[0].a.row_ptr != nullptr && v.levels[0].a.nnz > 0
+                                   ? static_cast<long long>(v.levels[0].a.nnz)
+                                   : 7LL * static_cast<long long>(n_);
         for (int l = 1; l + 1 < L; ++l) {
             const int nl = rows[static_cast<std::size_t>(l)];
-            if (nl > kDenseWideMaxOrder) {
+            const long long cells = static_cast<long long>(nl) * nl;
+            if (nl > kDenseWideMaxOrder || (nl > kTailOrder && 10 * cells > nnz0)) {
                 continue;
             }
             const cfdg_csr& a = v.levels[l].a;

@@ -1,6 +1,7 @@
 """Dense tail of the FP32 V-cycle: the first level below the finest that is not the coarsest and
-has at most 2048 rows is solved exactly with its dense inverse (computed on the matrix cores when
-the hierarchy is set) and the FP32 cycle stops there, while the FP64 cycle (the reference's,
+has at most 512 rows, or at most 2048 rows with n^2 at most a tenth of the finest operator's
+entries, is solved exactly with its dense inverse (computed on the matrix cores when the
+hierarchy is set) and the FP32 cycle stops there, while the FP64 cycle (the reference's,
 krylov_kernels.inc:209-238) keeps every level. Mixed solves with the tail (the default) and
 without it (CFDG_DENSE_TAIL=0) reach the tolerance and solutions within the tolerance-implied
 bound of the FP64 oracle's; the tail needs no more iterations (+1 for rounding); a batch that
@@ -18,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 # (40, 40, 40) jittered: levels [64000, 7809, 437, 19] (level 2 becomes the tail);
 # (20, 20, 60): [24000, 2942, 170] (no level qualifies: the coarsest is the tail already);
-# (48, 48, 48): [110592, 13310, 746, 32] (a tail above 512 rows: the multi-workgroup inverse)
+# (48, 48, 48): [110592, 13310, 746, 32] (746^2 is above a tenth of the finest operator's
+# entries: no tail)
 SHAPES = [((40, 40, 40), 0.2, 1.0), ((20, 20, 60), 0.0, 3.0), ((36, 30, 44), 0.3, 1.0), ((48, 48, 48), 0.2, 1.0)]
 
 
@@ -61,8 +63,7 @@ def _solve(systems, prec, batch):
 def test_tail_against_the_full_cycle_and_the_oracle(monkeypatch, oracle):
     systems = [_system(shape, jitter, 7 + i, lz) for i, (shape, jitter, lz) in enumerate(SHAPES)]
     sizes = [h.level_sizes() for *_, h in systems]
-    assert any(any(n <= 512 for n in ls[1:-1]) for ls in sizes)  # a one-workgroup tail inverse
-    assert any(any(512 < n <= 2048 for n in ls[1:-1]) for ls in sizes)  # a multi-workgroup one
+    assert any(any(n <= 512 for n in ls[1:-1]) for ls in sizes)  # at least one system has a tail
     monkeypatch.delenv("CFDG_DENSE_TAIL", raising=False)
     tail = _solve(systems, Precision.MIXED_FP32, False)
     monkeypatch.setenv("CFDG_DENSE_TAIL", "0")
@@ -90,6 +91,33 @@ def test_batch_with_and_without_tail_levels_matches_single_solves():
     for (sa, xa), (sb, xb) in zip(alone, together):
         assert sa.iterations == sb.iterations and sa.residual == sb.residual
         assert np.array_equal(xa, xb)
+
+
+def test_wide_tail_on_the_160_cube(monkeypatch):
+    """160^3 jittered box, levels [4096000, 492605, 27367, 944, 38]: level 3 (944 rows, 944^2 below
+    a tenth of the finest operator's 28.5 M entries) takes the tail, inverted by the device-wide
+    Gauss-Jordan; the tail solve reaches the FP64 tolerance in its true FP64 residual, with no more
+    iterations than the full cycle (+1), and so does the full cycle."""
+    mesh = cases.box_mesh(160, 160, 160, jitter=0.2, seed=3)
+    s = cases.pressure_system(mesh, rhs_seed=4)
+    pat = cases.csr_pattern(s.n, s.owner, s.neighbour)
+    a = cases.ldu_to_csr(pat, s.diag, s.upper, s.lower)
+    h = AmgHierarchy(a, AmgSettings(strength_decay=0.5), device=True)
+    ls = h.level_sizes()
+    tail = [l for l in range(1, len(ls) - 1) if ls[l] <= 512 or (ls[l] <= 2048 and 10 * ls[l] ** 2 <= a.nnz)]
+    assert tail and ls[tail[0]] > 512, ls
+    systems = [(s, pat, a, h)]
+    monkeypatch.delenv("CFDG_DENSE_TAIL", raising=False)
+    (st, xt), = _solve(systems, Precision.MIXED_FP32, False)
+    monkeypatch.setenv("CFDG_DENSE_TAIL", "0")
+    (sf, xf), = _solve(systems, Precision.MIXED_FP32, False)
+    tol = 1e-8 * float(np.linalg.norm(s.source))
+    print(f"levels {ls}: tail at level {tail[0]}, it {st.iterations} ({st.solve_ms:.2f} ms), full cycle it "
+          f"{sf.iterations} ({sf.solve_ms:.2f} ms); true residuals {st.true_residual:.3e} {sf.true_residual:.3e} "
+          f"tol {tol:.3e}; |xt - xf| / |xf| {np.linalg.norm(xt - xf) / np.linalg.norm(xf):.2e}")
+    assert st.converged and sf.converged
+    assert st.true_residual <= 1.05 * tol and sf.true_residual <= 1.05 * tol
+    assert st.iterations <= sf.iterations + 1
 
 
 def test_fp64_cycle_is_unaffected(monkeypatch):
