@@ -1332,12 +1332,27 @@ enum FinOp : int {
     FIN_CG_RES_BETA = 6,// Jacobi CG: res check then beta
     FIN_TRUE = 7,       // true residual
     FIN_BICG_INIT = 8,
-    FIN_BICG_RHO = 9,
     FIN_BICG_ALPHA = 10,
     FIN_BICG_S = 11,
     FIN_BICG_OMEGA = 12,
     FIN_BICG_RES = 13
 };
+
+// Start of a BiCGStab iteration (krylov_kernels.inc:96-169): rho_new = r0.r (kept in rz), the
+// iteration limit, the breakdown guards rho_new == 0 and omega == 0, and beta. Run by thread 0 of
+// the finalize that ends the previous step (the start-up or the previous iteration's residual),
+// so an iteration needs no launch of its own for it.
+__device__ __forceinline__ void bicg_rho(SolveState* st) {
+    const double rho_new = st->rz;
+    if (st->it > st->max_it) {
+        st->status = ST_MAXIT;
+    } else if (rho_new == 0.0 || st->omega == 0.0) {
+        st->status = ST_BREAKDOWN;
+    } else {
+        st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
+        st->rho = rho_new;
+    }
+}
 
 template <int Op>
 __device__ __forceinline__ void finalize_step(const SysDev& S) {
@@ -1384,6 +1399,7 @@ __device__ __forceinline__ void finalize_step(const SysDev& S) {
                 st->omega = 1.0;
                 st->rz = r0r; // rho_new of the first iteration
                 st->it = 1;
+                bicg_rho(st);
             }
         }
         return;
@@ -1452,20 +1468,6 @@ __device__ __forceinline__ void finalize_step(const SysDev& S) {
         }
         return;
     }
-    if (Op == FIN_BICG_RHO) {
-        if (threadIdx.x == 0) {
-            const double rho_new = st->rz;
-            if (st->it > st->max_it) {
-                st->status = ST_MAXIT;
-            } else if (rho_new == 0.0 || st->omega == 0.0) {
-                st->status = ST_BREAKDOWN;
-            } else {
-                st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
-                st->rho = rho_new;
-            }
-        }
-        return;
-    }
     if (Op == FIN_BICG_ALPHA) {
         const double r0v = reduce_slot(S, SLOT_R0V);
         if (threadIdx.x == 0) {
@@ -1511,6 +1513,7 @@ __device__ __forceinline__ void finalize_step(const SysDev& S) {
             } else {
                 st->rz = r0r;
                 st->it += 1;
+                bicg_rho(st);
             }
         }
         return;

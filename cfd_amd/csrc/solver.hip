@@ -1657,8 +1657,7 @@ void enqueue_iteration(const Plan& pl, const SysDev* d, const Tables& tb, hipStr
     const int g0 = tb.tn[0];
     const int2* b0 = tb.t[0].get();
     if (pl.method == CFDG_METHOD_BICGSTAB) {
-        fin<FIN_BICG_RHO>(d, tb, s);
-        CFDG_LAUNCH(k_bicg_p, g0, s, d, b0);
+        CFDG_LAUNCH(k_bicg_p, g0, s, d, b0); // rho and beta: ended the previous finalize (bicg_rho)
         CFDG_LAUNCH((k_spmv_dot<false, false>), g0, s, d, b0, V_PH, V_V, V_R0, SLOT_R0V, 0, 0);
         fin<FIN_BICG_ALPHA>(d, tb, s);
         CFDG_LAUNCH(k_bicg_s, g0, s, d, b0);
@@ -2816,7 +2815,6 @@ extern "C" cfdg_status cfdg_solve_csr_noalloc(const cfdg_csr* a, const double* b
                 k_update_xr<1><<<nblk, k_block, 0, nullptr>>>(desc, blocks, parity);
                 na_fin<FIN_CG_RES_BETA>(desc, ids);
             } else {
-                na_fin<FIN_BICG_RHO>(desc, ids);
                 k_bicg_p<<<nblk, k_block, 0, nullptr>>>(desc, blocks);
                 k_spmv_dot<false, true><<<nblk, k_block, 0, nullptr>>>(desc, blocks, V_PH, V_V, V_R0, SLOT_R0V, 0, 0);
                 na_fin<FIN_BICG_ALPHA>(desc, ids);
