@@ -371,9 +371,9 @@ def served(count: int = 24, rate: float = 200.0, seed: int = 20240607) -> dict:
     the mixed-precision server (continuous batching), the FP64 server, and the plain FP64 path
     (one solve at a time); then the mixed server again with a device-memory budget of a third of
     its unconstrained peak, so that admission by footprint holds requests back, and with each
-    request's hierarchy built when it arrives (host setup on four threads), so that request
-    latency includes hierarchy construction. Reported: solves/s over the makespan and p50 / p99
-    latencies."""
+    request's hierarchy built when it arrives (setup with its sparse products on the GPU, four
+    requests at a time), so that request latency includes hierarchy construction. Reported:
+    solves/s over the makespan and p50 / p99 latencies."""
     import threading
 
     import cfd_amd
@@ -403,8 +403,8 @@ def served(count: int = 24, rate: float = 200.0, seed: int = 20240607) -> dict:
             pool = ThreadPoolExecutor(4) if build_at_arrival else None
 
             def submit(i, q):
-                if build_at_arrival:  # the request's hierarchy is built when it arrives
-                    q = dict(q, amg=AmgHierarchy(csrs[i], AmgSettings(strength_decay=0.5)).view())
+                if build_at_arrival:  # the request's hierarchy is built when it arrives (products on the GPU)
+                    q = dict(q, amg=AmgHierarchy(csrs[i], AmgSettings(strength_decay=0.5), device=True).view())
                 submitted[i] = time.perf_counter()
                 handles[i] = srv.submit(**q, keep_solutions=False)
 
