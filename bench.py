@@ -726,6 +726,8 @@ def main(argv=None) -> int:
     if args.solves_only:
         _, dm, im, _ = run(Precision.MIXED_FP32, args.steps)
         _, dd, idd, _ = run(Precision.FP64, args.steps)
+        log("mixed solves (device ms): " + " ".join(f"{v:.3f}" for v in dm))
+        log("FP64 solves (device ms): " + " ".join(f"{v:.3f}" for v in dd))
         print(json.dumps({"cells": p_sys.n, "mixed_device_ms": float(np.median(dm)), "mixed_iterations": im[-1],
                           "fp64_device_ms": float(np.median(dd)), "fp64_iterations": idd[-1],
                           "set_amg_ms": round(set_amg_ms, 1)}), flush=True)

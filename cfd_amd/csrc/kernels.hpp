@@ -481,6 +481,13 @@ __device__ __forceinline__ int xcd_block() {
     return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
+// The launch table's (system, block) entry of this workgroup, read as one 8-byte load: both
+// halves arrive with one wait (the block half is otherwise fetched again after the status test).
+__device__ __forceinline__ int2 block_entry(const int2* __restrict__ blocks) {
+    const auto e = gm(reinterpret_cast<const unsigned long long*>(blocks))[xcd_block()];
+    return make_int2(static_cast<int>(static_cast<unsigned>(e)), static_cast<int>(static_cast<unsigned>(e >> 32)));
+}
+
 // Outer vectors by id: 0 x, 1 r, 2 p, 3 q, 4 z, 5 r0, 6 v, 7 s, 8 t, 9 ph, 10 sh.
 __device__ __forceinline__ double* outer_vec(const SysDev& S, int id) {
     switch (id) {
@@ -528,7 +535,7 @@ __device__ __forceinline__ double a0_row(const EllView& m, int slice, int lane, 
 }
 
 #define CFDG_BLOCK_PROLOGUE(sysarr, blocks)                                                     \
-    const int2 bk_ = (blocks)[xcd_block()];                                                       \
+    const int2 bk_ = block_entry(blocks);                                                         \
     const SysDev& S = (sysarr)[bk_.x];                                                            \
     const int wave_ = threadIdx.x / k_wave;                                                       \
     const int lane = threadIdx.x % k_wave;                                                        \
@@ -664,7 +671,8 @@ template <class TZ, bool Csr>
 __global__ void __launch_bounds__(k_block) k_spmv_pz(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int parity,
                           int which, int z_buf) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
-    if (S.st->status != ST_RUNNING) {
+    const EllView A0 = S.a64; // requested with the status (one wait)
+    if ((S.st->status != ST_RUNNING) | (A0.tpr <= 0)) {
         return;
     }
     const auto z = gm(which == 0 ? reinterpret_cast<const TZ*>(S.z) : static_cast<const TZ*>(S.lv[0].v[z_buf]));
@@ -672,8 +680,8 @@ __global__ void __launch_bounds__(k_block) k_spmv_pz(const SysDev* __restrict__ 
     const auto pn = gm(p_new_of(S, parity));
     const double beta = S.st->beta;
     double d = 0.0;
-    if (a0_live<Csr>(S.a64, slice)) {
-        const double qi = a0_row_f<Csr>(S.a64, slice, lane, row, [&](int c) {
+    if (a0_live<Csr>(A0, slice)) {
+        const double qi = a0_row_f<Csr>(A0, slice, lane, row, [&](int c) {
             return static_cast<double>(z[c]) + beta * po[c];
         });
         if (row < S.n) {
@@ -885,18 +893,17 @@ __global__ void __launch_bounds__(k_block) k_bicg_xr(const SysDev* __restrict__ 
 
 // Row mapping of a V-cycle kernel over matrix M: 256 / tpr rows per workgroup, tpr lanes per
 // row. Early returns here are uniform over the workgroup.
+// The system's status and the level's descriptor fields are requested before either is tested
+// (one wait instead of a chain of dependent loads: the small levels' kernels are latency-bound).
 #define CFDG_LEVEL_PROLOGUE(sysarr, blocks, level, MAT)                                            \
-    const int2 bk_ = (blocks)[xcd_block()];                                                       \
+    const int2 bk_ = block_entry(blocks);                                                         \
     const SysDev& S = (sysarr)[bk_.x];                                                            \
-    if (S.st->status != ST_RUNNING) {                                                             \
-        return;                                                                                   \
-    }                                                                                             \
     const LevelDev& L = S.lv[level];                                                              \
-    if (L.coarsest) {                                                                             \
+    const EllView M = L.MAT;                                                                      \
+    const int tpr = M.tpr;                                                                        \
+    if ((S.st->status != ST_RUNNING) | (L.coarsest != 0) | (tpr <= 0)) {                          \
         return;                                                                                   \
     }                                                                                             \
-    const EllView& M = L.MAT;                                                                     \
-    const int tpr = M.tpr;                                                                        \
     const int row = bk_.y * (k_block / tpr) + static_cast<int>(threadIdx.x) / tpr;                \
     const int sub = static_cast<int>(threadIdx.x) & (tpr - 1);                                    \
     (void)sub;
@@ -1016,11 +1023,8 @@ template <class T>
 __global__ void __launch_bounds__(k_block) k_jacobi_zero(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out,
                                                          T f) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)
-    if (S.st->status != ST_RUNNING) {
-        return;
-    }
-    const LevelDev& L = S.lv[level];
-    if (L.coarsest || row >= L.n) {
+    const LevelDev& L = S.lv[level]; // fields requested with the status (one wait)
+    if ((S.st->status != ST_RUNNING) | (L.coarsest != 0) | (row >= L.n)) {
         return;
     }
     op_jacobi_zero<T>(L, row, out, f);
@@ -1087,16 +1091,13 @@ __global__ void __launch_bounds__(k_block) k_jacobi(const SysDev* __restrict__ s
 constexpr int k_block_pair = k_block / 2;
 
 #define CFDG_PAIR_PROLOGUE(sysarr, blocks, level)                                                 \
-    const int2 bk_ = (blocks)[xcd_block()];                                                       \
+    const int2 bk_ = block_entry(blocks);                                                         \
     const SysDev& S = (sysarr)[bk_.x];                                                            \
-    if (S.st->status != ST_RUNNING) {                                                             \
-        return;                                                                                   \
-    }                                                                                             \
     const LevelDev& L = S.lv[level];                                                              \
-    if (L.coarsest) {                                                                             \
+    const EllView M = L.a;                                                                        \
+    if ((S.st->status != ST_RUNNING) | (L.coarsest != 0) | (M.tpr <= 0)) {                        \
         return;                                                                                   \
     }                                                                                             \
-    const EllView& M = L.a;                                                                       \
     const int lane = static_cast<int>(threadIdx.x) % k_wave;                                      \
     const int wave_ = static_cast<int>(threadIdx.x) / k_wave;                                     \
     const int s0 = bk_.y * k_slices_per_block + 2 * wave_;                                        \
@@ -1202,13 +1203,10 @@ __device__ __forceinline__ void pair_rows(const EllView& m, int y, F load, E epi
 }
 
 #define CFDG_PAIR_LEVEL_PROLOGUE(sysarr, blocks, level)                                           \
-    const int2 bk_ = (blocks)[xcd_block()];                                                       \
+    const int2 bk_ = block_entry(blocks);                                                         \
     const SysDev& S = (sysarr)[bk_.x];                                                            \
-    if (S.st->status != ST_RUNNING) {                                                             \
-        return;                                                                                   \
-    }                                                                                             \
     const LevelDev& L = S.lv[level];                                                              \
-    if (L.coarsest) {                                                                             \
+    if ((S.st->status != ST_RUNNING) | (L.coarsest != 0)) {                                       \
         return;                                                                                   \
     }
 
@@ -1281,16 +1279,13 @@ __global__ void __launch_bounds__(k_block) k_residual_x1(const SysDev* __restric
 // index groups of four rows.
 template <class T>
 __global__ void __launch_bounds__(k_block) k_coarse(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int level, int out) {
-    const int2 bk = blocks[xcd_block()];
+    const int2 bk = block_entry(blocks);
     const SysDev& S = sys[bk.x];
-    if (S.st->status != ST_RUNNING) {
-        return;
-    }
-    const LevelDev& L = S.lv[level];
+    const LevelDev& L = S.lv[level]; // fields requested with the status (one wait)
     const int wave = threadIdx.x / k_wave;
     const int lane = threadIdx.x % k_wave;
     const int i = bk.y * k_slices_per_block + wave;
-    if (!L.coarsest || i >= L.n) {
+    if ((S.st->status != ST_RUNNING) | (L.coarsest == 0) | (i >= L.n)) {
         return;
     }
     op_coarse_row<T>(L, i, lane, out);
