@@ -1469,11 +1469,15 @@ struct SweepFactors {
     [[nodiscard]] double post(int j, int k) const { return pre[k - 1 - j]; }
 };
 
-// Iterations a runner keeps queued ahead of its convergence read-back (CFDG_QUEUE_DEPTH, 1 or 2).
+// Iterations a runner keeps queued ahead of its convergence read-back (CFDG_QUEUE_DEPTH, 1 to
+// k_max_queue_depth; default 1).
+constexpr int k_max_queue_depth = 7;
+
 int queue_depth() {
     static const int d = [] {
         const char* e = std::getenv("CFDG_QUEUE_DEPTH");
-        return e != nullptr && std::atoi(e) == 2 ? 2 : 1;
+        const int v = e != nullptr ? std::atoi(e) : 1;
+        return std::min(std::max(v, 1), k_max_queue_depth);
     }();
     return d;
 }
@@ -1821,10 +1825,10 @@ public:
         }
         hip_check(hipGetLastError(), "init launch");
         // Iterations are enqueued `depth` ahead of the convergence read-back (CFDG_QUEUE_DEPTH,
-        // 1 or 2; default 1); surplus iterations are no-ops on the device because every kernel
+        // 1 to 7; default 1); surplus iterations are no-ops on the device because every kernel
         // tests the system's status first. Iteration k reports into status slot k & 1; with
-        // depth 2 the read after iteration k - 2's event may see iteration k's report in the same
-        // slot, which is only more recent (a status only ever leaves RUNNING).
+        // depth 2 or more the read after iteration j's event may see a later iteration's report
+        // in the same slot, which is only more recent (a status only ever leaves RUNNING).
         const int depth = queue_depth();
         for (int k = 1; k <= max_it + 1 + depth; ++k) {
             const int slot = k & 1;
@@ -1833,11 +1837,11 @@ public:
             } else {
                 enqueue_iteration(pl_, d, tb_, s_, slot, running_dev_ + slot * n_sys);
             }
-            hip_check(hipEventRecord(ev_[k % 3].get(), s_), "event record");
+            hip_check(hipEventRecord(ev_[k % k_events].get(), s_), "event record");
             hip_check(hipGetLastError(), "iteration launch");
             if (k > depth) {
                 const int j = k - depth;
-                spin_wait(ev_[j % 3].get(), "event sync");
+                spin_wait(ev_[j % k_events].get(), "event sync");
                 if (!any_running(j & 1)) {
                     break;
                 }
@@ -1934,7 +1938,8 @@ private:
     int* running_dev_ = nullptr;
     PinnedBuf<SolveState> init_host_;
     PinnedBuf<SolveState> out_host_;
-    Event ev_[3];
+    static constexpr int k_events = k_max_queue_depth + 1; // one per queued iteration and the one waited on
+    Event ev_[k_events];
     Event done_;
     Event t0_;
     Event t1_;

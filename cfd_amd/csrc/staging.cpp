@@ -4,6 +4,9 @@
 // from a process-wide free list (one per concurrent caller, kept until the process ends).
 #include "hip_util.hpp"
 
+#include <omp.h>
+
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -11,6 +14,22 @@
 namespace cfdg {
 
 namespace {
+
+// The host threads of the library's OpenMP regions (LLVM libomp) sleep 1 ms after a region
+// instead of the runtime's default 200 ms of spinning, unless the process set KMP_BLOCKTIME or
+// OMP_WAIT_POLICY itself. With a CPU quota equal to the worker count (a cgroup cpu.max of 16
+// CPUs for 16 threads) the spinning workers and the thread that polls the GPU exceed the quota
+// after every host-side setup, the process is throttled for the rest of the scheduler period, and
+// solves queued meanwhile stall for milliseconds between two kernels. Runs when the library is
+// loaded: kmp_set_blocktime covers the loading thread, kmp_set_defaults threads that start
+// OpenMP regions later.
+__attribute__((constructor)) void short_omp_blocktime() {
+    if (std::getenv("KMP_BLOCKTIME") != nullptr || std::getenv("OMP_WAIT_POLICY") != nullptr) {
+        return;
+    }
+    kmp_set_defaults("KMP_BLOCKTIME=1");
+    kmp_set_blocktime(1);
+}
 
 constexpr std::size_t k_chunk = std::size_t(16) << 20; // bytes per chunk; two chunks per area
 
