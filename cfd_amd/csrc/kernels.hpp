@@ -238,10 +238,98 @@ __device__ __forceinline__ T csr_lane_sum(const gmem_t<const int>* __restrict__ 
     return s;
 }
 
+#ifndef CFDG_SYM_GROUP
+#define CFDG_SYM_GROUP 8
+#endif
+constexpr int k_sym_group = CFDG_SYM_GROUP; // entries whose loads are issued together
+
+// Row sum of a row in symmetric stencil storage (EllView::sym_w > 0), entries in offset order:
+// the value of offset o >= 0 from the row's own stored slot, of o < 0 from row + o's stored slot
+// of -o (zero where row + o < 0); the column row + o clamped to the matrix, as in the per-slice
+// stencil layout (whose missing offsets hold zeros). Loads are issued k_sym_group entries at a
+// time.
+template <class T, class F>
+__device__ __forceinline__ T ell_row_sym(const EllView& m, int slice, int lane, F load) {
+    const int row = slice * k_wave + lane;
+    const int n = m.sym_n;
+    const int stride = m.sym_w * k_wave; // values per slice
+    const auto tab = gm(m.soff);         // offsets [0, n), stored slots [n, 2 n)
+    const auto val = gm(static_cast<const T*>(m.val));
+    const int last = m.n_rows - 1;
+    auto entry = [&](int k, T& a, T& x) {
+        const int o = tab[k];
+        const int j = tab[n + k];
+        const int r = o < 0 ? row + o : row; // the row whose stored slot holds a(row, row + o)
+        const int rr = max(r, 0);
+        const T v = val[(rr >> 6) * stride + j * k_wave + (rr & (k_wave - 1))];
+        a = r < 0 ? T(0) : v;
+        x = load(min(max(row + o, 0), last));
+    };
+    T s = T(0);
+    for (int k = 0; k < n; k += k_sym_group) {
+        T a[k_sym_group];
+        T x[k_sym_group];
+#pragma unroll
+        for (int g = 0; g < k_sym_group; ++g) {
+            entry(min(k + g, n - 1), a[g], x[g]); // past the stencil: repeats its last entry, not added
+        }
+#pragma unroll
+        for (int g = 0; g < k_sym_group; ++g) {
+            if (k + g < n) {
+                s = madd(a[g], x[g], s);
+            }
+        }
+    }
+    return s;
+}
+
+// ell_row_sym for the rows of slices s0 and s0 + 1 (lane's row in each), their loads issued
+// together: the same sums as two ell_row_sym calls.
+template <class T, class F>
+__device__ __forceinline__ void ell_row_sym_pair(const EllView& m, int s0, int lane, F load, T& ya, T& yb) {
+    const int row0 = s0 * k_wave + lane;
+    const int n = m.sym_n;
+    const int stride = m.sym_w * k_wave;
+    const auto tab = gm(m.soff);
+    const auto val = gm(static_cast<const T*>(m.val));
+    const int last = m.n_rows - 1;
+    auto entry = [&](int row, int o, int j, T& a, T& x) {
+        const int r = o < 0 ? row + o : row;
+        const int rr = max(r, 0);
+        const T v = val[(rr >> 6) * stride + j * k_wave + (rr & (k_wave - 1))];
+        a = r < 0 ? T(0) : v;
+        x = load(min(max(row + o, 0), last));
+    };
+    ya = T(0);
+    yb = T(0);
+    for (int k = 0; k < n; k += k_sym_group) {
+        T a[2][k_sym_group];
+        T x[2][k_sym_group];
+#pragma unroll
+        for (int g = 0; g < k_sym_group; ++g) {
+            const int kk = min(k + g, n - 1);
+            const int o = tab[kk];
+            const int j = tab[n + kk];
+            entry(row0, o, j, a[0][g], x[0][g]);
+            entry(row0 + k_wave, o, j, a[1][g], x[1][g]);
+        }
+#pragma unroll
+        for (int g = 0; g < k_sym_group; ++g) {
+            if (k + g < n) {
+                ya = madd(a[0][g], x[0][g], ya);
+                yb = madd(a[1][g], x[1][g], yb);
+            }
+        }
+    }
+}
+
 // Row sum of an ELL row with the x entry of column c produced by load(c), in storage order.
 // BF16 rows (fmt 1, FP32 kernels only) give diag_i x_i + rscale_i sum_k bf16_k x_ck.
 template <class T, class F>
 __device__ __forceinline__ T ell_row_f(const EllView& m, int slice, int lane, F load) {
+    if (m.sym_w > 0) {
+        return ell_row_sym<T>(m, slice, lane, load);
+    }
     const int base = m.sp[slice];
     const int w = (m.sp[slice + 1] - base) / k_wave;
     const int lg = __builtin_ctz(m.tpr); // lanes per row = 2^lg
@@ -373,6 +461,10 @@ __device__ __forceinline__ void ell_row_pair(const EllView& m, int s0, int lane,
     }
     if (s0 + 1 >= m.n_slices) {
         ya = ell_row_f<T>(m, s0, lane, load);
+        return;
+    }
+    if (m.sym_w > 0) {
+        ell_row_sym_pair<T>(m, s0, lane, load, ya, yb);
         return;
     }
     const int b0 = m.sp[s0];
@@ -1622,6 +1714,16 @@ __global__ void __launch_bounds__(k_block) k_ldu_gather(long long n_slots, const
     val64[k] = v;
     if (val32 != nullptr) {
         val32[k] = static_cast<float>(v);
+    }
+}
+
+// flag = 1 if some face's upper and lower coefficients differ in any bit (the operator is not
+// exactly symmetric); flag is cleared by the caller.
+__global__ void __launch_bounds__(k_block) k_ldu_asymmetric(int nf, const double* __restrict__ upper,
+                                                            const double* __restrict__ lower, int* __restrict__ flag) {
+    const int f = blockIdx.x * k_block + threadIdx.x;
+    if (f < nf && __double_as_longlong(upper[f]) != __double_as_longlong(lower[f])) {
+        *flag = 1;
     }
 }
 

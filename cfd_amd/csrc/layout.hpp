@@ -66,12 +66,20 @@ struct SolveState {
 // offsets a row lacks hold value 0. fmt 1 stores the values as BF16 scaled per row (a_ij = rscale_i * bf16_ij) with
 // the diagonal slot zero and the diagonal in FP32 in diag (rscale and diag padded to whole
 // slices; tpr == 1 only).
+// Symmetric stencil storage (sym_w > 0; square, exactly symmetric, tpr == 1): one stencil of
+// sym_n offsets for all rows, soff = [offsets ascending | stored slot of each], and only the
+// entries with offset >= 0 stored: sym_w slots per row, slice s holding rows 64 s .. 64 s + 63
+// at val[s sym_w 64 + j 64 + lane] for the j-th offset >= 0. An entry with offset o < 0 is read
+// from row + o's stored slot of -o (a_ij = a_ji); a row's entries are summed in offset order,
+// the order (and result) of the per-slice stencil layout.
 struct EllView {
     int n_rows;
     int tpr;
     int n_slices;    // ELL only
     int fmt;         // 0 values of the kernel's type, 1 BF16 + row scale + FP32 diagonal
     int csr;         // 1: CSR rows, 0: sliced ELL
+    int sym_w;       // symmetric stencil storage: stored slots per row (0: not used)
+    int sym_n;       // symmetric stencil storage: offsets of the stencil
     const int* sp;   // ELL slice offsets or CSR row offsets
     const int* col;
     const short* dcol;  // 16-bit columns: col = row + dcol, or col = cbase[slice] + (unsigned) dcol

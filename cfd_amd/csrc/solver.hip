@@ -554,6 +554,7 @@ System::System(const cfdg_ldu_pattern& pt) {
             hip_check(hipStreamSynchronize(s), "pattern build sync");
             a0_.v64.resize(st_slots);
             a0_.v32.resize(st_slots);
+            build_symmetric_layout(pt, csr_src);
             ldu_.resize(static_cast<std::size_t>(n_) + 2 * static_cast<std::size_t>(nf_));
             init_vectors();
             return;
@@ -583,6 +584,98 @@ System::System(const cfdg_ldu_pattern& pt) {
     a0_.v32.resize(slots);
     ldu_.resize(static_cast<std::size_t>(n_) + 2 * static_cast<std::size_t>(nf_));
     init_vectors();
+}
+
+void System::build_symmetric_layout(const cfdg_ldu_pattern& pt, const std::vector<int>& csr_src) {
+    sym_w_ = 0;
+    sym_n_ = 0;
+    sym_active_ = false;
+    if (std::getenv("CFDG_NO_SYM") != nullptr) {
+        return;
+    }
+    // The stencil: the distinct column offsets of all rows (at most k_max_offsets).
+    constexpr int k_max_offsets = 16;
+    std::vector<int> off;
+    bool small = true;
+#pragma omp parallel
+    {
+        std::vector<int> mine;
+        bool ok = true;
+#pragma omp for schedule(static) nowait
+        for (int i = 0; i < n_; ++i) {
+            for (int k = pt.row_ptr[i]; k < pt.row_ptr[i + 1] && ok; ++k) {
+                const int o = pt.col[k] - i;
+                if (std::find(mine.begin(), mine.end(), o) == mine.end()) {
+                    mine.push_back(o);
+                    ok = static_cast<int>(mine.size()) <= k_max_offsets;
+                }
+            }
+        }
+#pragma omp critical
+        {
+            small = small && ok;
+            for (const int o : mine) {
+                if (std::find(off.begin(), off.end(), o) == off.end()) {
+                    off.push_back(o);
+                }
+            }
+        }
+    }
+    if (!small || static_cast<int>(off.size()) > k_max_offsets) {
+        return;
+    }
+    std::sort(off.begin(), off.end());
+    std::vector<int> up; // offsets >= 0: the stored slots
+    for (const int o : off) {
+        if (!std::binary_search(off.begin(), off.end(), -o)) {
+            return; // not structurally symmetric
+        }
+        if (o >= 0) {
+            up.push_back(o);
+        }
+    }
+    const int n_off = static_cast<int>(off.size());
+    const int w = static_cast<int>(up.size());
+    std::vector<int> tab(static_cast<std::size_t>(2 * n_off));
+    for (int k = 0; k < n_off; ++k) {
+        tab[static_cast<std::size_t>(k)] = off[static_cast<std::size_t>(k)];
+        const int a = off[static_cast<std::size_t>(k)] < 0 ? -off[static_cast<std::size_t>(k)] : off[static_cast<std::size_t>(k)];
+        tab[static_cast<std::size_t>(n_off + k)] = static_cast<int>(std::lower_bound(up.begin(), up.end(), a) - up.begin());
+    }
+    const std::size_t per_slice = static_cast<std::size_t>(w) * k_wave;
+    const std::size_t slots = static_cast<std::size_t>(ceil_div(n_, k_wave)) * per_slice;
+    std::vector<int> src(slots, -1);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n_; ++i) {
+        for (int k = pt.row_ptr[i]; k < pt.row_ptr[i + 1]; ++k) {
+            const int o = pt.col[k] - i;
+            if (o >= 0) {
+                const std::size_t j = static_cast<std::size_t>(std::lower_bound(up.begin(), up.end(), o) - up.begin());
+                src[static_cast<std::size_t>(i / k_wave) * per_slice + j * k_wave + static_cast<std::size_t>(i % k_wave)] =
+                    csr_src[static_cast<std::size_t>(k)];
+            }
+        }
+    }
+    hipStream_t s = stream_.get();
+    sym_src_.upload(src.data(), src.size(), s);
+    sym_tab_.upload(tab.data(), tab.size(), s);
+    sym_v64_.resize(slots);
+    sym_v32_.resize(slots);
+    sym_flag_.resize(1);
+    hip_check(hipStreamSynchronize(s), "symmetric layout sync");
+    sym_w_ = w;
+    sym_n_ = n_off;
+}
+
+EllView System::a0_view(bool fp32) const {
+    EllView e = fp32 ? a0_.view32() : a0_.view64();
+    if (sym_active_) {
+        e.sym_w = sym_w_;
+        e.sym_n = sym_n_;
+        e.soff = sym_tab_.get();
+        e.val = fp32 ? static_cast<const void*>(sym_v32_.get()) : static_cast<const void*>(sym_v64_.get());
+    }
+    return e;
 }
 
 System::System(const cfdg_csr& a) {
@@ -623,7 +716,7 @@ void System::init_vectors() {
 
 void System::residual_norms(double out[3]) {
     require(assembled_, "normalised residual: the matrix has not been assembled");
-    const EllView a = a0_.view64();
+    const EllView a = a0_view(false);
     require(a.tpr == 1 && a.csr == 0, "normalised residual: unexpected level-0 layout");
     hipStream_t s = stream_.get();
     const int nblk = ceil_div(n_, k_block);
@@ -685,10 +778,31 @@ void System::finish_assembly() {
     k_ldu_gather<<<ceil_div(slots, k_block), k_block, 0, s>>>(slots, a0_src_.get(), ldu,
                                                                a0_.v64.get(), a0_.v32.get());
     hip_check(hipGetLastError(), "k_ldu_gather launch");
+    if (sym_w_ > 0) { // both layouts get the values; the symmetry check picks the one kernels read
+        const long long ss = static_cast<long long>(sym_src_.size());
+        k_ldu_gather<<<ceil_div(ss, k_block), k_block, 0, s>>>(ss, sym_src_.get(), ldu, sym_v64_.get(),
+                                                                sym_v32_.get());
+        hip_check(hipGetLastError(), "k_ldu_gather launch");
+        hip_check(hipMemsetAsync(sym_flag_.get(), 0, sizeof(int), s), "symmetry flag clear");
+        if (nf_ > 0) {
+            k_ldu_asymmetric<<<ceil_div(nf_, k_block), k_block, 0, s>>>(nf_, ldu + n_, ldu + n_ + nf_,
+                                                                        sym_flag_.get());
+            hip_check(hipGetLastError(), "k_ldu_asymmetric launch");
+        }
+    }
     k_diag_inverse<<<ceil_div(n_, k_block), k_block, 0, s>>>(n_, ldu, weight_, weight0_32_, dinv0_.get(),
                                                              wd0_64_.get(), wd0_32_.get());
     hip_check(hipGetLastError(), "k_diag_inverse launch");
+    int asym = 1;
+    if (sym_w_ > 0) {
+        hip_check(hipMemcpyAsync(&asym, sym_flag_.get(), sizeof(int), hipMemcpyDeviceToHost, s), "D2H symmetry flag");
+    }
     hip_check(hipStreamSynchronize(s), "assemble sync");
+    const bool sym = sym_w_ > 0 && asym == 0;
+    if (sym != sym_active_) {
+        sym_active_ = sym;
+        ++generation_; // cached descriptors hold the level-0 view
+    }
     assembled_ = true;
     bf16_ready_ = false;
 }
@@ -753,6 +867,7 @@ std::size_t estimate_footprint(const cfdg_ldu_pattern& pt, const cfdg_amg_view* 
     // 15 FP64 vectors, 3 FP32 vectors and the BF16 scales.
     std::size_t b = layout_slots(pt.n, pt.row_ptr) * (4 + 2 + 8 + 4 + 2 + 4);
     b += n * (15 * 8 + 3 * 4 + 2 * 4) + (n + 2 * static_cast<std::size_t>(pt.n_faces)) * 8;
+    b += n * 5 * (4 + 8 + 4); // symmetric stencil storage (box stencils store 3 to 5 slots per row)
     b += static_cast<std::size_t>(k_slots) * nblk0 * 8 + sizeof(SolveState) + 64 * 1024;
     if (amg != nullptr && amg->levels != nullptr) {
         for (int l = 0; l < amg->n_levels; ++l) {
@@ -1138,7 +1253,7 @@ SysDev System::descriptor(bool fp32, bool use_amg, bool bf16) {
     d.partials = partials_.get();
     d.nblk0 = ceil_div(n_, k_block);
     d.n = n_;
-    d.a64 = a0_.view64();
+    d.a64 = a0_view(false);
     d.b = b_.get();
     d.x = x_.get();
     d.r = r_.get();
@@ -1170,7 +1285,7 @@ SysDev System::descriptor(bool fp32, bool use_amg, bool bf16) {
         o.n_next = s.n_next;
         o.coarsest = coarsest ? 1 : 0;
         if (l == 0) {
-            o.a = bf16 ? a0_.viewbf() : (fp32 ? a0_.view32() : a0_.view64());
+            o.a = bf16 ? a0_.viewbf() : a0_view(fp32);
             o.b = fp32 ? static_cast<void*>(r32_.get()) : static_cast<void*>(r_.get());
             o.wd = fp32 ? static_cast<const void*>(wd0_32_.get()) : static_cast<const void*>(wd0_64_.get());
         } else {
@@ -1196,7 +1311,8 @@ SysDev System::descriptor(bool fp32, bool use_amg, bool bf16) {
 }
 
 std::size_t System::footprint() const {
-    std::size_t b = a0_.bytes() + a0_src_.bytes() + ldu_.bytes() + dinv0_.bytes() + wd0_64_.bytes() +
+    std::size_t b = a0_.bytes() + a0_src_.bytes() + sym_src_.bytes() + sym_tab_.bytes() + sym_v64_.bytes() +
+                    sym_v32_.bytes() + sym_flag_.bytes() + ldu_.bytes() + dinv0_.bytes() + wd0_64_.bytes() +
                     wd0_32_.bytes() + r32_.bytes() + partials_.bytes() + state_.bytes();
     for (const DevBuf<double>* v : {&b_, &x_, &r_, &p_, &p2_, &q_, &z_, &r0_, &v_, &s_, &t_, &ph_, &sh_}) {
         b += v->bytes();

@@ -142,6 +142,11 @@ private:
     void init_vectors();
     void finish_assembly(); // slot values and inverse diagonal from ldu_, then synchronise
     void ensure_level_vectors(bool fp32);
+    // Symmetric stencil storage of the level-0 operator (EllView::sym_w), prepared with an LDU
+    // pattern whose offsets form one small stencil, used while the assembled values are exactly
+    // symmetric (upper == lower); the full per-slice layout otherwise (and for BF16).
+    void build_symmetric_layout(const cfdg_ldu_pattern& pt, const std::vector<int>& csr_src);
+    [[nodiscard]] EllView a0_view(bool fp32) const;
 
     int n_ = 0;
     int nf_ = 0;
@@ -149,6 +154,14 @@ private:
     Stream stream_;
     DevEll a0_;
     DevBuf<int> a0_src_;      // ELL slot -> index into [diag | upper | lower], -1 padding
+    DevBuf<int> sym_src_;     // symmetric storage slot -> LDU index, -1 absent (empty: no such layout)
+    DevBuf<int> sym_tab_;     // stencil offsets ascending, then the stored slot of each
+    DevBuf<double> sym_v64_;
+    DevBuf<float> sym_v32_;
+    DevBuf<int> sym_flag_;    // set by the symmetry check of an assembly
+    int sym_w_ = 0;
+    int sym_n_ = 0;
+    bool sym_active_ = false; // the last assembly was symmetric: kernels read the symmetric storage
     DevBuf<double> ldu_;      // staging of LDU coefficients
     DevBuf<double> dinv0_;
     DevBuf<double> wd0_64_;

@@ -18,6 +18,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _solve_pattern(monkeypatch, stencil, s, pat, view, method, prec, use_amg):
+    monkeypatch.setenv("CFDG_NO_SYM", "1")  # the per-slice stencil itself (symmetric storage: test_gpu_symmetric)
     if stencil:
         monkeypatch.delenv("CFDG_NO_STENCIL", raising=False)
     else:
