@@ -281,14 +281,27 @@ def batched(steps: int, count: int = 256) -> dict:
     }
 
 
-def roofline(probe, kernel, nnz, traffic=None, working_set=None):
+def stored_values(a) -> int:
+    """Value slots the library stores for the level-0 operator of CSR matrix a when it is
+    exactly symmetric (the pressure systems): the symmetric stencil storage keeps the entries
+    with offset >= 0 only, one slot per stencil offset >= 0 for every row padded to 64; a matrix
+    whose offsets form no small symmetric stencil keeps its nnz values."""
+    rows = np.repeat(np.arange(a.n, dtype=np.int64), np.diff(a.row_ptr))
+    off = np.unique(a.col.astype(np.int64) - rows)
+    if len(off) > 16 or not np.array_equal(np.sort(-off), off):
+        return int(a.nnz)
+    return int(-(-a.n // 64) * 64 * int(np.count_nonzero(off >= 0)))
+
+
+def roofline(probe, kernel, nnz, traffic=None, working_set=None, values=None, vbytes=8):
     """Roofline entry of one probed kernel: algorithmic bytes per launch under the 4-byte
     column-index convention (every stored entry counted with a 4-byte index, vectors once) over
     the HIP-event average duration, and the same on the bytes the layout stores (the level-0
-    stencil layout keeps one offset table per 64-row slice and no per-entry index)."""
+    stencil layout keeps one offset table per 64-row slice and no per-entry index; with
+    symmetric storage `values` value slots of vbytes each instead of nnz)."""
     ms, launches, nbytes = probe
     gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    stored = nbytes - 4.0 * nnz
+    stored = nbytes - 4.0 * nnz - (0 if values is None else vbytes * (nnz - values))
     gbs_stored = stored / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     out = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
@@ -352,10 +365,11 @@ def stress(n_side: int = 160, reps: int = 5) -> dict:
             dev.solve_resident(Method.Cg, True, Precision.MIXED_FP32, ctl)
         probes[kind] = dev.probe_read(Precision.MIXED_FP32)
     dev.probe(0)
+    sv = stored_values(a)
     out["roofline_outer_spmv"] = roofline(probes[2], K_SPMV_MIXED, a.nnz, traffic.get(f"{K_SPMV_MIXED}@{n_side}"),
-                                          dev.footprint())
+                                          dev.footprint(), sv, 8)
     out["roofline_l0_fp32_sweep"] = roofline(probes[1], K_SWEEP_MIXED, a.nnz,
-                                             traffic.get(f"{K_SWEEP_MIXED}@{n_side}"), dev.footprint())
+                                             traffic.get(f"{K_SWEEP_MIXED}@{n_side}"), dev.footprint(), sv, 4)
     out["_solutions"] = xs  # for the CPU reference comparison; removed before printing
     out["_hierarchy"] = (h.view(), p.source, 1e-10 * bnorm)
     out["workload"] = (f"{n_side}^3 jittered box ({p.n} cells), rho 1000 in a sphere of radius "
@@ -680,6 +694,7 @@ def main(argv=None) -> int:
     mesh, p_sys, m_sys = cases.config2(args.size)
     pat = cases.csr_pattern(p_sys.n, p_sys.owner, p_sys.neighbour)
     a = cases.ldu_to_csr(pat, p_sys.diag, p_sys.upper, p_sys.lower)
+    sv0 = stored_values(a)  # level-0 value slots of the symmetric stencil storage
     log(f"generated {p_sys.n} cells / {p_sys.n_faces} faces in {time.perf_counter() - t0:.1f} s")
     hier = AmgHierarchy(a, AmgSettings(strength_decay=0.5))
     # the same setup with its sparse products on the GPU (second build: the first call in a
@@ -887,16 +902,17 @@ def main(argv=None) -> int:
         "roofline": roofline(probes[(Precision.MIXED_FP32, 2)],
                              f"{K_SPMV_MIXED}: outer FP64 q = A p_new, p_new = z + beta p_old (z FP32), p.q; "
                              "the mixed solve's longest launch per iteration", a.nnz,
-                             traffic.get(f"{K_SPMV_MIXED}@{side}")),
+                             traffic.get(f"{K_SPMV_MIXED}@{side}"), None, sv0, 8),
         "roofline_l0_sweep_mixed": roofline(probes[(Precision.MIXED_FP32, 1)],
                                             f"{K_SWEEP_MIXED}: last level-0 FP32 Jacobi sweep + r.z, z.q",
-                                            a.nnz, traffic.get(f"{K_SWEEP_MIXED}@{side}")),
+                                            a.nnz, traffic.get(f"{K_SWEEP_MIXED}@{side}"), None, sv0, 4),
         "roofline_fp64_spmv": roofline(probes[(Precision.FP64, 2)],
                                        f"{K_SPMV_FP64}: FP64 q = A p_new, p_new = z + beta p_old, p.q",
-                                       a.nnz, traffic.get(f"{K_SPMV_FP64}@{side}")),
+                                       a.nnz, traffic.get(f"{K_SPMV_FP64}@{side}"), None, sv0, 8),
         "roofline_fp64_sweep": roofline(probes[(Precision.FP64, 1)],
                                         f"{K_SWEEP_FP64}: last level-0 FP64 Jacobi sweep + r.z", a.nnz,
-                                        traffic.get(f"{K_SWEEP_FP64}@{side}")),
+                                        traffic.get(f"{K_SWEEP_FP64}@{side}"), None, sv0, 8),
+        "stored_value_slots_level0": sv0,
         "working_set_mb": round(dev.footprint() / 1e6, 1),
         "roofline_note": (f"at {side}^3 the level-0 FP64 values ({8 * a.nnz / 1e6:.0f} MB) fit the 256 MB "
                           "Infinity Cache, so these rates are not HBM evidence; precision_stress reports "
