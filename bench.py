@@ -914,9 +914,12 @@ def main(argv=None) -> int:
                                         traffic.get(f"{K_SWEEP_FP64}@{side}"), None, sv0, 8),
         "stored_value_slots_level0": sv0,
         "working_set_mb": round(dev.footprint() / 1e6, 1),
-        "roofline_note": (f"at {side}^3 the level-0 FP64 values ({8 * a.nnz / 1e6:.0f} MB) fit the 256 MB "
-                          "Infinity Cache, so these rates are not HBM evidence; precision_stress reports "
-                          "the same kernels at 160^3, beyond it"),
+        "roofline_note": (f"at {side}^3 the level-0 FP64 values fit the 256 MB Infinity Cache, so these "
+                          "rates are not HBM evidence; precision_stress reports the same kernels at 160^3, "
+                          "beyond it. frac follows the 4-byte-index convention (every entry counted with "
+                          "its value and a 4-byte index); the library stores the symmetric operator's "
+                          f"{sv0} value slots ({sv0 / a.nnz:.2f} per entry) and no indices, so frac can "
+                          "exceed 1; frac_stored is on the stored bytes"),
         "cpu_baseline": None,
     }
     if not args.skip_batch:
