@@ -243,84 +243,131 @@ __device__ __forceinline__ T csr_lane_sum(const gmem_t<const int>* __restrict__ 
 #endif
 constexpr int k_sym_group = CFDG_SYM_GROUP; // entries whose loads are issued together
 
-// Row sum of a row in symmetric stencil storage (EllView::sym_w > 0), entries in offset order:
-// the value of offset o >= 0 from the row's own stored slot, of o < 0 from row + o's stored slot
-// of -o (zero where row + o < 0); the column row + o clamped to the matrix, as in the per-slice
-// stencil layout (whose missing offsets hold zeros). Loads are issued k_sym_group entries at a
-// time.
+// One entry of a row in symmetric stencil storage: offset o, stored slot j of |o|. The value of
+// o >= 0 is in the row's own slot, of o < 0 in row + o's slot of -o (zero where row + o < 0); the
+// column is row + o clamped to the matrix, as in the per-slice stencil layout (whose missing
+// offsets hold zeros).
 template <class T, class F>
-__device__ __forceinline__ T ell_row_sym(const EllView& m, int slice, int lane, F load) {
-    const int row = slice * k_wave + lane;
-    const int n = m.sym_n;
-    const int stride = m.sym_w * k_wave; // values per slice
-    const auto tab = gm(m.soff);         // offsets [0, n), stored slots [n, 2 n)
+__device__ __forceinline__ void sym_entry(const gmem_t<const T>* __restrict__ val, int stride, int last, int row, int o,
+                                          int j, F load, T& a, T& x) {
+    const int r = o < 0 ? row + o : row;
+    const int rr = max(r, 0);
+    const T v = val[(rr >> 6) * stride + j * k_wave + (rr & (k_wave - 1))];
+    a = r < 0 ? T(0) : v;
+    x = load(min(max(row + o, 0), last));
+}
+
+// Row sums (rows row0, and row0 + 64 when Rows == 2) of a stencil of N offsets known at compile
+// time: the table is read with constant indices (one batch of scalar loads, one wait) and every
+// entry's loads are issued before the sums, which run in offset order.
+template <int N, int Rows, class T, class F>
+__device__ __forceinline__ void sym_rows_fixed(const EllView& m, int row0, F load, T (&y)[2]) {
+    const auto tab = gm(m.soff); // offsets [0, N), stored slots [N, 2 N)
     const auto val = gm(static_cast<const T*>(m.val));
+    const int stride = m.sym_w * k_wave;
     const int last = m.n_rows - 1;
-    auto entry = [&](int k, T& a, T& x) {
-        const int o = tab[k];
-        const int j = tab[n + k];
-        const int r = o < 0 ? row + o : row; // the row whose stored slot holds a(row, row + o)
-        const int rr = max(r, 0);
-        const T v = val[(rr >> 6) * stride + j * k_wave + (rr & (k_wave - 1))];
-        a = r < 0 ? T(0) : v;
-        x = load(min(max(row + o, 0), last));
-    };
-    T s = T(0);
+    int o[N];
+    int j[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        o[k] = tab[k];
+        j[k] = tab[N + k];
+    }
+    T a[Rows][N];
+    T x[Rows][N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+#pragma unroll
+        for (int h = 0; h < Rows; ++h) {
+            sym_entry<T>(val, stride, last, row0 + h * k_wave, o[k], j[k], load, a[h][k], x[h][k]);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < Rows; ++h) {
+        T s = T(0);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            s = madd(a[h][k], x[h][k], s);
+        }
+        y[h] = s;
+    }
+}
+
+// Any stencil size: the table read per entry, loads issued k_sym_group entries at a time.
+template <int Rows, class T, class F>
+__device__ __forceinline__ void sym_rows_any(const EllView& m, int row0, F load, T (&y)[2]) {
+    const int n = m.sym_n;
+    const auto tab = gm(m.soff);
+    const auto val = gm(static_cast<const T*>(m.val));
+    const int stride = m.sym_w * k_wave;
+    const int last = m.n_rows - 1;
+    T s[Rows];
+#pragma unroll
+    for (int h = 0; h < Rows; ++h) {
+        s[h] = T(0);
+    }
     for (int k = 0; k < n; k += k_sym_group) {
-        T a[k_sym_group];
-        T x[k_sym_group];
+        T a[Rows][k_sym_group];
+        T x[Rows][k_sym_group];
 #pragma unroll
         for (int g = 0; g < k_sym_group; ++g) {
-            entry(min(k + g, n - 1), a[g], x[g]); // past the stencil: repeats its last entry, not added
+            const int kk = min(k + g, n - 1); // past the stencil: repeats its last entry, not added
+            const int o = tab[kk];
+            const int j = tab[n + kk];
+#pragma unroll
+            for (int h = 0; h < Rows; ++h) {
+                sym_entry<T>(val, stride, last, row0 + h * k_wave, o, j, load, a[h][g], x[h][g]);
+            }
         }
 #pragma unroll
         for (int g = 0; g < k_sym_group; ++g) {
             if (k + g < n) {
-                s = madd(a[g], x[g], s);
+#pragma unroll
+                for (int h = 0; h < Rows; ++h) {
+                    s[h] = madd(a[h][g], x[h][g], s[h]);
+                }
             }
         }
     }
-    return s;
+#pragma unroll
+    for (int h = 0; h < Rows; ++h) {
+        y[h] = s[h];
+    }
+}
+
+template <int Rows, class T, class F>
+__device__ __forceinline__ void sym_rows(const EllView& m, int row0, F load, T (&y)[2]) {
+    switch (m.sym_n) { // uniform: the stencils of 3-D, 2-D and 1-D box meshes unrolled
+    case 7:
+        sym_rows_fixed<7, Rows, T>(m, row0, load, y);
+        return;
+    case 5:
+        sym_rows_fixed<5, Rows, T>(m, row0, load, y);
+        return;
+    case 3:
+        sym_rows_fixed<3, Rows, T>(m, row0, load, y);
+        return;
+    default:
+        sym_rows_any<Rows, T>(m, row0, load, y);
+    }
+}
+
+// Row sum of a row in symmetric stencil storage (EllView::sym_w > 0), entries in offset order.
+template <class T, class F>
+__device__ __forceinline__ T ell_row_sym(const EllView& m, int slice, int lane, F load) {
+    T y[2];
+    sym_rows<1, T>(m, slice * k_wave + lane, load, y);
+    return y[0];
 }
 
 // ell_row_sym for the rows of slices s0 and s0 + 1 (lane's row in each), their loads issued
 // together: the same sums as two ell_row_sym calls.
 template <class T, class F>
 __device__ __forceinline__ void ell_row_sym_pair(const EllView& m, int s0, int lane, F load, T& ya, T& yb) {
-    const int row0 = s0 * k_wave + lane;
-    const int n = m.sym_n;
-    const int stride = m.sym_w * k_wave;
-    const auto tab = gm(m.soff);
-    const auto val = gm(static_cast<const T*>(m.val));
-    const int last = m.n_rows - 1;
-    auto entry = [&](int row, int o, int j, T& a, T& x) {
-        const int r = o < 0 ? row + o : row;
-        const int rr = max(r, 0);
-        const T v = val[(rr >> 6) * stride + j * k_wave + (rr & (k_wave - 1))];
-        a = r < 0 ? T(0) : v;
-        x = load(min(max(row + o, 0), last));
-    };
-    ya = T(0);
-    yb = T(0);
-    for (int k = 0; k < n; k += k_sym_group) {
-        T a[2][k_sym_group];
-        T x[2][k_sym_group];
-#pragma unroll
-        for (int g = 0; g < k_sym_group; ++g) {
-            const int kk = min(k + g, n - 1);
-            const int o = tab[kk];
-            const int j = tab[n + kk];
-            entry(row0, o, j, a[0][g], x[0][g]);
-            entry(row0 + k_wave, o, j, a[1][g], x[1][g]);
-        }
-#pragma unroll
-        for (int g = 0; g < k_sym_group; ++g) {
-            if (k + g < n) {
-                ya = madd(a[0][g], x[0][g], ya);
-                yb = madd(a[1][g], x[1][g], yb);
-            }
-        }
-    }
+    T y[2];
+    sym_rows<2, T>(m, s0 * k_wave + lane, load, y);
+    ya = y[0];
+    yb = y[1];
 }
 
 // Row sum of an ELL row with the x entry of column c produced by load(c), in storage order.
