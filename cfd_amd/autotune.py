@@ -23,6 +23,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import os
 import time
 from pathlib import Path
 
@@ -156,7 +157,8 @@ def apply(table: dict) -> None:
                               float(r["max_mean"]), int(r["lanes"]), int(r["csr"]))
     L.call("cfdg_set_layout_rules", arr, len(rules))
     sm = table.get("mixed_smoother")
-    if sm:
+    # the environment's smoother settings, where given, take precedence over the table's
+    if sm and not (os.environ.get("CFDG_MIXED_SWEEPS") or os.environ.get("CFDG_MIXED_RATIO")):
         L.call("cfdg_set_mixed_smoother", int(sm["sweeps"]), float(sm["ratio"]))
 
 
