@@ -286,6 +286,7 @@ _SIGNATURES = {
                                C.POINTER(VP)]),
     "cfdg_batch_destroy": (ST, [VP]),
     "cfdg_batch_solve": (ST, [VP, C.POINTER(SolveControl), C.POINTER(SolveStats)]),
+    "cfdg_batch_zero_x": (ST, [VP]),
     "cfdg_set_layout_rules": (ST, [C.POINTER(LayoutRule), C.c_int]),
     "cfdg_layout_of": (ST, [C.c_int, P_INT, P_INT, P_INT]),
     "cfdg_set_mixed_smoother": (ST, [C.c_int, C.c_double]),

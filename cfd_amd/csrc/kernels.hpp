@@ -838,6 +838,14 @@ __global__ void __launch_bounds__(k_block) k_spmv_pz(const SysDev* __restrict__ 
     }
 }
 
+// x = 0 for every system of a launch table (one launch for a whole batch).
+__global__ void __launch_bounds__(k_block) k_zero_x(const SysDev* __restrict__ sys, const int2* __restrict__ blocks) {
+    CFDG_BLOCK_PROLOGUE(sys, blocks)
+    if (row < S.n) {
+        gm(S.x)[row] = 0.0;
+    }
+}
+
 template <int Mode> // 0 AMG FP64, 1 Jacobi CG, 2 mixed AMG
 __global__ void __launch_bounds__(k_block) k_update_xr(const SysDev* __restrict__ sys, const int2* __restrict__ blocks, int parity) {
     CFDG_BLOCK_PROLOGUE(sys, blocks)

@@ -816,6 +816,7 @@ void System::set_x(const double* x) {
     if (x == nullptr) { // x = 0, enqueued on the system's stream ahead of the next solve
         k_fill<<<ceil_div(n_, k_block), k_block, 0, stream_.get()>>>(n_, x_.get(), 0.0);
         hip_check(hipGetLastError(), "k_fill launch");
+        pending_ = true;
         return;
     }
     x_.upload(x, static_cast<std::size_t>(n_), stream_.get());
@@ -2041,6 +2042,13 @@ private:
         }
     }
 
+public:
+    // x = 0 for every system, one launch on the runner's stream (ordered before its next solve).
+    void zero_x() {
+        CFDG_LAUNCH(k_zero_x, tb_.tn[0], s_, desc_.get(), tb_.t[0].get());
+    }
+
+private:
     std::vector<System*> sys_;
     std::vector<unsigned> gens_;
     std::vector<unsigned long long> uids_;
@@ -2660,6 +2668,30 @@ extern "C" cfdg_status cfdg_batch_destroy(cfdg_batch* batch) {
     return guarded([&] { delete batch; });
 }
 
+// Fills queued on the members' streams are completed (the batch runs on its own stream) and the
+// runner is (re)built when the members or their hierarchies changed.
+static void batch_prepare(cfdg_batch& batch) {
+    for (System* s : batch.systems) {
+        if (batch.opt.use_amg && batch.opt.precision == CFDG_PREC_MIXED_BF16) {
+            s->ensure_bf16();
+        }
+        s->finish_pending();
+    }
+    if (!batch.runner || !batch.runner->matches(batch.systems, batch.opt)) {
+        batch.runner.reset();
+        batch.runner = std::make_unique<Runner>(batch.systems, batch.opt, batch.stream.get());
+    }
+}
+
+extern "C" cfdg_status cfdg_batch_zero_x(cfdg_batch* batch) {
+    return guarded([&] {
+        require(batch != nullptr, "null argument");
+        batch_prepare(*batch);
+        batch->runner->zero_x();
+        hip_check(hipGetLastError(), "batch zero x");
+    });
+}
+
 extern "C" cfdg_status cfdg_batch_solve(cfdg_batch* batch, const cfdg_solve_control* controls,
                                         cfdg_solve_stats* stats) {
     return guarded([&] {
@@ -2669,16 +2701,7 @@ extern "C" cfdg_status cfdg_batch_solve(cfdg_batch* batch, const cfdg_solve_cont
         for (const cfdg_solve_control& x : c) {
             check_control(&x);
         }
-        for (System* s : batch->systems) { // uploads and fills queued on the systems' streams
-            if (batch->opt.use_amg && batch->opt.precision == CFDG_PREC_MIXED_BF16) {
-                s->ensure_bf16();
-            }
-            hip_check(hipStreamSynchronize(s->stream()), "batch: system stream sync");
-        }
-        if (!batch->runner || !batch->runner->matches(batch->systems, batch->opt)) {
-            batch->runner.reset();
-            batch->runner = std::make_unique<Runner>(batch->systems, batch->opt, batch->stream.get());
-        }
+        batch_prepare(*batch);
         std::vector<cfdg_solve_stats> st;
         batch->runner->solve(c, st, nullptr);
         std::copy(st.begin(), st.end(), stats);

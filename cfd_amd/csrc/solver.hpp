@@ -133,6 +133,15 @@ public:
         }
     }
 
+    // Completes work a call left queued on the system's stream (set_x(nullptr)'s fill), so that
+    // work on another stream may follow; a no-op when nothing is pending.
+    void finish_pending() {
+        if (pending_) {
+            hip_check(hipStreamSynchronize(stream_.get()), "system stream sync");
+            pending_ = false;
+        }
+    }
+
     // Generation counter, bumped when the hierarchy or vector set changes (descriptor caches).
     [[nodiscard]] unsigned generation() const { return generation_; }
     // Process-wide unique identity of this system (addresses of destroyed systems are reused).
@@ -162,6 +171,7 @@ private:
     int sym_w_ = 0;
     int sym_n_ = 0;
     bool sym_active_ = false; // the last assembly was symmetric: kernels read the symmetric storage
+    bool pending_ = false;    // set_x(nullptr)'s fill is queued on stream_ and not yet waited for
     DevBuf<double> ldu_;      // staging of LDU coefficients
     DevBuf<double> dinv0_;
     DevBuf<double> wd0_64_;

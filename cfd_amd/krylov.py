@@ -387,6 +387,11 @@ class Batch:
     def __exit__(self, *exc):
         self.close()
 
+    def zero_x(self) -> None:
+        """Sets every member's initial guess to zero with one launch (the batched form of
+        DeviceSolver.set_x(None))."""
+        L.call("cfdg_batch_zero_x", self._h)
+
     def solve(self, controls: Sequence[SolveControl]) -> List[SolveStats]:
         """Solves every member from its resident right-hand side and iterate."""
         n = len(self._systems)

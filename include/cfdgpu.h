@@ -244,6 +244,9 @@ cfdg_status cfdg_batch_destroy(cfdg_batch* batch);
 /* controls and stats hold one entry per system, in creation order. */
 cfdg_status cfdg_batch_solve(cfdg_batch* batch, const cfdg_solve_control* controls,
                              cfdg_solve_stats* stats);
+/* Sets every member's initial guess to zero with one launch on the batch's stream (what
+ * cfdg_system_set_x(member, NULL) does member by member, without a launch per member). */
+cfdg_status cfdg_batch_zero_x(cfdg_batch* batch);
 
 /* ------------------------------------------------------------------------------------------
  * Serving loop (no counterpart in the reference): a request is one mesh's system with a run of

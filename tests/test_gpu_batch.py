@@ -85,3 +85,29 @@ def test_batch_rejects_mismatched_controls():
     with cfd_amd.Batch([d0, d1]) as batch:
         with pytest.raises(InvalidArgument):
             batch.solve([SolveControl()])
+
+
+@pytest.mark.parametrize("prec", [Precision.FP64, Precision.MIXED_FP32])
+def test_batch_zero_x_matches_per_member_set_x(prec):
+    """Batch.zero_x (one launch for every member's initial guess) gives the solves that
+    DeviceSolver.set_x(None) member by member gives, bit for bit, on repeated solves; members
+    whose x was last set from the host are zeroed too."""
+    members = [_member(ns, 40 + ns) for ns in (9, 14, 6)]
+    devs = [d for d, _ in members]
+    ctls = [SolveControl(1e-9 * np.linalg.norm(s.source), 300) for _, s in members]
+    try:
+        with cfd_amd.Batch(devs, Method.Cg, True, prec) as batch:
+            for d in devs:
+                d.set_x(None)
+            ref = [(st.iterations, st.residual, d.get_x()) for st, d in zip(batch.solve(ctls), devs)]
+            for rep in range(2):
+                if rep == 1:
+                    devs[1].set_x(np.ones(devs[1].n))  # a host-set guess, then zeroed by the batch
+                batch.zero_x()
+                got = [(st.iterations, st.residual, d.get_x()) for st, d in zip(batch.solve(ctls), devs)]
+                for a, b in zip(ref, got):
+                    assert a[0] == b[0] and a[1] == b[1]
+                    assert np.array_equal(a[2], b[2])
+    finally:
+        for d in devs:
+            d.close()
